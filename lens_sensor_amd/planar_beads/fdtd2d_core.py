@@ -1,0 +1,868 @@
+"""2D Yee FDTD solver (planar waveguide + bead at the waveguide/tear interface) on the
+MI355X step engine.
+
+Entry points keep the reference's signatures (planar_beads/fdtd2d_core.py):
+  run_beads           :531-927   full domain (method='full'), dispatch to stitch
+  run_beads_stitched  :1197-1583 domain decomposition with phasor handoff
+Materials and coefficients (host NumPy, as in the reference):
+  build_materials :122-259, _coeffs :262-270, _tm_loss :283-294, _apply_bead_tm :297-349,
+  _tm_coeffs :445-455, node colocation :458-486
+TE = polarisation 's' (Ez, Hx, Hy); TM = 'p' (Hz, Ex, Ey).
+
+The time loop is executed by the step engine (HIP library); per step the host only
+fills table entries (envelope, phasors) with the reference's expressions.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+_REPO_ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+if _REPO_ROOT not in sys.path:
+    sys.path.insert(0, _REPO_ROOT)
+
+from lens_sensor_amd.common import backend                                    # noqa: E402
+from lens_sensor_amd.common.physics import (C0, EPS0, MU0, N_AIR, n_at,       # noqa: E402
+                                            cornea_default, loss_sigma, core_source_mask)
+from lens_sensor_amd.common.subpixel import circle_fraction, blend, cell_average_column  # noqa: E402
+from lens_sensor_amd.common.flux import PlaneFluxBook, FluxAccumulator, guard_in_plane  # noqa: E402
+from lens_sensor_amd.common import vcsel                                      # noqa: E402
+from lens_sensor_amd.common.layer_stack import (stack_total_um, legacy_stack,  # noqa: E402
+                                                legacy_layers_tuple, describe as describe_stack)
+from lens_sensor_amd.engine.spec import new_steps, fill_phasors, cw_envelope, pulse_envelope  # noqa: E402
+from lens_sensor_amd.planar_beads.boundaries import make_boundary            # noqa: E402
+
+BEAD_SUBPIXEL = True
+N_POLYSTYR, N_PMMA, N_AQ, N_MUCIN, N_CORNEA, N_LIPID = 1.590, 1.491, 1.336, 1.342, 1.376, 1.480
+L_DEMO = 1.0e-3
+AIR_BUFFER = 5.0e-6
+TEAR_BUFFER = 10.0e-6
+KAPPA = {}
+SRC_CORE_ONLY = False
+LASER_POWER_MW = vcsel.VCSEL_DEFAULTS['power_mw']
+LAM_LOSS = 850e-9
+T_MU = 0.5e-6
+T_AQ = 3.5e-6
+BEAD_X_UM = 500.0
+DET_LEN = 30e-6
+N_G_FACTOR = 1.10
+CW_TOL = 0.01
+CW_MAX_EXTEND = 2.0
+NPML = 10
+
+
+def _bead_list(place_bead, bead_x_um, bead_d_um, bead_y_um, bead_n, beads):
+    if beads is not None:
+        return [b for b in beads if b.get('d_um', 0) > 0]
+    if not place_bead or not bead_d_um or bead_d_um <= 0:
+        return []
+    return [dict(x_um=bead_x_um, d_um=bead_d_um, y_um=bead_y_um, n=bead_n)]
+
+
+# ----------------------------------------------------------------------------- materials
+def build_materials(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_start_um=0.0,
+                    place_bead=True, n_aq=N_AQ, n_mucin=N_MUCIN, n_cornea=N_CORNEA, t_aq_um=None,
+                    t_mu_um=None, t_lip_um=0.0, n_lip=N_LIPID, input_gap_um=0.0, length_um=None,
+                    bead_y_um=None, curved=False, r_bend_m=8.3e-3, tm_patch=None, stack=None,
+                    beads=None):
+    """eps_r (Nx,Ny) float32 and sigma (Nx,Ny) float32 of the planar stack with the core on
+    [input_gap, L] x [0, t_wg] and disc beads blended by area fraction.
+    Reference fdtd2d_core.py:122-259.  ``beads`` (list of {x_um, d_um, y_um, n}) replaces
+    the single bead for several beads per window."""
+    t_aq = (t_aq_um * 1e-6) if t_aq_um is not None else T_AQ
+    t_mu = (t_mu_um * 1e-6) if t_mu_um is not None else T_MU
+    t_lip = (t_lip_um or 0.0) * 1e-6
+    L = (length_um * 1e-6) if length_um is not None else L_DEMO
+    x_wg_start = (input_gap_um or 0.0) * 1e-6
+    x_wg_end = L
+    kp = dict(KAPPA or {})
+    lossy = any(v > 0 for v in kp.values())
+    if stack is not None:
+        stack = list(stack)
+        lossy = lossy or any(complex(Ly['n']).imag > 0 for Ly in stack)
+    blist = _bead_list(place_bead, bead_x_um, bead_d_um, bead_y_um, bead_n, beads)
+    if lossy:
+        wg_n = complex(wg_n, kp.get('wg', 0.0))
+        n_aq = complex(n_aq, kp.get('aq', 0.0))
+        n_mucin = complex(n_mucin, kp.get('mu', 0.0))
+        n_lip = complex(n_lip, kp.get('lip', 0.0))
+        k_co = kp.get('co', 0.0)
+    else:
+        k_co = 0.0
+    edt = np.complex128 if lossy else np.float64
+    eps = np.full((Nx, Ny), N_AIR ** 2, dtype=np.complex128 if lossy else np.float32)
+    ys = -TEAR_BUFFER + np.arange(Ny) * dx
+    y_lip_bot = -t_lip
+    y_aq_bot = y_lip_bot - t_aq
+    y_mu_bot = y_aq_bot - t_mu
+    bounds = None
+    if stack is not None:
+        bounds, yt = [], 0.0
+        for Ly in stack:
+            yb = yt - float(Ly['t_um']) * 1e-6
+            nL = complex(Ly['n'])
+            bounds.append((yt, yb, nL if lossy else nL.real))
+            yt = yb
+        y_mu_bot = yt
+
+    def eps_y(y, core):
+        e = np.full(y.shape, N_AIR ** 2, dtype=edt)
+        below = y < y_mu_bot
+        if hasattr(n_cornea, 'at_depth'):
+            e[below] = ((n_cornea.at_depth((y_mu_bot - y[below]) * 1e6) + 1j * k_co) ** 2 if lossy
+                        else n_cornea.at_depth((y_mu_bot - y[below]) * 1e6) ** 2)
+        else:
+            e[below] = (float(n_cornea) + 1j * k_co) ** 2 if lossy else n_cornea ** 2
+        if bounds is not None:
+            for yt_, yb_, n_ in bounds:
+                e[(y >= yb_) & (y < yt_)] = n_ ** 2
+        else:
+            e[(y >= y_mu_bot) & (y < y_aq_bot)] = n_mucin ** 2
+            e[(y >= y_aq_bot) & (y < y_lip_bot)] = n_aq ** 2
+            if t_lip > 1e-12:
+                e[(y >= y_lip_bot) & (y < 0.0)] = n_lip ** 2
+        if core:
+            e[(y >= 0.0) & (y < wg_top)] = wg_n ** 2
+        return e
+
+    col = cell_average_column(ys, dx, lambda y: eps_y(y, False))
+    col_core = cell_average_column(ys, dx, lambda y: eps_y(y, True))
+    eps[:, :] = col[None, :]
+    xs_abs = x_start_um * 1e-6 + np.arange(Nx) * dx
+    in_wg_x = (xs_abs >= x_wg_start) & (xs_abs <= x_wg_end)
+    if in_wg_x.any():
+        eps[in_wg_x, :] = col_core[None, :]
+    if tm_patch is not None and not curved:
+        tm_patch['layer'] = dict(
+            in_wg_x=in_wg_x,
+            h_out=cell_average_column(ys[:-1], dx, lambda y: eps_y(y, False).real,
+                                      harmonic=True, shift=0.5 * dx),
+            h_core=cell_average_column(ys[:-1], dx, lambda y: eps_y(y, True).real,
+                                       harmonic=True, shift=0.5 * dx))
+    x0 = x_start_um * 1e-6
+    xs = x0 + np.arange(Nx) * dx
+    for bd in blist:
+        r = bd['d_um'] * 1e-6 / 2.0
+        xc = bd['x_um'] * 1e-6
+        bn = bd['n']
+        bn_c = complex(bn, kp.get('bead', 0.0)) if lossy else bn
+        if xc + r < xs[0] or xc - r > xs[-1]:
+            continue
+        yc = (bd['y_um'] * 1e-6) if bd.get('y_um') is not None else -r
+        ix0 = max(0, int((xc - r - x0) / dx) - 2)
+        ix1 = min(Nx, int((xc + r - x0) / dx) + 2)
+        if BEAD_SUBPIXEL:
+            jy0 = max(0, int((yc - r - ys[0]) / dx) - 2)
+            jy1 = min(Ny, int((yc + r - ys[0]) / dx) + 3)
+            if ix1 > ix0 and jy1 > jy0:
+                if tm_patch is not None and not curved:
+                    tm_patch.setdefault('beads', []).append(dict(
+                        eps_bg=eps[ix0:ix1, jy0:jy1].real.astype(np.float64), ix0=ix0, jy0=jy0,
+                        xs=xs[ix0:ix1], ys=ys[jy0:jy1], xc=xc, yc=yc, r=r, n=bn))
+                blend(eps[ix0:ix1, jy0:jy1],
+                      circle_fraction(xs[ix0:ix1], ys[jy0:jy1], xc, yc, r, dx), bn_c)
+        else:
+            for i in range(ix0, ix1):
+                ddx = xs[i] - xc
+                if abs(ddx) > r:
+                    continue
+                ddy = np.sqrt(max(0.0, r * r - ddx * ddx))
+                eps[i, (ys >= yc - ddy) & (ys <= yc + ddy)] = bn_c ** 2
+    if curved:
+        u = ys - wg_top / 2.0
+        metric = (1.0 + u / r_bend_m).astype(np.float32) ** 2
+        eps = eps * metric[None, :]
+    if lossy:
+        sig = loss_sigma(eps.imag, LAM_LOSS).astype(np.float32)
+        eps = eps.real.astype(np.float32)
+    else:
+        sig = np.zeros((Nx, Ny), dtype=np.float32)
+    return eps, sig
+
+
+def _coeffs(eps_r, sig, dt, dx):
+    """TE: Ce_E = (1-a)/(1+a), Ce_H = dt/(eps0 eps dx)/(1+a), a = sigma dt/(2 eps0 eps)
+    (fdtd2d_core.py:262-270)."""
+    Ce_n = (1 - sig * dt / (2 * EPS0 * eps_r))
+    Ce_d = (1 + sig * dt / (2 * EPS0 * eps_r))
+    Ce_E = (Ce_n / Ce_d).astype(np.float32)
+    Ce_H = (dt / (EPS0 * eps_r * dx) / Ce_d).astype(np.float32)
+    return Ce_E, Ce_H
+
+
+def _tm_coeffs(eps_r, dt, dx):
+    """TM: Ca at the staggered Ex/Ey positions from the averaged node eps
+    (fdtd2d_core.py:445-455)."""
+    eps_ex = 0.5 * (eps_r[:-1, :] + eps_r[1:, :])
+    eps_ey = 0.5 * (eps_r[:, :-1] + eps_r[:, 1:])
+    Ca_ex = (dt / (EPS0 * eps_ex * dx)).astype(np.float32)
+    Ca_ey = (dt / (EPS0 * eps_ey * dx)).astype(np.float32)
+    return Ca_ex, Ca_ey, np.float32(dt / (MU0 * dx))
+
+
+def _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt):
+    """Semi-implicit TM loss: Ca /= (1+a) in place, returns D = (1-a)/(1+a) per component
+    or (None, None) (fdtd2d_core.py:283-294)."""
+    if sig is None or not bool(np.any(sig > 0)):
+        return None, None
+    s_ex = 0.5 * (sig[:-1, :] + sig[1:, :])
+    e_ex = 0.5 * (eps_r[:-1, :] + eps_r[1:, :])
+    s_ey = 0.5 * (sig[:, :-1] + sig[:, 1:])
+    e_ey = 0.5 * (eps_r[:, :-1] + eps_r[:, 1:])
+    a_ex = s_ex * dt / (2 * EPS0 * e_ex)
+    a_ey = s_ey * dt / (2 * EPS0 * e_ey)
+    Ca_ex /= (1 + a_ex).astype(np.float32)
+    Ca_ey /= (1 + a_ey).astype(np.float32)
+    return ((1 - a_ex) / (1 + a_ex)).astype(np.float32), ((1 - a_ey) / (1 + a_ey)).astype(np.float32)
+
+
+def _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx):
+    """Layer-harmonic Ey coefficient and the anisotropic (Farjadpour) bead-edge tensor for
+    Ex/Ey: (1/eps)_ii = n_i^2 <1/eps> + (1 - n_i^2)/<eps>  (fdtd2d_core.py:297-349)."""
+    if not bp:
+        return
+    c0 = dt / (EPS0 * dx)
+    lp = bp.get('layer')
+    h = None
+    if lp is not None:
+        h = np.where(lp['in_wg_x'][:, None], lp['h_core'][None, :], lp['h_out'][None, :])
+        Ca_ey[...] = (c0 / h).astype(np.float32)
+    for bd in bp.get('beads', []):
+        e_bg, ix0, jy0 = bd['eps_bg'], bd['ix0'], bd['jy0']
+        xs, ys, xc, yc, r, eb = bd['xs'], bd['ys'], bd['xc'], bd['yc'], bd['r'], bd['n'] ** 2
+        c = dt / (EPS0 * dx)
+
+        def tensor(xp_, yp_, ebg_a, ebg_h, comp):
+            f = circle_fraction(xp_, yp_, xc, yc, r, dx)
+            X = xp_[:, None] - xc
+            Y = yp_[None, :] - yc
+            d = np.maximum(np.sqrt(X * X + Y * Y), 1e-30)
+            nc2 = (X / d) ** 2 if comp == 'x' else (Y / d) ** 2
+            mean = (1 - f) * ebg_a + f * eb
+            hmean = (1 - f) / ebg_h + f / eb
+            return f, nc2 * hmean + (1 - nc2) / mean
+
+        f, inv = tensor(xs[:-1] + 0.5 * dx, ys, 0.5 * (e_bg[:-1] + e_bg[1:]),
+                        1.0 / (0.5 * (1.0 / e_bg[:-1] + 1.0 / e_bg[1:])), 'x')
+        sl = (slice(ix0, ix0 + inv.shape[0]), slice(jy0, jy0 + inv.shape[1]))
+        blk = Ca_ex[sl].astype(np.float64)
+        m = f > 0
+        blk[m] = c * inv[m]
+        Ca_ex[sl] = blk.astype(np.float32)
+        eh = 1.0 / (0.5 * (1.0 / e_bg[:, :-1] + 1.0 / e_bg[:, 1:]))
+        if h is not None:
+            hb = h[ix0:ix0 + eh.shape[0], jy0:jy0 + eh.shape[1]]
+            eh[:hb.shape[0], :hb.shape[1]] = hb
+        f, inv = tensor(xs, ys[:-1] + 0.5 * dx, 0.5 * (e_bg[:, :-1] + e_bg[:, 1:]), eh, 'y')
+        sl = (slice(ix0, ix0 + inv.shape[0]), slice(jy0, jy0 + inv.shape[1]))
+        blk = Ca_ey[sl].astype(np.float64)
+        m = f > 0
+        blk[m] = c * inv[m]
+        Ca_ey[sl] = blk.astype(np.float32)
+
+
+def _tm_node_field(Ey):
+    """Ey (Nx,Ny-1) -> node grid (Nx,Ny): neighbour mean inside, copies at the edges."""
+    out = np.zeros((Ey.shape[0], Ey.shape[1] + 1), dtype=np.float32)
+    out[:, 1:-1] = 0.5 * (Ey[:, :-1] + Ey[:, 1:])
+    out[:, 0] = Ey[:, 0]
+    out[:, -1] = Ey[:, -1]
+    return out
+
+
+def _n_source_2d(src_x_um, iy_src, y0_m, dx, wg_n, T_WG, n_below, gap_um):
+    if gap_um and src_x_um < gap_um:
+        return float(N_AIR)
+    y = y0_m + iy_src * dx
+    if 0.0 <= y < T_WG:
+        return float(np.real(wg_n))
+    return float(N_AIR) if y >= T_WG else float(np.real(n_below))
+
+
+def _vcsel_geom_2d(src_ix, dx, distance_um):
+    air = vcsel.air_path_um(distance_um, dx * 1e6)
+    return src_ix * dx * 1e6 + air, air, vcsel.z_at_source(distance_um, air)
+
+
+def _vcsel_source_2d(Ny, dx, dt, T_WG, src_ix, vcsel_offset, vcsel_waist, lambda_nm, tilt,
+                     n_src, z_um, f0, y0_m=None, y_center_m=None):
+    """Gaussian line source as a one-sided Huygens pair in column src_ix
+    (fdtd2d_core.py:378-397); returns (iy_src, TE source, TM Ey source)."""
+    if y0_m is None:
+        y0_m = -TEAR_BUFFER
+    yc = (T_WG / 2 if y_center_m is None else y_center_m) + vcsel_offset * 1e-6
+    iy_src = int(round((yc - y0_m) / dx))
+    y_um = (y0_m + np.arange(Ny) * dx) * 1e6
+    amp, ph = vcsel.profile_1d(y_um, yc * 1e6, z_um, vcsel_waist, lambda_nm, n_src, tilt)
+    if SRC_CORE_ONLY and y_center_m is None:
+        amp = amp * core_source_mask(y_um, T_WG * 1e6)
+    kdx = vcsel.numerical_kx_dx(lambda_nm, dx, C0 * dt, n_src, tilt)
+    src = vcsel.HuygensSource(src_ix, amp, ph, 2 * np.pi * f0, kdx)
+    return iy_src, src, src.to_ey()
+
+
+def _result_dict(label, P_in, P_out, Tr, frames, total_time, bead_d_um, wg_mat, bead_mat, wg_n,
+                 bead_n, wg_th, lam, waist, tilt, off, src, method, place_bead=True, n_aq=N_AQ,
+                 n_mucin=N_MUCIN, n_cornea=N_CORNEA, frames_memmap=None, t_aq_um=None,
+                 t_mu_um=None, t_lip_um=0.0, bead_x_um=None, x_wg_start_um=0.0,
+                 x_wg_end_um=None, polarization='s', stack=None, n_lip=None):
+    """Result dictionary with the reference's keys (fdtd2d_core.py:2020-2048)."""
+    t_aq = (t_aq_um * 1e-6) if t_aq_um is not None else T_AQ
+    t_mu = (t_mu_um * 1e-6) if t_mu_um is not None else T_MU
+    t_lip = (t_lip_um or 0.0) * 1e-6
+    if stack is None:
+        kp = dict(KAPPA or {})
+        nl = float(n_lip) if n_lip is not None else N_LIPID
+        stack_out = legacy_stack(t_lip * 1e6, complex(nl, kp.get('lip', 0.0)),
+                                 t_aq * 1e6, complex(float(n_aq), kp.get('aq', 0.0)),
+                                 t_mu * 1e6, complex(float(n_mucin), kp.get('mu', 0.0)))
+        layers = (t_lip, t_aq, t_mu, n_aq)
+    else:
+        stack_out = [dict(Ly) for Ly in stack]
+        layers = legacy_layers_tuple(stack)
+    return dict(scenario=label, P_in=P_in, P_out=P_out, transmission=Tr, frames=frames,
+                frames_memmap=frames_memmap, stack=stack_out, stack_custom=stack is not None,
+                total_time=total_time, layers=layers, n_aq=n_aq, n_mucin=n_mucin,
+                n_cornea=n_cornea, wg_material=wg_mat, bead_material=bead_mat, wg_n=wg_n,
+                bead_n=bead_n, bead_diameter_um=(bead_d_um if place_bead else 0.0),
+                bead_x_um=(bead_x_um if bead_x_um is not None else BEAD_X_UM),
+                x_wg_start_um=x_wg_start_um, x_wg_end_um=x_wg_end_um, wg_thickness_um=wg_th,
+                lambda_nm=lam, vcsel_waist=waist, vcsel_tilt=tilt, vcsel_offset=off,
+                source_type=src, method=method, polarization=polarization)
+
+
+def _emit_power(fl, dx):
+    if not (fl.has('src_up') and fl.has('src_dn')):
+        return None, None
+    up, dn = fl.flux('src_up'), fl.flux('src_dn')
+    return (dn - up) * dx, -up * dx
+
+
+def _vcsel_result(res, p_emit, p_refl, distance_um, air_um, x_facet_um, vcsel_waist):
+    res['laser_power_mw'] = LASER_POWER_MW
+    res['vcsel_distance_um'] = float(distance_um or 0.0)
+    res['vcsel_air_um'] = float(air_um)
+    res['vcsel_waist_um'] = float(vcsel_waist)
+    res['x_facet_um'] = float(x_facet_um)
+    res['p_src_si'] = p_emit
+    if p_emit and p_emit > 0:
+        p_in = res.get('p_in_si')
+        res['coupling_efficiency'] = (p_in / p_emit) if p_in is not None else float('nan')
+        res['facet_reflection'] = (p_refl / p_emit) if p_refl is not None else float('nan')
+
+
+def _engine_mode(mode):
+    return mode or backend.DEFAULT_MODE
+
+
+def _safe(label):
+    return ''.join(c if (c.isalnum() or c in '-_.') else '_' for c in label)
+
+
+# ----------------------------------------------------------------------------- run_beads
+def run_beads(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n_snapshots=12,
+              wg_thickness_um=5.0, lambda_nm=850.0, vcsel_waist=3.0, vcsel_tilt=0.0,
+              vcsel_offset=0.0, source_type='cw', method='full', window_w_um=350.0,
+              slide_um=150.0, place_bead=True, t_aqueous_um=None, t_mucin_um=None,
+              t_lipid_um=0.0, input_gap_um=0.0, length_um=None, bead_x_um=None, bead_y_um=None,
+              wg_n=None, bead_n=None, n_aqueous=None, n_mucin=None, n_cornea=None, n_lipid=None,
+              polarization='s', curved=False, boundary='cpml', cad=None, pulse_cycles=30.0,
+              pulse_tmax_fs=None, pulse_nfreq=None, pulse_dispersion=True, pulse_mode='fourier',
+              n_aq_tear_offset=None, tear_buffer_um=None, stack=None,
+              npml=None, engine_mode=None, beads=None, steps_override=None):
+    """2D full-domain run (reference fdtd2d_core.py:531-927); method='stitch' dispatches
+    to run_beads_stitched.  Extra keywords of this build: ``npml`` (CPML cells, reference
+    fixed 10), ``engine_mode`` ('ref32' | 'fp64acc'), ``beads`` (several beads),
+    ``steps_override`` (exact step count, no CW extension)."""
+    global TEAR_BUFFER, LAM_LOSS
+    if cad is not None:
+        raise NotImplementedError('CAD geometry import is outside the device path')
+    if method == 'sliding':
+        raise NotImplementedError("method 'sliding' is not part of the device path")
+    if str(source_type).lower() == 'pulse' and method == 'stitch':
+        raise NotImplementedError('pulse stitching is not part of the device path')
+    npml_v = NPML if npml is None else int(npml)
+    if stack is not None and tear_buffer_um is None:
+        need = stack_total_um(stack) + 1.5 + npml_v * dx_nm * 1e-3
+        if need * 1e-6 > TEAR_BUFFER + 1e-12:
+            tear_buffer_um = need
+    if tear_buffer_um is not None:
+        old = TEAR_BUFFER
+        TEAR_BUFFER = float(tear_buffer_um) * 1e-6
+        try:
+            kw = dict(locals())
+            args = {k: kw[k] for k in _RUN_BEADS_ARGS}
+            args['tear_buffer_um'] = None
+            args['stack'] = stack
+            return _run_beads_body(**args)
+        finally:
+            TEAR_BUFFER = old
+    kw = dict(locals())
+    return _run_beads_body(**{k: kw[k] for k in _RUN_BEADS_ARGS})
+
+
+def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n_snapshots=12,
+                    wg_thickness_um=5.0, lambda_nm=850.0, vcsel_waist=3.0, vcsel_tilt=0.0,
+                    vcsel_offset=0.0, source_type='cw', method='full', window_w_um=350.0,
+                    slide_um=150.0, place_bead=True, t_aqueous_um=None, t_mucin_um=None,
+                    t_lipid_um=0.0, input_gap_um=0.0, length_um=None, bead_x_um=None,
+                    bead_y_um=None, wg_n=None, bead_n=None, n_aqueous=None, n_mucin=None,
+                    n_cornea=None, n_lipid=None, polarization='s', curved=False, boundary='cpml',
+                    cad=None, pulse_cycles=30.0, pulse_tmax_fs=None, pulse_nfreq=None,
+                    pulse_dispersion=True, pulse_mode='fourier', n_aq_tear_offset=None,
+                    tear_buffer_um=None, stack=None, npml=None, engine_mode=None, beads=None,
+                    steps_override=None):
+    global LAM_LOSS
+    if method == 'stitch':
+        return run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots,
+                                  wg_thickness_um, lambda_nm, vcsel_waist, vcsel_tilt,
+                                  vcsel_offset, source_type, window_w_um, slide_um, place_bead,
+                                  t_aqueous_um, t_mucin_um=t_mucin_um, t_lipid_um=t_lipid_um,
+                                  input_gap_um=input_gap_um, length_um=length_um,
+                                  bead_x_um=bead_x_um, bead_y_um=bead_y_um, wg_n=wg_n,
+                                  bead_n=bead_n, n_aqueous=n_aqueous, n_mucin=n_mucin,
+                                  n_cornea=n_cornea, n_lipid=n_lipid, polarization=polarization,
+                                  curved=curved, boundary=boundary, stack=stack, npml=npml,
+                                  engine_mode=engine_mode, beads=beads)
+    pol = 'p' if str(polarization).lower() in ('p', 'tm') else 's'
+    kind = 'tm2d' if pol == 'p' else 'te2d'
+    t_aq_um = t_aqueous_um if t_aqueous_um is not None else T_AQ * 1e6
+    t_total = time.time()
+    wg_n = wg_n if wg_n is not None else n_at(wg_mat, lambda_nm)
+    bead_n = bead_n if bead_n is not None else n_at(bead_mat, lambda_nm)
+    n_aq_l = n_aqueous if n_aqueous is not None else n_at('aqueous', lambda_nm)
+    n_mu_l = n_mucin if n_mucin is not None else n_at('mucin', lambda_nm)
+    n_co_l = n_cornea if n_cornea is not None else cornea_default(lambda_nm)
+    n_lip_l = n_lipid if n_lipid is not None else n_at('lipid', lambda_nm)
+    L = (length_um * 1e-6) if length_um is not None else L_DEMO
+    L_um = L * 1e6
+    bx_um = bead_x_um if bead_x_um is not None else L_um / 2.0
+    T_WG = wg_thickness_um * 1e-6
+    LAM = lambda_nm * 1e-9
+    LAM_LOSS = LAM
+    label = f'WG-{wg_mat}_Bead-{bead_mat}_d{bead_d_um:g}_L{lambda_nm:g}' + ('' if place_bead else '_ref')
+    dx = dx_nm * 1e-9
+    dt = 0.5 * dx / (C0 * np.sqrt(2))
+    NP = NPML if npml is None else int(npml)
+    if str(boundary).lower() != 'cpml':
+        raise NotImplementedError(f"boundary '{boundary}' is not part of the device path; use 'cpml'")
+    src_ix = NP + 2
+    _vd = float(input_gap_um or 0.0)
+    x_facet_um, _air, _zsrc = _vcsel_geom_2d(src_ix, dx, _vd)
+    input_gap_um = x_facet_um
+    y0_m = -TEAR_BUFFER
+    H_total = T_WG + AIR_BUFFER + TEAR_BUFFER
+    Nx, Ny = int(round(L / dx)), int(round(H_total / dx))
+    print(f'\n===== Bead-Demo [full]: {label} =====  grid {Nx}x{Ny} dx={dx_nm}nm  pol {pol}  '
+          f'engine {backend.engine_name()}')
+    bp = {}
+    eps_r, sig = build_materials(Nx, Ny, dx, wg_n, bead_n, T_WG, bx_um, bead_d_um, 0.0, place_bead,
+                                 n_aq=n_aq_l, n_mucin=n_mu_l, n_cornea=n_co_l, t_aq_um=t_aq_um,
+                                 t_mu_um=t_mucin_um, t_lip_um=t_lipid_um, n_lip=n_lip_l,
+                                 input_gap_um=input_gap_um, length_um=length_um, curved=curved,
+                                 tm_patch=bp, bead_y_um=bead_y_um, stack=stack, beads=beads)
+    Ch = np.float32(dt / (MU0 * dx))
+    eng = backend.make_engine(kind, (Nx, Ny), _engine_mode(engine_mode))
+    if pol == 'p':
+        Ca_ex, Ca_ey, Ch_tm = _tm_coeffs(eps_r, dt, dx)
+        _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx)
+        Da_ex, Da_ey = _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt)
+        eng.set_coeffs(Ch_tm, cax=Ca_ex, cay=Ca_ey, dax=Da_ex, day=Da_ey)
+    else:
+        Ce_E, Ce_H = _coeffs(eps_r, sig, dt, dx)
+        eng.set_coeffs(Ch, ceE=Ce_E, ceH=Ce_H)
+    del eps_r, sig
+    f0 = C0 / LAM
+    sigma_t = 4 / (2 * np.pi * f0)
+    iy0 = int(round((T_WG / 2 + vcsel_offset * 1e-6 - y0_m) / dx))
+    n_src = _n_source_2d(src_ix * dx * 1e6, iy0, y0_m, dx, wg_n, T_WG,
+                         n_aq_l if stack is None else (float(complex(stack[0]['n']).real) if stack
+                                                       else float(n_co_l)), input_gap_um)
+    iy_src, vs, vs_ey = _vcsel_source_2d(Ny, dx, dt, T_WG, src_ix, vcsel_offset, vcsel_waist,
+                                         lambda_nm, vcsel_tilt, n_src, _zsrc, f0, y0_m=y0_m)
+    bnd = make_boundary('cpml', pol, Nx, Ny, dx, dt, npml=NP, lam_m=LAM)
+    bnd.attach(eng)
+    eng.finalize()
+    src = vs_ey if pol == 'p' else vs
+    field_comp = 'Ey' if pol == 'p' else 'Ez'
+    eng.set_source(field_comp, src.ix, src.amp, src.ph1, src.ph2)
+    steps_total = int((6 * sigma_t + 2 * L * N_G_FACTOR * wg_n / C0) / dt)
+    if steps_override:
+        steps_total = int(steps_override)
+    steps_max = int(CW_MAX_EXTEND * steps_total)
+    snap_every = max(1, steps_total // n_snapshots)
+    iy_lo = int(round((0.0 + TEAR_BUFFER) / dx))
+    iy_hi = int(round((T_WG + TEAR_BUFFER) / dx))
+    hw = max(1, int(round(min(DET_LEN / 2, 0.05 * L_um * 1e-6) / dx)))
+
+    def xwin(xu):
+        c = int(round(xu * 1e-6 / dx))
+        return max(0, c - hw), min(Nx, c + hw)
+    ixin0, ixin1 = xwin(0.1 * L_um)
+    ixout0, ixout1 = xwin(0.9 * L_um)
+    ci = (ixin0 + ixin1) // 2
+    sh = guard_in_plane(ci, max(src_ix, int(round(x_facet_um / (dx * 1e6)))), Nx) - ci
+    ixin0, ixin1 = ixin0 + sh, min(Nx, ixin1 + sh)
+    omega = 2 * np.pi * f0
+    steps_per_T = max(4, int(round((1.0 / f0) / dt)))
+    n_acc_cw = 8 * steps_per_T
+    conv = source_type != 'pulse' and not steps_override
+    cw_rel_change = float('nan')
+    n_avg0 = steps_total // 2
+    wide = eng.wide_acc
+    acc_mode = 'c128' if wide else 'c64p'
+    dft_src = 'ey_node' if pol == 'p' else 'Ez'
+    mon_pin = eng.add_box(((ixin0, ixin1), (iy_lo, iy_hi)))
+    mon_pout = eng.add_box(((ixout0, ixout1), (iy_lo, iy_hi)))
+    mon_cw = eng.add_dft(dft_src, 0, Nx, acc=acc_mode)
+    mon_cwex = eng.add_dft('ex_node', 0, Nx, acc=acc_mode) if pol == 'p' else None
+    mon_prev = eng.add_dft(dft_src, 0, Nx, acc=acc_mode) if conv else None
+    mon_I = eng.add_sq('tm' if pol == 'p' else 'te')
+    book = PlaneFluxBook(eng, kind)
+    for key, il in (('in', max(1, (ixin0 + ixin1) // 2)), ('out', max(1, (ixout0 + ixout1) // 2)),
+                    ('src_up', vs.plane_up), ('src_dn', vs.plane_dn)):
+        book.add_plane(key, il)
+    flux_bits = sum(1 << book.mon[k] for k in book.keys())
+    frames = []
+    mm_path = None
+    ez_mm = None
+    k_fr = 0
+    if save_frames:
+        os.makedirs('results', exist_ok=True)
+        mm_path = os.path.join('results', f'_stream_beads_{_safe(label)}_full.npy')
+        ez_mm = np.lib.format.open_memmap(mm_path, mode='w+', dtype=np.float32,
+                                          shape=(max(1, n_snapshots + 2), Nx, Ny))
+    nacc_cw = nacc_prev = nacc_I = 0
+    t_start = time.time()
+
+    def masks(n0, n1):
+        nonlocal nacc_cw, nacc_prev, nacc_I
+        ns = np.arange(n0, n1)
+        mk = np.full(len(ns), (1 << mon_pin) | (1 << mon_pout), dtype=np.uint32)
+        in_cw = ns >= steps_total - n_acc_cw
+        bits = (1 << mon_cw) | flux_bits | ((1 << mon_cwex) if mon_cwex is not None else 0)
+        mk[in_cw] |= np.uint32(bits)
+        cnt = int(in_cw.sum())
+        nacc_cw += cnt
+        book.note_samples(book.keys(), cnt)
+        if conv:
+            in_prev = (~in_cw) & (ns >= steps_total - 2 * n_acc_cw)
+            mk[in_prev] |= np.uint32(1 << mon_prev)
+            nacc_prev += int(in_prev.sum())
+        in_I = ns >= n_avg0
+        mk[in_I] |= np.uint32(1 << mon_I)
+        nacc_I += int(in_I.sum())
+        return mk
+
+    n = 0
+    while n < steps_total:
+        stop = steps_total
+        if ez_mm is not None and k_fr < ez_mm.shape[0]:
+            stop = min(stop, ((n // snap_every) + 1) * snap_every)
+        tab = new_steps(stop - n)
+        t_phys = np.arange(n, stop) * dt
+        env = (pulse_envelope(t_phys, f0, pulse_cycles) if source_type == 'pulse'
+               else cw_envelope(t_phys, sigma_t))
+        fill_phasors(tab, omega, dt, t_phys, env, src_omega=src.omega)
+        tab['mask'] = masks(n, stop)
+        eng.run(tab)
+        n = stop
+        if ez_mm is not None and n % snap_every == 0 and k_fr < ez_mm.shape[0]:
+            f = eng.read_field(field_comp)
+            ez_mm[k_fr] = (_tm_node_field(f) if pol == 'p' else f).astype(np.float32)
+            frames.append({'x_start': 0.0, 'x_end': L_um, 'y_start': y0_m * 1e6,
+                           'y_end': (y0_m + H_total) * 1e6, 'slide_i': 0, 'step': n,
+                           'time': time.time() - t_start})
+            k_fr += 1
+        if conv and n == steps_total and nacc_prev > 0 and nacc_cw > 0:
+            a = eng.read_monitor(mon_cw) / nacc_cw
+            b = eng.read_monitor(mon_prev) / nacc_prev
+            if not wide:
+                a, b = a.astype(np.complex64), b.astype(np.complex64)
+            cw_rel_change = float(np.linalg.norm(a - b) / max(float(np.linalg.norm(a)), 1e-30))
+            if cw_rel_change > CW_TOL and steps_total + n_acc_cw <= steps_max:
+                print(f'  [CW] not yet in steady state (change {100 * cw_rel_change:.2f}%) '
+                      f'-> +{n_acc_cw} steps')
+                mon_cw, mon_prev = mon_prev, mon_cw
+                nacc_prev, nacc_cw = nacc_cw, 0
+                eng.zero_monitor(mon_cw)
+                if mon_cwex is not None:
+                    eng.zero_monitor(mon_cwex)
+                book.reset()
+                steps_total += n_acc_cw
+            else:
+                print(f'  [CW] phasor change over the last 2 DFT windows: {100 * cw_rel_change:.3f}%')
+    if ez_mm is not None:
+        ez_mm.flush()
+        del ez_mm
+    Ecw = ((2.0 / nacc_cw) * eng.read_monitor(mon_cw)) if nacc_cw > 0 else None
+    Ex_cw = ((2.0 / nacc_cw) * eng.read_monitor(mon_cwex)) if (mon_cwex is not None and nacc_cw) else None
+    P_in = float(eng.read_monitor(mon_pin)) * dx * dx
+    P_out = float(eng.read_monitor(mon_pout)) * dx * dx
+    Iavg = (eng.read_monitor(mon_I) / max(1, nacc_I)) if nacc_I else None
+    Tr_I = P_out / max(P_in, 1e-30)
+    Tr, tr_def, p_si = Tr_I, 'intensity', None
+    if Ecw is not None and source_type != 'pulse' and book.has('in') and book.has('out'):
+        S_in, S_out = book.flux('in'), book.flux('out')
+        p_si = S_in * dx
+        if abs(S_in) > 0:
+            Tr, tr_def = S_out / S_in, 'poynting'
+    emit = _emit_power(book, dx) if source_type != 'pulse' else (None, None)
+    eng.close()
+    total_time = time.time() - t_total
+    print(f'[Result] {label} [full]: total {total_time:.1f}s  Transmission={Tr:.4f} ({tr_def})')
+    res = _result_dict(label, P_in, P_out, Tr, frames, total_time, bead_d_um, wg_mat, bead_mat,
+                       wg_n, bead_n, wg_thickness_um, lambda_nm, vcsel_waist, vcsel_tilt,
+                       vcsel_offset, source_type, 'full', place_bead, n_aq_l, n_mu_l, n_co_l,
+                       frames_memmap=mm_path, t_aq_um=t_aq_um, t_mu_um=t_mucin_um,
+                       t_lip_um=t_lipid_um, bead_x_um=bx_um, x_wg_start_um=(input_gap_um or 0.0),
+                       x_wg_end_um=L_um, polarization=pol, stack=stack, n_lip=n_lip_l)
+    res.update(Iavg=Iavg, Ecw=Ecw, cw_rel_change=cw_rel_change, transmission_intensity=Tr_I,
+               transmission_def=tr_def, steps_total=steps_total, p_in_si=p_si, p_in_unit='W/m',
+               view_primary='transient', cells=Nx * Ny, grid=(Nx, Ny))
+    if Ex_cw is not None:
+        res['Ex_cw_amp'] = Ex_cw
+    _vcsel_result(res, *emit, _vd, _air, x_facet_um, vcsel_waist)
+    return res
+
+
+# ----------------------------------------------------------------------------- stitch
+def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots,
+                       wg_thickness_um, lambda_nm, vcsel_waist, vcsel_tilt, vcsel_offset,
+                       source_type, window_w_um, slide_um, place_bead=True, t_aqueous_um=None,
+                       t_mucin_um=None, t_lipid_um=0.0, input_gap_um=0.0, length_um=None,
+                       bead_x_um=None, bead_y_um=None, wg_n=None, bead_n=None, n_aqueous=None,
+                       n_mucin=None, n_cornea=None, n_lipid=None, polarization='s', curved=False,
+                       boundary='cpml', cad=None, return_efull=False, stack=None, npml=None,
+                       engine_mode=None, beads=None, steps_win_override=None, n_windows=None,
+                       window_hook=None):
+    """2D domain decomposition (reference fdtd2d_core.py:1197-1583): windows of
+    ``window_w_um`` every ``slide_um``, each run to CW steady state; the previous window's
+    complex field is imposed with a cosine taper over the overlap; inner seams end in a
+    CPML extension that is not assembled.  ``steps_win_override`` / ``n_windows`` cap the
+    work per window / the number of windows (benchmarks); ``window_hook(w, info)`` is
+    called after each window."""
+    if cad is not None:
+        raise NotImplementedError('CAD geometry import is outside the device path')
+    if (boundary or 'cpml').lower() != 'cpml':
+        raise NotImplementedError("only 'cpml' outer boundaries are part of the device path")
+    global LAM_LOSS
+    pol = 'p' if str(polarization).lower() in ('p', 'tm') else 's'
+    kind = 'tm2d' if pol == 'p' else 'te2d'
+    t_total = time.time()
+    t_aq_um = t_aqueous_um if t_aqueous_um is not None else T_AQ * 1e6
+    wg_n = wg_n if wg_n is not None else n_at(wg_mat, lambda_nm)
+    bead_n = bead_n if bead_n is not None else n_at(bead_mat, lambda_nm)
+    n_aq_l = n_aqueous if n_aqueous is not None else n_at('aqueous', lambda_nm)
+    n_mu_l = n_mucin if n_mucin is not None else n_at('mucin', lambda_nm)
+    n_co_l = n_cornea if n_cornea is not None else cornea_default(lambda_nm)
+    n_lip_l = n_lipid if n_lipid is not None else n_at('lipid', lambda_nm)
+    L = (length_um * 1e-6) if length_um is not None else L_DEMO
+    L_um = L * 1e6
+    T_WG = wg_thickness_um * 1e-6
+    LAM = lambda_nm * 1e-9
+    LAM_LOSS = LAM
+    bx_um = bead_x_um if bead_x_um is not None else L_um / 2.0
+    label = f'WG-{wg_mat}_Bead-{bead_mat}_d{bead_d_um:g}_L{lambda_nm:g}' + ('' if place_bead else '_ref')
+    dx = dx_nm * 1e-9
+    dt = 0.5 * dx / (C0 * np.sqrt(2))
+    NP = NPML if npml is None else int(npml)
+    _vd = float(input_gap_um or 0.0)
+    x_facet_um, _air, _zsrc = _vcsel_geom_2d(NP + 2, dx, _vd)
+    input_gap_um = x_facet_um
+    y0_m = -TEAR_BUFFER
+    H_total = T_WG + AIR_BUFFER + TEAR_BUFFER
+    Ny = int(round(H_total / dx))
+    Nx_win = int(round(window_w_um * 1e-6 / dx))
+    S_cells = int(round(slide_um * 1e-6 / dx))
+    O_cells = Nx_win - S_cells
+    Nx_full = int(round(L / dx))
+    if O_cells < 8:
+        O_cells = max(O_cells, 1)
+    f0 = C0 / LAM
+    omega = 2 * np.pi * f0
+    sigma_t = 4 / (2 * np.pi * f0)
+    transit = window_w_um * 1e-6 * N_G_FACTOR * wg_n / C0
+    steps_win = int(3.0 * transit / dt) + 400
+    if steps_win_override:
+        steps_win = int(steps_win_override)
+    steps_per_T = max(4, int(round((1.0 / f0) / dt)))
+    n_acc = steps_per_T * 8
+    n_win = (int(np.ceil((Nx_full - Nx_win) / S_cells)) + 1) if Nx_full > Nx_win else 1
+    n_win_run = n_win if n_windows is None else min(n_win, int(n_windows))
+    print(f'\n===== Bead-Demo [stitch]: {label} =====  {n_win} windows of {Nx_win}x{Ny}, overlap '
+          f'{O_cells}, steps/window {steps_win}, pol {pol}, engine {backend.engine_name()}')
+    Ch = np.float32(dt / (MU0 * dx))
+    src_ix = NP + 2
+    iy0 = int(round((T_WG / 2 + vcsel_offset * 1e-6 - y0_m) / dx))
+    n_src = _n_source_2d(src_ix * dx * 1e6, iy0, y0_m, dx, wg_n, T_WG,
+                         n_aq_l if stack is None else (float(complex(stack[0]['n']).real) if stack
+                                                       else float(n_co_l)), input_gap_um)
+    iy_src, vs, vs_ey = _vcsel_source_2d(Ny, dx, dt, T_WG, src_ix, vcsel_offset, vcsel_waist,
+                                         lambda_nm, vcsel_tilt, n_src, _zsrc, f0, y0_m=y0_m)
+    src = vs_ey if pol == 'p' else vs
+    Nyc = (Ny - 1) if pol == 'p' else Ny
+    comp = 'Ey' if pol == 'p' else 'Ez'
+    wtap = (0.5 * (1.0 + np.cos(np.pi * np.arange(O_cells) / max(O_cells - 1, 1)))).astype(np.float32)
+    Efull = np.zeros((Nx_full, Nyc), dtype=np.complex64)
+    Ehand = None
+    ExI_full = np.zeros((Nx_full, Ny), dtype=np.float32) if pol == 'p' else None
+    ExA_full = np.zeros((Nx_full, Ny), dtype=np.complex64) if pol == 'p' else None
+
+    def gx0(w):
+        return min(w * S_cells, max(Nx_full - Nx_win, 0))
+    EXT = NP + 4
+    len_mat = L_um + EXT * dx * 1e6
+    iy_lo = int(round((0.0 + TEAR_BUFFER) / dx))
+    iy_hi = int(round((T_WG + TEAR_BUFFER) / dx))
+    ix_in = int(round(0.1 * Nx_full))
+    ix_out = int(round(0.9 * Nx_full))
+    ix_in = guard_in_plane(ix_in, max(src_ix, int(round(x_facet_um / (dx * 1e6)))), Nx_full)
+    fluxes = FluxAccumulator()
+    mode = _engine_mode(engine_mode)
+    win_info = []
+    for w in range(n_win_run):
+        g0 = gx0(w)
+        x_start_um = g0 * dx * 1e6
+        last = w == n_win - 1
+        glo = 0 if w == 0 else g0 + O_cells
+        ghi = min(g0 + Nx_win, Nx_full)
+        fpl = {k: g - g0 for k, g in (('in', ix_in), ('out', ix_out)) if glo <= g < ghi and g - g0 >= 1}
+        if w == 0:
+            fpl.update(src_up=vs.plane_up, src_dn=vs.plane_dn)
+        Nxw = Nx_win + (0 if last else EXT)
+        bp = {}
+        eps_r, sig = build_materials(Nxw, Ny, dx, wg_n, bead_n, T_WG, bx_um, bead_d_um, x_start_um,
+                                     place_bead, n_aq=n_aq_l, n_mucin=n_mu_l, n_cornea=n_co_l,
+                                     t_aq_um=t_aq_um, t_mu_um=t_mucin_um, t_lip_um=t_lipid_um,
+                                     n_lip=n_lip_l, input_gap_um=input_gap_um, length_um=len_mat,
+                                     curved=curved, tm_patch=bp, bead_y_um=bead_y_um, stack=stack,
+                                     beads=beads)
+        eng = backend.make_engine(kind, (Nxw, Ny), mode)
+        if pol == 'p':
+            Ca_ex, Ca_ey, Ch_tm = _tm_coeffs(eps_r, dt, dx)
+            _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx)
+            Da_ex, Da_ey = _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt)
+            eng.set_coeffs(Ch_tm, cax=Ca_ex, cay=Ca_ey, dax=Da_ex, day=Da_ey)
+            del Ca_ex, Ca_ey, Da_ex, Da_ey
+        else:
+            Ce_E, Ce_H = _coeffs(eps_r, sig, dt, dx)
+            eng.set_coeffs(Ch, ceE=Ce_E, ceH=Ce_H)
+            del Ce_E, Ce_H
+        del eps_r, sig
+        pml_l = w == 0
+        pml_r = True
+        bnd = make_boundary('cpml', pol, Nxw, Ny, dx, dt,
+                            npml_x=(NP if pml_l else 0, NP if pml_r else 0), npml_y=NP, lam_m=LAM)
+        bnd.attach(eng)
+        eng.finalize()
+        if w == 0:
+            eng.set_source(comp, src.ix, src.amp, src.ph1, src.ph2)
+        elif Ehand is not None:
+            eng.set_drive(O_cells, wtap, {comp: Ehand}, math='c128')
+        wide = eng.wide_acc
+        acc_mode = 'c128' if wide else 'c64'
+        mon_acc = eng.add_dft(comp, 0, Nxw, acc=acc_mode, phasor='p')
+        mon_ex2 = eng.add_sq('tm_ex2') if pol == 'p' else None
+        mon_exa = eng.add_dft('ex_node', 0, Nxw, acc=acc_mode, phasor='p') if pol == 'p' else None
+        book = PlaneFluxBook(eng, kind)
+        for k, il in fpl.items():
+            book.add_plane(k, il)
+        bits = (1 << mon_acc) | sum(1 << book.mon[k] for k in fpl)
+        if pol == 'p':
+            bits |= (1 << mon_ex2) | (1 << mon_exa)
+        ns = np.arange(steps_win)
+        t_phys = ns * dt
+        tab = new_steps(steps_win)
+        fill_phasors(tab, omega, dt, t_phys, cw_envelope(t_phys, sigma_t), src_omega=src.omega)
+        win = ns >= steps_win - n_acc
+        tab['mask'] = np.where(win, np.uint32(bits), np.uint32(0))
+        acc_n = int(win.sum())
+        book.note_samples(list(fpl), acc_n)
+        tw = time.time()
+        chunk = max(1, steps_win // 10)
+        for c0 in range(0, steps_win, chunk):
+            eng.run(tab[c0:c0 + chunk])
+        eng.sync()
+        tw = time.time() - tw
+        Ew = ((2.0 / max(acc_n, 1)) * eng.read_monitor(mon_acc)).astype(np.complex64)
+        if not last:
+            off = gx0(w + 1) - g0
+            Ehand = Ew[off:off + O_cells, :].copy()
+        if w == 0:
+            g_lo, l_lo = 0, 0
+        else:
+            g_lo, l_lo = g0 + O_cells, O_cells
+        g_hi = min(g0 + Nx_win, Nx_full)
+        l_hi = l_lo + (g_hi - g_lo)
+        if g_hi > g_lo:
+            Efull[g_lo:g_hi, :] = Ew[l_lo:l_hi, :]
+            if ExI_full is not None:
+                ExI_full[g_lo:g_hi, :] = eng.read_monitor(mon_ex2)[l_lo:l_hi, :] / max(acc_n, 1)
+                ExA_full[g_lo:g_hi, :] = eng.read_monitor(mon_exa)[l_lo:l_hi, :] * (2.0 / max(acc_n, 1))
+        fluxes.take(book, list(fpl))
+        eng.close()
+        info = dict(window=w, cells=Nxw * Ny, steps=steps_win, seconds=tw)
+        win_info.append(info)
+        if window_hook is not None:
+            window_hook(w, info)
+        print(f'  [Window {w + 1}/{n_win}] x0={x_start_um:.0f}um  {tw:.2f}s  '
+              f'max|E|={np.abs(Ew).max():.3e}', flush=True)
+    total_time = time.time() - t_total
+    frames = []
+    mm_path = None
+    nfr = max(1, n_snapshots)
+    if save_frames:
+        os.makedirs('results', exist_ok=True)
+        mm_path = os.path.join('results', f'_stream_beads_{_safe(label)}_stitch.npy')
+        ez_mm = np.lib.format.open_memmap(mm_path, mode='w+', dtype=np.float32,
+                                          shape=(nfr, Nx_full, Ny))
+        for k in range(nfr):
+            tk = (k / nfr) * (1.0 / f0)
+            fr = np.real(Efull * np.exp(1j * omega * tk)).astype(np.float32)
+            ez_mm[k] = _tm_node_field(fr) if pol == 'p' else fr
+            frames.append({'x_start': 0.0, 'x_end': L_um, 'y_start': y0_m * 1e6,
+                           'y_end': (y0_m + H_total) * 1e6, 'slide_i': 0, 'step': k, 'time': tk})
+        ez_mm.flush()
+        del ez_mm
+    P_in = float(np.sum(np.abs(Efull[ix_in, iy_lo:iy_hi]) ** 2))
+    P_out = float(np.sum(np.abs(Efull[ix_out, iy_lo:iy_hi]) ** 2))
+    if ExI_full is not None:
+        P_in += 2.0 * float(np.sum(ExI_full[ix_in, iy_lo:iy_hi]))
+        P_out += 2.0 * float(np.sum(ExI_full[ix_out, iy_lo:iy_hi]))
+        for x0 in range(0, Nx_full, 4096):
+            x1 = min(x0 + 4096, Nx_full)
+            s = 0.5 * np.abs(Efull[x0:x1]) ** 2
+            ey2 = np.empty((x1 - x0, Ny), dtype=np.float32)
+            ey2[:, 1:-1] = 0.5 * (s[:, :-1] + s[:, 1:])
+            ey2[:, 0] = s[:, 0]
+            ey2[:, -1] = s[:, -1]
+            ExI_full[x0:x1] += ey2
+    Tr_I = P_out / max(P_in, 1e-30)
+    p_si = None
+    if fluxes.has('in') and fluxes.has('out'):
+        S_in, S_out = fluxes.flux('in'), fluxes.flux('out')
+        p_si = S_in * dx
+    else:
+        S_in = S_out = 0.0
+    Tr = S_out / S_in if abs(S_in) > 0 else Tr_I
+    print(f'[Result] {label} [stitch]: total {total_time:.1f}s  Transmission={Tr:.4f}')
+    res = _result_dict(label, P_in, P_out, Tr, frames, total_time, bead_d_um, wg_mat, bead_mat,
+                       wg_n, bead_n, wg_thickness_um, lambda_nm, vcsel_waist, vcsel_tilt,
+                       vcsel_offset, source_type, 'stitch', place_bead, n_aq_l, n_mu_l, n_co_l,
+                       frames_memmap=mm_path, t_aq_um=t_aq_um, t_mu_um=t_mucin_um,
+                       t_lip_um=t_lipid_um, bead_x_um=bx_um, x_wg_start_um=(input_gap_um or 0.0),
+                       x_wg_end_um=L_um, polarization=pol, stack=stack, n_lip=n_lip_l)
+    res.update(transmission_intensity=Tr_I, transmission_def='poynting', view_primary='cw',
+               cw_only=True, p_in_si=p_si, p_in_unit='W/m', windows=win_info,
+               n_windows=n_win, steps_win=steps_win, Efull=Efull if return_efull else None)
+    _vcsel_result(res, *_emit_power(fluxes, dx), _vd, _air, x_facet_um, vcsel_waist)
+    if ExI_full is not None:
+        res['Iavg'] = ExI_full
+        res['Ex_cw_amp'] = ExA_full
+    return res
+
+
+import inspect as _inspect                                                   # noqa: E402
+_RUN_BEADS_ARGS = [k for k in _inspect.signature(_run_beads_body).parameters]

@@ -44,9 +44,17 @@ class PlaneFluxBook:
         self.mon = {}
         self.samples = {}
 
-    def add_plane(self, key, il):
+    def add_plane(self, key, il, carry=None):
+        """Register plane ``key`` at local index ``il``.  ``carry`` (a FluxAccumulator)
+        continues sums started in an earlier window: the reference keeps one running
+        sum per key across stitch windows (fdtd3d_core.py:1640-1641), so a plane that
+        two windows assemble keeps accumulating into the same sum."""
         self.mon[key] = self.engine.add_flux_plane(int(il))
         self.samples[key] = 0
+        if carry is not None and carry.has(key):
+            sums, n = carry.data[key]
+            self.engine.write_monitor(self.mon[key], [(aE, aH) for _s, aE, aH in sums])
+            self.samples[key] = n
 
     def keys(self):
         return list(self.mon)

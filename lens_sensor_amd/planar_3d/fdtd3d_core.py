@@ -780,7 +780,7 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
         mon_I = eng.add_sq('3d_p' if pol == 'p' else '3d_s', Nx_win)
         book = PlaneFluxBook(eng, 'yee3d')
         for k, il in fpl.items():
-            book.add_plane(k, il)
+            book.add_plane(k, il, carry=fluxes)
         acc_bits = (1 << mon_acc) | sum(1 << m for m in mon_hand) | sum(1 << book.mon[k] for k in fpl)
         tab = new_steps(steps_win)
         ns = np.arange(steps_win)

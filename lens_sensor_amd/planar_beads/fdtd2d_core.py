@@ -768,7 +768,7 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
         mon_exa = eng.add_dft('ex_node', 0, Nxw, acc=acc_mode, phasor='p') if pol == 'p' else None
         book = PlaneFluxBook(eng, kind)
         for k, il in fpl.items():
-            book.add_plane(k, il)
+            book.add_plane(k, il, carry=fluxes)
         bits = (1 << mon_acc) | sum(1 << book.mon[k] for k in fpl)
         if pol == 'p':
             bits |= (1 << mon_ex2) | (1 << mon_exa)

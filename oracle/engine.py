@@ -243,6 +243,19 @@ class OracleEngine:
             return m['a'].copy()
         return [(p[2].copy(), p[3].copy()) for p in m['pairs']]
 
+    def write_monitor(self, mid, value):
+        m = self.mons[mid]
+        if m['t'] == 'box':
+            m['val'] = value
+        elif m['t'] == 'sq':
+            m['acc'][...] = value
+        elif m['t'] == 'dft':
+            m['a'][...] = value
+        else:
+            for p, (aE, aH) in zip(m['pairs'], value):
+                p[2][...] = aE
+                p[3][...] = aH
+
     def read_flux(self, mid):
         return self.read_monitor(mid)
 

@@ -1,0 +1,236 @@
+"""Benchmark of the Yee time-stepping hot path on one MI355X.
+
+Workload (default, --workload tm2d_stitch): 2D TM (Ey/Hx/Hz) stitch mode, FP32 fields
+with FP64 phasor/flux sums: 10 windows of 8192 x 1536 cells at dx = 25 nm, 24-cell CPML
+(inner seams end in the non-assembled CPML extension), 20000 steps per window with the
+complex-phasor handoff; PMMA core n=1.49 (2 um) over aqueous 1.331+0.001i (3 um),
+mucin 1.34 (0.5 um), cornea 1.376; 0-8 polystyrene beads (n=1.59, kappa 0.002,
+r ~ U(0.2, 0.6) um) per window drawn from the seed.  A "step" is one Yee time step
+of one window (12.6 M cells); value = cell updates / second of device stepping time.
+
+--workload yee3d: 3D FP32 Yee + CPML, 512 x 256 x 256 at dx = 30 nm, 12-cell CPML on
+all faces, 4-layer stack + lipid, one bead, Gaussian CW source, 8000 steps, FP64 phasor
+sums on one monitor plane.
+
+Prints ONE JSON line (metric, value, roofline of the dominant kernel, CPU baseline of the
+NumPy oracle on a bounded sample of the same workload).
+"""
+import argparse
+import contextlib
+import io
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell update: every field read+written once, every per-cell
+# coefficient read once (psi slabs, sources and monitors not counted)
+BYTES_TM2D_LOSSY = 3 * 8 + 4 * 4      # Ex, Ey, Hz r/w + Ca_ex, Ca_ey, D_ex, D_ey = 40 B
+BYTES_YEE3D = 6 * 8 + 3 * 4           # 6 fields r/w + ce_x, ce_y, ce_z = 60 B
+
+
+def _timed_factory(records):
+    from lens_sensor_amd.engine.hip import HipEngine
+
+    class TimedHip(HipEngine):
+        def __init__(self, kind, dims, mode):
+            super().__init__(kind, dims, mode)
+            self.set_timing(True)
+            self._upd = 0.0
+            self._tot = 0.0
+            self._n = 0
+            records.append(self)
+
+        def run(self, tab):
+            super().run(tab)
+            u, t, n = self.kernel_ms()
+            self._upd += u
+            self._tot += t
+            self._n += n
+
+    return TimedHip
+
+
+def bench_tm2d_stitch(args):
+    from lens_sensor_amd.common import backend
+    from lens_sensor_amd.planar_beads import fdtd2d_core as d2
+    rng = np.random.default_rng(args.seed)
+    dx_nm = 25.0
+    nx_win, ny = 8192, 1536
+    slide = 6144
+    n_win = 10 if args.windows is None else args.windows
+    steps = 20000 if args.steps_override is None else args.steps_override
+    dx_um = dx_nm * 1e-3
+    length_um = (nx_win + 9 * slide) * dx_um
+    t_wg = 2.0
+    tear_um = ny * dx_um - t_wg - d2.AIR_BUFFER * 1e6
+    stack = [dict(name='Aqueous', t_um=3.0, n=complex(1.331, 0.001), kappa=0.001, mat=None),
+             dict(name='Mucin', t_um=0.5, n=complex(1.34, 0.0), kappa=0.0, mat=None)]
+    beads = []
+    for w in range(10):
+        x0 = w * slide * dx_um
+        for _ in range(int(rng.integers(0, 9))):
+            r = float(rng.uniform(0.2, 0.6))
+            beads.append(dict(x_um=float(x0 + rng.uniform(r, nx_win * dx_um - r)), d_um=2 * r,
+                              y_um=-r, n=1.59))
+    records = []
+    info = []
+    old_k, old_tb = d2.KAPPA, d2.TEAR_BUFFER
+    d2.KAPPA = {'bead': 0.002}
+    d2.TEAR_BUFFER = tear_um * 1e-6
+    t0 = time.time()
+    try:
+        with backend.use_engine(_timed_factory(records)), contextlib.redirect_stdout(io.StringIO()):
+            r = d2.run_beads_stitched(
+                'pmma', 'polystyrol', 0.0, dx_nm, False, 2, t_wg, 850.0, 3.0, 0.0, 0.0, 'cw',
+                nx_win * dx_um, slide * dx_um, place_bead=False, length_um=length_um,
+                wg_n=1.49, n_cornea=1.376, polarization='p', stack=stack, npml=24,
+                engine_mode='fp64acc', beads=beads, steps_win_override=steps,
+                n_windows=n_win, window_hook=lambda w, i: (info.append(i),
+                                                           print(f'[bench] window {w} {i}',
+                                                                 file=sys.stderr, flush=True)))
+    finally:
+        d2.KAPPA, d2.TEAR_BUFFER = old_k, old_tb
+    wall = time.time() - t0
+    cells = [i['cells'] for i in info]
+    step_s = sum(i['seconds'] for i in info)
+    cell_updates = sum(c * i['steps'] for c, i in zip(cells, info))
+    upd_ms = sum(e._upd for e in records)
+    n_upd = sum(e._n for e in records)
+    return dict(cell_updates=cell_updates, step_seconds=step_s, wall=wall, steps=n_upd,
+                upd_ms_per_step=upd_ms / max(n_upd, 1), cells_per_step=float(np.mean(cells)),
+                bytes_per_cell=BYTES_TM2D_LOSSY, transmission=float(r['transmission']),
+                windows=len(info), grid=(nx_win, ny), steps_per_window=steps)
+
+
+def bench_yee3d(args):
+    from lens_sensor_amd.common import backend
+    from lens_sensor_amd.planar_3d import fdtd3d_core as f3d
+    rng = np.random.default_rng(args.seed)
+    steps = 8000 if args.steps_override is None else args.steps_override
+    dx_um = 0.03
+    nx, ny, nz = 512, 256, 256
+    stack = [dict(name='Lipid', t_um=0.1, n=1.46, kappa=0.0, mat=None),
+             dict(name='Aqueous', t_um=3.0, n=1.333, kappa=0.0, mat=None),
+             dict(name='Mucin', t_um=0.5, n=1.34, kappa=0.0, mat=None)]
+    t_wg = 2.0
+    air = 0.6
+    tear = ny * dx_um - t_wg - air
+    bead = dict(x_um=float(rng.uniform(6.0, 12.0)), d_um=1.0, y_um=-0.6, n=1.59,
+                z_um=float(rng.uniform(-1.0, 1.0)))
+    records = []
+    t0 = time.time()
+    with backend.use_engine(_timed_factory(records)), contextlib.redirect_stdout(io.StringIO()):
+        r = f3d.run_3d('bench3d', 1.49, t_wg_um=t_wg, lam_nm=850.0, lx_um=nx * dx_um, air_um=air,
+                       tear_um=tear, lz_um=nz * dx_um, dx_nm=30.0, vcsel_waist_um=1.5,
+                       vcsel_waist_z_um=1.5, n_snapshots=1, bead=bead, stack=stack, n_co=1.376,
+                       npml=12, engine_mode='fp64acc', steps_override=steps)
+    wall = time.time() - t0
+    e = records[0]
+    cells = nx * ny * nz
+    return dict(cell_updates=cells * steps, step_seconds=e._tot / 1e3, wall=wall, steps=e._n,
+                upd_ms_per_step=e._upd / max(e._n, 1), cells_per_step=float(cells),
+                bytes_per_cell=BYTES_YEE3D, transmission=float(r['transmission']), grid=(nx, ny, nz),
+                steps_per_window=steps)
+
+
+def cpu_baseline(workload, seconds=12.0):
+    """NumPy oracle (one core) on a bounded sample: the same grid and coefficients, as many
+    steps as fit in ~``seconds``; Mcell-updates/s."""
+    from oracle.engine import OracleEngine
+    from lens_sensor_amd.planar_beads.boundaries import cpml_profile_1d
+    from lens_sensor_amd.engine.spec import new_steps, fill_phasors
+    if workload == 'tm2d_stitch':
+        kind, dims = 'tm2d', (8192 + 28, 1536)
+    else:
+        kind, dims = 'yee3d', (512, 256, 256)
+    eng = OracleEngine(kind, dims, 'ref32')
+    dx, dt = 25e-9, 0.5 * 25e-9 / (3e8 * np.sqrt(2))
+    if kind == 'tm2d':
+        eng.set_coeffs(np.float32(0.4), cax=np.full((dims[0] - 1, dims[1]), 0.1, np.float32),
+                       cay=np.full((dims[0], dims[1] - 1), 0.1, np.float32),
+                       dax=np.full((dims[0] - 1, dims[1]), 0.999, np.float32),
+                       day=np.full((dims[0], dims[1] - 1), 0.999, np.float32))
+    else:
+        sh = {'cex': (dims[0] - 1, dims[1], dims[2]), 'cey': (dims[0], dims[1] - 1, dims[2]),
+              'cez': (dims[0], dims[1], dims[2] - 1)}
+        eng.set_coeffs(np.float32(0.3), **{k: np.full(v, 0.1, np.float32) for k, v in sh.items()})
+    for ax, N in enumerate(dims):
+        e, h = cpml_profile_1d(N, 24 if kind == 'tm2d' else 12, 24 if kind == 'tm2d' else 12,
+                               dx, dt, lam_m=850e-9)
+        eng.set_cpml(ax, e, h, True, True)
+    eng.finalize()
+    comp = 'Ey' if kind == 'tm2d' else 'Ez'
+    plane = eng.F[comp].shape[1:]
+    eng.set_source(comp, 30, np.ones(plane, np.float32), np.zeros(plane, np.float32),
+                   np.full(plane, 0.7, np.float32))
+    tab = new_steps(1)
+    fill_phasors(tab, 2 * np.pi * 3.5e14, dt, np.array([1e-15]), np.array([1.0]))
+    eng.run(tab)                                   # warm-up
+    n, t0 = 0, time.time()
+    while time.time() - t0 < seconds:
+        eng.run(tab)
+        n += 1
+    el = time.time() - t0
+    cells = int(np.prod(dims))
+    return dict(value=cells * n / el / 1e6, unit='Mcell-updates/s', cores=1, kind='port',
+                sample=f'{n} {kind} steps of the NumPy oracle on {"x".join(map(str, dims))} '
+                       f'cells (same grid and CPML as the workload), {el:.1f}s, one core; '
+                       f'host has {os.cpu_count()} CPUs')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=None,
+                    help='accepted for the driver contract: the workload fixes its steps')
+    ap.add_argument('--warmup', type=int, default=0)
+    ap.add_argument('--workload', default='tm2d_stitch', choices=('tm2d_stitch', 'yee3d'))
+    ap.add_argument('--windows', type=int, default=None, help='tm2d_stitch: windows to run')
+    ap.add_argument('--steps-override', type=int, default=None,
+                    help='steps per window (reduced runs for profiling only)')
+    ap.add_argument('--seed', type=int, default=2024)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not '
+              'part of it', file=sys.stderr)
+        sys.exit(2)
+    from lens_sensor_amd.engine import hip
+    dev = hip.device_name()
+    if args.workload == 'tm2d_stitch':
+        res = bench_tm2d_stitch(args)
+        cfg = dict(workload='tm2d_stitch_10x8192x1536', windows=res['windows'],
+                   grid=list(res['grid']), steps_per_window=res['steps_per_window'],
+                   dx_nm=25.0, npml=24, fields='fp32', phasors='fp64')
+    else:
+        res = bench_yee3d(args)
+        cfg = dict(workload='yee3d_512x256x256', grid=list(res['grid']),
+                   steps=res['steps_per_window'], dx_nm=30.0, npml=12, fields='fp32',
+                   phasors='fp64')
+    value = res['cell_updates'] / res['step_seconds'] / 1e6
+    ms_step = res['step_seconds'] * 1e3 / max(1, res['steps'])
+    achieved = res['bytes_per_cell'] * res['cells_per_step'] / (res['upd_ms_per_step'] * 1e-3) / 1e9
+    out = dict(metric='Mcell-updates/s', value=value, unit='Mcell-updates/s', n_gpus=1,
+               steps=res['steps'], warmup=args.warmup, ms_per_step=ms_step,
+               higher_is_better=True, scaling='weak', vs_baseline=None, dtype='fp32',
+               data='synthetic', config=cfg, device=dev,
+               roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s',
+                             frac=achieved / HBM_PEAK_GBS, traffic=None,
+                             kernel='Yee H+E update (per step, HIP events)',
+                             bytes_per_cell_update=res['bytes_per_cell'],
+                             update_ms_per_step=res['upd_ms_per_step']),
+               wall_s=res['wall'], transmission=res['transmission'])
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(args.workload)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,115 @@
+/* lensfdtd.h - C ABI of the MI355X (gfx950) Yee-FDTD step engine.
+ *
+ * One context = one Yee domain resident in HBM (fields, CPML psi slabs, per-cell
+ * update coefficients, source planes, monitor accumulators) plus one HIP stream.
+ * The reference drives its solvers per array operation from Python
+ * (planar_beads/fused2d.py:105-126 te_update_H/te_update_E/tm_update_H/tm_update_E,
+ * planar_beads/boundaries.py:186-268 CPML_TE/CPML_TM.update_H/update_E/terminate,
+ * planar_3d/boundaries3d.py:305-362 CPML_3D.step/step_H/step_E); this ABI replaces
+ * those calls and the per-step work of the time loops (fdtd2d_core.py:781-873,
+ * :1403-1474; fdtd3d_core.py:1105-1197, :1616-1656) at loop level: lf_run advances
+ * a chunk of steps described by a table of per-step scalars.
+ *
+ * Conventions: arrays passed in or out are host pointers to C-order float32 (or the
+ * stated type) arrays with the reference's shapes; the library copies them and keeps
+ * no pointer after return.  Every function returns 0 on success and a negative code
+ * on error; lf_last_error() gives the message (thread-local).  Calls on one context
+ * are not thread-safe; distinct contexts may be used from distinct threads.
+ */
+#ifndef LENSFDTD_H
+#define LENSFDTD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lf_ctx lf_ctx;
+
+/* One time step's scalars (120 bytes, packed); filled by the host with the
+ * reference's expressions (env: fdtd2d_core.py:787; phasors: :839, flux.py:32-33;
+ * drive: fdtd2d_core.py:1426, fdtd3d_core.py:1627; split DFT: fdtd3d_core.py:1638). */
+typedef struct lf_step {
+    double env, ssin, scos;      /* envelope, env*sin(wt), env*cos(wt)            */
+    double pr, pi;               /* exp(-i w t)                                   */
+    double hr, hi;               /* exp(-i w (t - dt/2))                          */
+    double dr, di;               /* exp(+i w t)                                   */
+    float wt32, e32;             /* float32(w t), float32(env)                    */
+    float pr32, pi32;            /* complex64(exp(-i w t))                        */
+    float hr32, hi32;            /* complex64(exp(-i w (t - dt/2)))               */
+    float qr32, qi32;            /* float32(cos w t), -float32(sin w t)           */
+    float dr32, di32;            /* complex64(exp(+i w t))                        */
+    unsigned int mask;           /* bit m set: monitor m accumulates this step    */
+    unsigned int pad;
+} lf_step;
+
+enum { LF_TE2D = 0, LF_TM2D = 1, LF_YEE3D = 2 };
+enum { LF_MODE_REF32 = 0, LF_MODE_FP64ACC = 1 };
+/* field / coefficient ids */
+enum { LF_EX = 0, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ,
+       LF_CEE, LF_CEH,                 /* 2D TE  Ce_E, Ce_H  (Nx,Ny)              */
+       LF_CAX, LF_CAY,                 /* 2D TM  Ca_ex, Ca_ey                     */
+       LF_CEX, LF_CEY, LF_CEZ,         /* 3D     ce_x, ce_y, ce_z                 */
+       LF_DAX, LF_DAY, LF_DAZ,         /* loss factors D (TM 2D: x,y; 3D: x,y,z)  */
+       LF_NARRAYS };
+/* monitor kinds / options */
+enum { LF_SQ_TE = 0, LF_SQ_TM = 1, LF_SQ_TM_EX2 = 2, LF_SQ_3D_S = 3, LF_SQ_3D_P = 4 };
+enum { LF_SRC_EX = 0, LF_SRC_EY = 1, LF_SRC_EZ = 2, LF_SRC_EY_NODE = 3, LF_SRC_EX_NODE = 4 };
+enum { LF_PH_P = 0, LF_PH_Q = 1 };            /* complex64 phasor: (pr32,pi32) or (qr32,qi32) */
+enum { LF_ACC_C64P = 0, LF_ACC_C64 = 1, LF_ACC_C128 = 2 };
+enum { LF_DRIVE_C128 = 0, LF_DRIVE_C64 = 1 };
+
+/* Context lifetime.  dims: nx, ny (2D) or nx, ny, nz (3D; pass nz = 1 for 2D). */
+int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out);
+int lf_destroy(lf_ctx *ctx);
+const char *lf_last_error(void);
+int lf_device_count(int *count);
+int lf_device_name(char *buf, int len);
+
+/* Setup.  lf_upload/lf_download move one field or coefficient (reference shape). */
+int lf_upload(lf_ctx *ctx, int array_id, const float *host, long long count);
+int lf_download(lf_ctx *ctx, int array_id, float *host, long long count);
+int lf_set_ch(lf_ctx *ctx, float ch);
+/* CPML profiles of one axis (boundaries.py:104-151, boundaries3d.py:150-194):
+ * e_* on the n integer nodes, h_* on the n-1 half nodes; PEC on the PML ends. */
+int lf_set_cpml(lf_ctx *ctx, int axis, const float *e_kappa, const float *e_b, const float *e_c,
+                const float *h_kappa, const float *h_b, const float *h_c, int pec_lo, int pec_hi);
+int lf_finalize(lf_ctx *ctx);   /* allocate psi slabs (after every lf_set_cpml)  */
+/* Huygens source pair on planes ix (ph1) and ix+1 (ph2) of component comp
+ * (vcsel.py:151-156); amp/ph arrays have the component's plane shape. */
+int lf_set_source(lf_ctx *ctx, int comp, int ix, int plane_count, const float *amp,
+                  const float *ph1, const float *ph2);
+/* Stitch overlap imposition F[:n] = w d(t) + (1-w) F[:n] (fdtd2d_core.py:1426-1427,
+ * fdtd3d_core.py:1626-1630); drives: for each comp in comps, a complex64 array
+ * (interleaved re,im) of shape (n, plane) stored back to back. */
+int lf_set_drive(lf_ctx *ctx, int ncells, const float *wtap, int ncomp, const int *comps,
+                 const float *drives_c64, int math);
+
+/* Monitors; each returns its id (0..31) in *id. */
+int lf_add_box(lf_ctx *ctx, const int *ranges18, int *id);  /* 3 comps x 3 axes x (lo,hi) */
+int lf_add_sq(lf_ctx *ctx, int form, int nx_acc, int *id);
+int lf_add_dft(lf_ctx *ctx, int src, int x0, int x1, int kplane, int phasor, int acc, int *id);
+int lf_add_flux(lf_ctx *ctx, int il, int *id);
+int lf_monitor_bytes(lf_ctx *ctx, int id, long long *bytes);
+int lf_monitor_read(lf_ctx *ctx, int id, void *dst, long long bytes);
+int lf_monitor_write(lf_ctx *ctx, int id, const void *src, long long bytes);
+int lf_monitor_zero(lf_ctx *ctx, int id);
+
+/* Advance nsteps steps; steps[s] holds the scalars of step s. */
+int lf_run(lf_ctx *ctx, const lf_step *steps, int nsteps);
+int lf_sync(lf_ctx *ctx);
+/* Average duration [ms] of the Yee update kernels of the last lf_run, measured with
+ * HIP events on the context's stream (0 if timing is off); lf_set_timing(ctx, 1) on. */
+int lf_set_timing(lf_ctx *ctx, int on);
+/* Kernel variant (no reference counterpart: an engine tuning knob): 0 = single-pass
+ * H+E kernels where the kind has one (default), 1 = separate H and E kernels; lx = x
+ * rows per workgroup of the single-pass kernels. */
+int lf_set_variant(lf_ctx *ctx, int variant, int lx);
+/* y rows (wavefronts) per workgroup of the 3D single-pass kernel: 2, 4 or 8 (tuning
+ * knob, no reference counterpart). */
+int lf_set_tile(lf_ctx *ctx, int ty);
+int lf_kernel_ms(lf_ctx *ctx, double *update_ms, double *total_ms, int *nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

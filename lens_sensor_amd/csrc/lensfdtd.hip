@@ -1,0 +1,1955 @@
+// lensfdtd.hip - Yee FDTD step engine for MI355X (gfx950): kernels + C ABI.
+//
+// Layout in HBM: every field component and every per-cell coefficient of a context
+// lives in the same padded grid (Nx, Ny, P) (3D) or (Nx, 1, P) (2D, fast axis y),
+// P = Nz (3D) / Ny (2D) rounded up to a multiple of 64, so that one 64-lane wavefront
+// reads one contiguous 256-byte row segment and all components share one index
+// (i*Ny + j)*P + k.  Cells outside a component's extent stay zero.  CPML psi
+// auxiliaries exist only in the slabs of their stretch axis where c != 0 (elsewhere
+// psi stays exactly 0 in the reference too), compressed along that axis.
+//
+// Arithmetic: the file is built with -ffp-contract=off and follows the reference's
+// float32 operation order per cell (planar_beads/boundaries.py:186-268,
+// planar_3d/boundaries3d.py:240-362), so the float32 fields agree with the NumPy
+// reference to the last bit except for the float32 sin of the source phase.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <utility>
+#include <string>
+#include <vector>
+
+#include "../../include/lensfdtd.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define LF_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(-2, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+constexpr int kMaxMon = 32;
+constexpr int kBoxBlocks = 64;
+
+// ----------------------------------------------------------------------------- device structs
+struct Strip {            // psi slab of one stretch axis: v < lo -> v, v >= hi -> v-hi+lo
+    float *psi;
+    int lo, hi, len;      // len = lo + (n - hi); psi == nullptr -> no slab
+};
+
+struct Prof {             // 1-D CPML profile on one node family of one axis
+    const float *b, *c, *k, *km1;
+};
+
+__device__ __forceinline__ int strip_pos(const Strip &s, int v) {
+    if (s.psi == nullptr) return -1;
+    if (v < s.lo) return v;
+    if (v >= s.hi) return v - s.hi + s.lo;
+    return -1;
+}
+
+struct Grid {
+    int nx, ny, nz, P;    // logical dims, padded fast axis
+    __device__ __forceinline__ long long at(int i, int j, int k) const {
+        return ((long long)i * ny + j) * P + k;
+    }
+};
+
+struct Excite {           // source pair and stitch drive, applied inside the E kernel
+    int src_comp, src_ix, src_n;          // plane index = (j*nk + k) or j
+    const float *amp, *ph1, *ph2;
+    const double *a1c, *a1s, *a2c, *a2s;  // amp*cos/sin(phase) (FP64 phase mode)
+    int drv_n, drv_math;                  // overlap cells, LF_DRIVE_*
+    const float *wtap;
+    const float2 *drv[3];                 // per E component (or nullptr)
+    int drv_stride[3];                    // plane size of each driven component
+};
+
+struct StepArgs {
+    float e32, wt32;
+    double ssin, scos, env;
+    float dr32, di32;
+    double dr, di;
+};
+
+// ----------------------------------------------------------------------------- excite helpers
+__device__ __forceinline__ float src_term_ref32(float e32, float wt32, float ph, float amp) {
+    return (e32 * sinf(wt32 + ph)) * amp;          // vcsel.py:155 (float32)
+}
+
+__device__ __forceinline__ float apply_source(float v, int comp, int i, int plane, const Excite &x,
+                                              const StepArgs &s, int fp64) {
+    if (x.amp == nullptr || comp != x.src_comp) return v;
+    if (i == x.src_ix) {
+        if (fp64) v = v + (float)(s.ssin * x.a1c[plane] + s.scos * x.a1s[plane]);
+        else v = v + src_term_ref32(s.e32, s.wt32, x.ph1[plane], x.amp[plane]);
+    } else if (i == x.src_ix + 1) {
+        if (fp64) v = v - (float)(s.ssin * x.a2c[plane] + s.scos * x.a2s[plane]);
+        else v = v - src_term_ref32(s.e32, s.wt32, x.ph2[plane], x.amp[plane]);
+    }
+    return v;
+}
+
+__device__ __forceinline__ float apply_drive(float v, int comp, int i, int plane, const Excite &x,
+                                             const StepArgs &s) {
+    if (i >= x.drv_n || x.drv[comp] == nullptr) return v;
+    const float2 D = x.drv[comp][(long long)i * x.drv_stride[comp] + plane];
+    const float w = x.wtap[i];
+    if (x.drv_math == LF_DRIVE_C128) {       // fdtd2d_core.py:1426-1427 (float64 blend)
+        const double din = s.env * ((double)D.x * s.dr - (double)D.y * s.di);
+        const float keep = (1.0f - w) * v;
+        return (float)((double)w * din + (double)keep);
+    }
+    const float re = D.x * s.dr32 - D.y * s.di32;  // fdtd3d_core.py:1627-1630 (float32)
+    return w * (s.e32 * re) + (1.0f - w) * v;
+}
+
+// ============================================================================ 2D TE kernels
+struct TEArgs {
+    Grid g;
+    float *Ez, *Hx, *Hy;
+    const float *ceE, *ceH;
+    float ch;
+    Prof xe, xh, ye, yh;
+    Strip pHxy, pHyx, pEzx, pEzy;
+    int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
+};
+
+__global__ void __launch_bounds__(256) te_h_kernel(TEArgs a) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int Nx = a.g.nx, Ny = a.g.nz;
+    if (i >= Nx || j >= Ny) return;
+    const long long c = (long long)i * a.g.P + j;
+    if (j < Ny - 1) {                               // Hx (Nx, Ny-1): boundaries.py:189-191
+        const float d = a.Ez[c + 1] - a.Ez[c];
+        const int s = strip_pos(a.pHxy, j);
+        float t;
+        if (s >= 0) {
+            float *pp = a.pHxy.psi + (long long)i * a.pHxy.len + s;
+            const float p = a.yh.b[j] * (*pp) + a.yh.c[j] * d;
+            *pp = p;
+            t = d / a.yh.k[j] + p;
+        } else {
+            t = d;
+        }
+        a.Hx[c] = a.Hx[c] - a.ch * t;
+    }
+    if (i < Nx - 1) {                               // Hy (Nx-1, Ny): :192-194
+        const float d = a.Ez[c + a.g.P] - a.Ez[c];
+        const int s = strip_pos(a.pHyx, i);
+        float t;
+        if (s >= 0) {
+            float *pp = a.pHyx.psi + (long long)s * a.g.P + j;
+            const float p = a.xh.b[i] * (*pp) + a.xh.c[i] * d;
+            *pp = p;
+            t = d / a.xh.k[i] + p;
+        } else {
+            t = d;
+        }
+        a.Hy[c] = a.Hy[c] + a.ch * t;
+    }
+}
+
+__global__ void __launch_bounds__(256) te_e_kernel(TEArgs a, Excite x, StepArgs st, int fp64) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int Nx = a.g.nx, Ny = a.g.nz;
+    if (i >= Nx || j >= Ny) return;
+    const long long c = (long long)i * a.g.P + j;
+    float v = a.Ez[c];
+    if (i >= 1 && i <= Nx - 2 && j >= 1 && j <= Ny - 2) {   // boundaries.py:199-205
+        const float dHy = a.Hy[c] - a.Hy[c - a.g.P];
+        const float dHx = a.Hx[c] - a.Hx[c - 1];
+        const int sx = strip_pos(a.pEzx, i);
+        const int sy = strip_pos(a.pEzy, j);
+        float t;
+        if (sx >= 0) {
+            float *pp = a.pEzx.psi + (long long)sx * a.g.P + j;
+            const float p = a.xe.b[i] * (*pp) + a.xe.c[i] * dHy;
+            *pp = p;
+            t = dHy / a.xe.k[i] + p;
+        } else {
+            t = dHy;
+        }
+        if (sy >= 0) {
+            float *pp = a.pEzy.psi + (long long)i * a.pEzy.len + sy;
+            const float p = a.ye.b[j] * (*pp) + a.ye.c[j] * dHx;
+            *pp = p;
+            t = (t - dHx / a.ye.k[j]) - p;
+        } else {
+            t = t - dHx;
+        }
+        v = a.ceE[c] * v + a.ceH[c] * t;
+    }
+    v = apply_source(v, LF_EZ, i, j, x, st, fp64);          // vcsel.py:151-156
+    v = apply_drive(v, LF_EZ, i, j, x, st);                  // fdtd2d_core.py:1462-1463
+    if ((a.pec_xlo && i == 0) || (a.pec_xhi && i == Nx - 1) || (a.pec_ylo && j == 0) ||
+        (a.pec_yhi && j == Ny - 1))
+        v = 0.0f;                                           // boundaries.py:207-215
+    a.Ez[c] = v;
+}
+
+// ============================================================================ 2D TM kernels
+struct TMArgs {
+    Grid g;
+    float *Ex, *Ey, *Hz;
+    const float *cax, *cay, *dax, *day;
+    float ch;
+    Prof xe, xh, ye, yh;
+    Strip pHzx, pHzy, pExy, pEyx;
+    int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
+};
+
+__global__ void __launch_bounds__(256) tm_h_kernel(TMArgs a) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int Nx = a.g.nx, Ny = a.g.nz;
+    if (i >= Nx - 1 || j >= Ny - 1) return;
+    const long long c = (long long)i * a.g.P + j;
+    const float dEyx = a.Ey[c + a.g.P] - a.Ey[c];           // boundaries.py:243-248
+    const float dExy = a.Ex[c + 1] - a.Ex[c];
+    const int sx = strip_pos(a.pHzx, i);
+    const int sy = strip_pos(a.pHzy, j);
+    float tx, ty;
+    if (sx >= 0) {
+        float *pp = a.pHzx.psi + (long long)sx * a.g.P + j;
+        const float p = a.xh.b[i] * (*pp) + a.xh.c[i] * dEyx;
+        *pp = p;
+        tx = dEyx / a.xh.k[i] + p;
+    } else {
+        tx = dEyx;
+    }
+    if (sy >= 0) {
+        float *pp = a.pHzy.psi + (long long)i * a.pHzy.len + sy;
+        const float p = a.yh.b[j] * (*pp) + a.yh.c[j] * dExy;
+        *pp = p;
+        ty = dExy / a.yh.k[j] + p;
+    } else {
+        ty = dExy;
+    }
+    a.Hz[c] = a.Hz[c] - a.ch * (tx - ty);
+}
+
+__global__ void __launch_bounds__(256) tm_e_kernel(TMArgs a, Excite x, StepArgs st, int fp64) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int Nx = a.g.nx, Ny = a.g.nz;
+    if (i >= Nx || j >= Ny) return;
+    const long long c = (long long)i * a.g.P + j;
+    if (i < Nx - 1) {                                       // Ex (Nx-1, Ny)
+        float v = a.Ex[c];
+        if (a.dax) v = v * a.dax[c];                        // fdtd2d_core.py:796
+        if (j >= 1 && j <= Ny - 2) {                        // boundaries.py:253-255
+            const float d = a.Hz[c] - a.Hz[c - 1];
+            const int s = strip_pos(a.pExy, j);
+            float t;
+            if (s >= 0) {
+                float *pp = a.pExy.psi + (long long)i * a.pExy.len + s;
+                const float p = a.ye.b[j] * (*pp) + a.ye.c[j] * d;
+                *pp = p;
+                t = d / a.ye.k[j] + p;
+            } else {
+                t = d;
+            }
+            v = v + a.cax[c] * t;
+        }
+        if ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1)) v = 0.0f;
+        a.Ex[c] = v;
+    }
+    if (j < Ny - 1) {                                       // Ey (Nx, Ny-1)
+        float v = a.Ey[c];
+        if (a.day) v = v * a.day[c];
+        if (i >= 1 && i <= Nx - 2) {                        // boundaries.py:256-258
+            const float d = a.Hz[c] - a.Hz[c - a.g.P];
+            const int s = strip_pos(a.pEyx, i);
+            float t;
+            if (s >= 0) {
+                float *pp = a.pEyx.psi + (long long)s * a.g.P + j;
+                const float p = a.xe.b[i] * (*pp) + a.xe.c[i] * d;
+                *pp = p;
+                t = d / a.xe.k[i] + p;
+            } else {
+                t = d;
+            }
+            v = v + (-a.cay[c]) * t;
+        }
+        v = apply_source(v, LF_EY, i, j, x, st, fp64);
+        v = apply_drive(v, LF_EY, i, j, x, st);
+        if ((a.pec_xlo && i == 0) || (a.pec_xhi && i == Nx - 1)) v = 0.0f;
+        a.Ey[c] = v;
+    }
+}
+
+
+// ---------------------------------------------------------------------------- fused TM step
+// One pass per step: each workgroup owns 256 consecutive y cells and an x chunk of
+// `lx` rows and marches along x.  Hz^{n+1/2}(i) is formed from E^n, then Ex/Ey^{n+1}(i)
+// from Hz^{n+1/2}(i), Hz^{n+1/2}(i, j-1) (LDS, neighbour lane; the lane below the block
+// is recomputed) and Hz^{n+1/2}(i-1) (register).  Old and new fields live in separate
+// buffers (ping-pong), so recomputed halo values always see E^n / H^{n-1/2}; psi of Hz is
+// ping-ponged for the same reason, psi of Ex/Ey is touched by its owner only.
+// Per cell the arithmetic is the one of tm_h_kernel + tm_e_kernel (same float32 order).
+struct TMFArgs {
+    Grid g;
+    const float *Ex0, *Ey0, *Hz0;
+    float *Ex1, *Ey1, *Hz1;
+    const float *cax, *cay, *dax, *day;
+    float ch;
+    Prof xe, xh, ye, yh;
+    Strip pHzx0, pHzy0, pHzx1, pHzy1, pExy, pEyx;
+    int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
+    int lx;
+};
+
+__device__ __forceinline__ float tm_hz_new(const TMFArgs &a, int i, int j, float ey_n, float ey_c,
+                                           float ex_n, float ex_c, bool store) {
+    const long long c = (long long)i * a.g.P + j;
+    const float dEyx = ey_n - ey_c;
+    const float dExy = ex_n - ex_c;
+    float tx, ty;
+    const int sx = strip_pos(a.pHzx0, i);
+    if (sx >= 0) {
+        const long long q = (long long)sx * a.g.P + j;
+        const float p = a.xh.b[i] * a.pHzx0.psi[q] + a.xh.c[i] * dEyx;
+        if (store) a.pHzx1.psi[q] = p;
+        tx = dEyx / a.xh.k[i] + p;
+    } else {
+        tx = dEyx;
+    }
+    const int sy = strip_pos(a.pHzy0, j);
+    if (sy >= 0) {
+        const long long q = (long long)i * a.pHzy0.len + sy;
+        const float p = a.yh.b[j] * a.pHzy0.psi[q] + a.yh.c[j] * dExy;
+        if (store) a.pHzy1.psi[q] = p;
+        ty = dExy / a.yh.k[j] + p;
+    } else {
+        ty = dExy;
+    }
+    return a.Hz0[c] - a.ch * (tx - ty);
+}
+
+__global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, StepArgs st, int fp64) {
+    __shared__ float ex_s[256];
+    __shared__ float hz_s[257];
+    const int t = threadIdx.x;
+    const int j0 = blockIdx.x * 256;
+    const int j = j0 + t;
+    const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
+    const int i0 = blockIdx.y * a.lx;
+    const int i1 = min(i0 + a.lx, Nx);
+    const int ia = i0 > 0 ? i0 - 1 : 0;
+    float hz_prev = 0.0f;
+    for (int i = ia; i < i1; ++i) {
+        const bool own = i >= i0;
+        const long long c = (long long)i * P + j;
+        const float exc = (j < Ny && i < Nx - 1) ? a.Ex0[c] : 0.0f;
+        const float eyc = (j < Ny - 1) ? a.Ey0[c] : 0.0f;
+        const float eyn = (j < Ny - 1 && i < Nx - 1) ? a.Ey0[c + P] : 0.0f;
+        ex_s[t] = exc;
+        __syncthreads();
+        float hz = 0.0f;
+        if (i < Nx - 1 && j < Ny - 1) {
+            const float exn = t < 255 ? ex_s[t + 1] : a.Ex0[c + 1];
+            hz = tm_hz_new(a, i, j, eyn, eyc, exn, exc, own);
+            if (own) a.Hz1[c] = hz;
+        }
+        if (t == 0) {
+            float h = 0.0f;
+            if (j0 >= 1 && i < Nx - 1) {              // lane below the block
+                const long long cm = c - 1;
+                h = tm_hz_new(a, i, j0 - 1, a.Ey0[cm + P], a.Ey0[cm], exc, a.Ex0[cm], false);
+            }
+            hz_s[0] = h;
+        }
+        hz_s[t + 1] = hz;
+        __syncthreads();
+        if (own && j < Ny) {
+            if (i < Nx - 1) {                          // Ex (Nx-1, Ny)
+                float v = exc;
+                if (a.dax) v = v * a.dax[c];
+                if (j >= 1 && j <= Ny - 2) {
+                    const float d = hz - hz_s[t];
+                    const int s = strip_pos(a.pExy, j);
+                    float tt;
+                    if (s >= 0) {
+                        float *pp = a.pExy.psi + (long long)i * a.pExy.len + s;
+                        const float p = a.ye.b[j] * (*pp) + a.ye.c[j] * d;
+                        *pp = p;
+                        tt = d / a.ye.k[j] + p;
+                    } else {
+                        tt = d;
+                    }
+                    v = v + a.cax[c] * tt;
+                }
+                if ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1)) v = 0.0f;
+                a.Ex1[c] = v;
+            }
+            if (j < Ny - 1) {                          // Ey (Nx, Ny-1)
+                float v = eyc;
+                if (a.day) v = v * a.day[c];
+                if (i >= 1 && i <= Nx - 2) {
+                    const float d = hz - hz_prev;
+                    const int s = strip_pos(a.pEyx, i);
+                    float tt;
+                    if (s >= 0) {
+                        float *pp = a.pEyx.psi + (long long)s * P + j;
+                        const float p = a.xe.b[i] * (*pp) + a.xe.c[i] * d;
+                        *pp = p;
+                        tt = d / a.xe.k[i] + p;
+                    } else {
+                        tt = d;
+                    }
+                    v = v + (-a.cay[c]) * tt;
+                }
+                v = apply_source(v, LF_EY, i, j, x, st, fp64);
+                v = apply_drive(v, LF_EY, i, j, x, st);
+                if ((a.pec_xlo && i == 0) || (a.pec_xhi && i == Nx - 1)) v = 0.0f;
+                a.Ey1[c] = v;
+            }
+        }
+        hz_prev = hz;
+        __syncthreads();
+    }
+}
+
+// ============================================================================ 3D kernels
+struct YeeArgs {
+    Grid g;
+    float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+    const float *cex, *cey, *cez, *dax, *day, *daz;
+    float ch;
+    Prof pe[3], ph[3];        // per axis: full-node (E) and half-node (H) profiles
+    // psi slabs: H: Hx_z Hx_y Hy_x Hy_z Hz_y Hz_x ; E: Ex_y Ex_z Ey_z Ey_x Ez_x Ez_y
+    Strip sH[6], sE[6];
+    int pec[6];               // xlo xhi ylo yhi zlo zhi
+};
+
+// psi element of a slab compressed along `axis`
+__device__ __forceinline__ float *psi_at(const Strip &s, int axis, int pos, int i, int j, int k,
+                                         const Grid &g) {
+    if (axis == 0) return s.psi + ((long long)pos * g.ny + j) * g.P + k;
+    if (axis == 1) return s.psi + ((long long)i * s.len + pos) * g.P + k;
+    return s.psi + ((long long)i * g.ny + j) * s.len + pos;
+}
+
+// F += coef * add with add = psi_new + km1 * d  (boundaries3d.py:240-251)
+__device__ __forceinline__ float psi_term(const Strip &s, int axis, int pos, const Prof &pr, int n,
+                                          float d, int i, int j, int k, const Grid &g) {
+    float *pp = psi_at(s, axis, pos, i, j, k, g);
+    float p = (*pp) * pr.b[n];
+    p = p + pr.c[n] * d;
+    *pp = p;
+    return p + pr.km1[n] * d;
+}
+
+__global__ void __launch_bounds__(256) yee_h_kernel(YeeArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    const Grid &g = a.g;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long c = g.at(i, j, k);
+    const long long sj = g.P, si = (long long)g.ny * g.P;
+    const float ch = a.ch;
+    if (j < g.ny - 1 && k < g.nz - 1) {                     // Hx: boundaries3d.py:316-320
+        const float d1 = a.Ey[c + 1] - a.Ey[c];
+        const float d2 = a.Ez[c + sj] - a.Ez[c];
+        float h = a.Hx[c] + ch * (d1 - d2);
+        int s = strip_pos(a.sH[0], k);
+        if (s >= 0) h = h + ch * psi_term(a.sH[0], 2, s, a.ph[2], k, d1, i, j, k, g);
+        s = strip_pos(a.sH[1], j);
+        if (s >= 0) h = h + (-ch) * psi_term(a.sH[1], 1, s, a.ph[1], j, d2, i, j, k, g);
+        a.Hx[c] = h;
+    }
+    if (i < g.nx - 1 && k < g.nz - 1) {                     // Hy: :321-325
+        const float d1 = a.Ez[c + si] - a.Ez[c];
+        const float d2 = a.Ex[c + 1] - a.Ex[c];
+        float h = a.Hy[c] + ch * (d1 - d2);
+        int s = strip_pos(a.sH[2], i);
+        if (s >= 0) h = h + ch * psi_term(a.sH[2], 0, s, a.ph[0], i, d1, i, j, k, g);
+        s = strip_pos(a.sH[3], k);
+        if (s >= 0) h = h + (-ch) * psi_term(a.sH[3], 2, s, a.ph[2], k, d2, i, j, k, g);
+        a.Hy[c] = h;
+    }
+    if (i < g.nx - 1 && j < g.ny - 1) {                     // Hz: :326-330
+        const float d1 = a.Ex[c + sj] - a.Ex[c];
+        const float d2 = a.Ey[c + si] - a.Ey[c];
+        float h = a.Hz[c] + ch * (d1 - d2);
+        int s = strip_pos(a.sH[4], j);
+        if (s >= 0) h = h + ch * psi_term(a.sH[4], 1, s, a.ph[1], j, d1, i, j, k, g);
+        s = strip_pos(a.sH[5], i);
+        if (s >= 0) h = h + (-ch) * psi_term(a.sH[5], 0, s, a.ph[0], i, d2, i, j, k, g);
+        a.Hz[c] = h;
+    }
+}
+
+__global__ void __launch_bounds__(256) yee_e_kernel(YeeArgs a, Excite x, StepArgs st, int fp64,
+                                                    int nk_src) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    const Grid &g = a.g;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long c = g.at(i, j, k);
+    const long long sj = g.P, si = (long long)g.ny * g.P;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    if (i < NX - 1) {                                       // Ex (Nx-1, Ny, Nz)
+        float v = a.Ex[c];
+        if (a.dax) v = v * a.dax[c];                        // boundaries3d.py:333-336
+        if (j >= 1 && j <= NY - 2 && k >= 1 && k <= NZ - 2) {   // :338-343
+            const float d1 = a.Hz[c] - a.Hz[c - sj];
+            const float d2 = a.Hy[c] - a.Hy[c - 1];
+            const float ce = a.cex[c];
+            v = v + ce * (d1 - d2);
+            int s = strip_pos(a.sE[0], j);
+            if (s >= 0) v = v + ce * psi_term(a.sE[0], 1, s, a.pe[1], j, d1, i, j, k, g);
+            s = strip_pos(a.sE[1], k);
+            if (s >= 0) v = v + (-ce) * psi_term(a.sE[1], 2, s, a.pe[2], k, d2, i, j, k, g);
+        }
+        if ((a.pec[2] && j == 0) || (a.pec[3] && j == NY - 1) || (a.pec[4] && k == 0) ||
+            (a.pec[5] && k == NZ - 1))
+            v = 0.0f;                                       // :356-362
+        v = apply_drive(v, LF_EX, i, j * NZ + k, x, st);
+        a.Ex[c] = v;
+    }
+    if (j < NY - 1) {                                       // Ey (Nx, Ny-1, Nz)
+        float v = a.Ey[c];
+        if (a.day) v = v * a.day[c];
+        if (i >= 1 && i <= NX - 2 && k >= 1 && k <= NZ - 2) {   // :344-349
+            const float d1 = a.Hx[c] - a.Hx[c - 1];
+            const float d2 = a.Hz[c] - a.Hz[c - si];
+            const float ce = a.cey[c];
+            v = v + ce * (d1 - d2);
+            int s = strip_pos(a.sE[2], k);
+            if (s >= 0) v = v + ce * psi_term(a.sE[2], 2, s, a.pe[2], k, d1, i, j, k, g);
+            s = strip_pos(a.sE[3], i);
+            if (s >= 0) v = v + (-ce) * psi_term(a.sE[3], 0, s, a.pe[0], i, d2, i, j, k, g);
+        }
+        if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) || (a.pec[4] && k == 0) ||
+            (a.pec[5] && k == NZ - 1))
+            v = 0.0f;
+        const int plane = j * NZ + k;                       // Ey plane (Ny-1, Nz)
+        v = apply_source(v, LF_EY, i, plane, x, st, fp64);  // fdtd3d_core.py:1127
+        v = apply_drive(v, LF_EY, i, plane, x, st);
+        a.Ey[c] = v;
+    }
+    if (k < NZ - 1) {                                       // Ez (Nx, Ny, Nz-1)
+        float v = a.Ez[c];
+        if (a.daz) v = v * a.daz[c];
+        if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2) {   // :350-355
+            const float d1 = a.Hy[c] - a.Hy[c - si];
+            const float d2 = a.Hx[c] - a.Hx[c - sj];
+            const float ce = a.cez[c];
+            v = v + ce * (d1 - d2);
+            int s = strip_pos(a.sE[4], i);
+            if (s >= 0) v = v + ce * psi_term(a.sE[4], 0, s, a.pe[0], i, d1, i, j, k, g);
+            s = strip_pos(a.sE[5], j);
+            if (s >= 0) v = v + (-ce) * psi_term(a.sE[5], 1, s, a.pe[1], j, d2, i, j, k, g);
+        }
+        if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) || (a.pec[2] && j == 0) ||
+            (a.pec[3] && j == NY - 1))
+            v = 0.0f;
+        const int plane = j * (NZ - 1) + k;                 // Ez plane (Ny, Nz-1)
+        v = apply_source(v, LF_EZ, i, plane, x, st, fp64);
+        v = apply_drive(v, LF_EZ, i, plane, x, st);
+        a.Ez[c] = v;
+    }
+    (void)nk_src;
+}
+
+
+// ---------------------------------------------------------------------------- fused 3D step
+// One pass per step.  Workgroup = TY waves; lane = z, wave = y: owns a 64 x TY (z, y)
+// tile of an x chunk of `lx` planes and marches along x.  For plane i: E^n(i) of the tile
+// plus a one-cell rim is staged in LDS, H^{n+1/2}(i) of the tile is formed (Ey/Ez of plane
+// i+1 come from registers loaded for the next plane), the rim H values the E update
+// needs (Hx/Hz at j0-1, Hx/Hy at k0-1) are recomputed, then E^{n+1}(i) is formed with
+// Hy/Hz(i-1) kept in registers.  Old and new fields (and H psi) are separate buffers, so
+// recomputed rim values always read E^n / H^{n-1/2}.  Arithmetic per cell is that of
+// yee_h_kernel + yee_e_kernel (boundaries3d.py:311-362 operation order).
+struct YFArgs {
+    Grid g;
+    const float *Ex0, *Ey0, *Ez0, *Hx0, *Hy0, *Hz0;
+    float *Ex1, *Ey1, *Ez1, *Hx1, *Hy1, *Hz1;
+    const float *cex, *cey, *cez, *dax, *day, *daz;
+    float ch;
+    Prof pe[3], ph[3];
+    Strip sH0[6], sH1[6], sE[6];
+    int pec[6];
+    int lx;
+};
+
+// psi update of an H slab with separate old/new buffers (store only when owned)
+__device__ __forceinline__ float psi_term_pp(const Strip &s0, const Strip &s1, int axis, int pos,
+                                             const Prof &pr, int n, float d, int i, int j, int k,
+                                             const Grid &g, bool store) {
+    const float *po = psi_at(s0, axis, pos, i, j, k, g);
+    float p = (*po) * pr.b[n];
+    p = p + pr.c[n] * d;
+    if (store) *psi_at(s1, axis, pos, i, j, k, g) = p;
+    return p + pr.km1[n] * d;
+}
+
+__device__ __forceinline__ float yf_hx(const YFArgs &a, int i, int j, int k, float d1, float d2,
+                                       bool store) {
+    float h = a.Hx0[a.g.at(i, j, k)] + a.ch * (d1 - d2);
+    int s = strip_pos(a.sH0[0], k);
+    if (s >= 0) h = h + a.ch * psi_term_pp(a.sH0[0], a.sH1[0], 2, s, a.ph[2], k, d1, i, j, k, a.g, store);
+    s = strip_pos(a.sH0[1], j);
+    if (s >= 0) h = h + (-a.ch) * psi_term_pp(a.sH0[1], a.sH1[1], 1, s, a.ph[1], j, d2, i, j, k, a.g, store);
+    return h;
+}
+
+__device__ __forceinline__ float yf_hy(const YFArgs &a, int i, int j, int k, float d1, float d2,
+                                       bool store) {
+    float h = a.Hy0[a.g.at(i, j, k)] + a.ch * (d1 - d2);
+    int s = strip_pos(a.sH0[2], i);
+    if (s >= 0) h = h + a.ch * psi_term_pp(a.sH0[2], a.sH1[2], 0, s, a.ph[0], i, d1, i, j, k, a.g, store);
+    s = strip_pos(a.sH0[3], k);
+    if (s >= 0) h = h + (-a.ch) * psi_term_pp(a.sH0[3], a.sH1[3], 2, s, a.ph[2], k, d2, i, j, k, a.g, store);
+    return h;
+}
+
+__device__ __forceinline__ float yf_hz(const YFArgs &a, int i, int j, int k, float d1, float d2,
+                                       bool store) {
+    float h = a.Hz0[a.g.at(i, j, k)] + a.ch * (d1 - d2);
+    int s = strip_pos(a.sH0[4], j);
+    if (s >= 0) h = h + a.ch * psi_term_pp(a.sH0[4], a.sH1[4], 1, s, a.ph[1], j, d1, i, j, k, a.g, store);
+    s = strip_pos(a.sH0[5], i);
+    if (s >= 0) h = h + (-a.ch) * psi_term_pp(a.sH0[5], a.sH1[5], 0, s, a.ph[0], i, d2, i, j, k, a.g, store);
+    return h;
+}
+
+template <int TY>
+__global__ void __launch_bounds__(64 * TY) yee_fused_kernel(YFArgs a, Excite x, StepArgs st, int fp64) {
+    constexpr int TZ = 64;
+    constexpr int NRIM = 2 * (TZ + 2) + 2 * TY;
+    __shared__ float se[3][TY + 2][TZ + 2];      // E^n(i): local (j - j0 + 1, k - k0 + 1)
+    __shared__ float sh[3][TY + 1][TZ + 1];      // H^{n+1/2}(i): local (j - j0 + 1, k - k0 + 1)
+    const int lane = threadIdx.x, w = threadIdx.y, tid = w * 64 + lane;
+    const Grid &g = a.g;
+    const int NX = g.nx, NY = g.ny, NZ = g.nz;
+    const int k0 = blockIdx.x * TZ, j0 = blockIdx.y * TY;
+    const int k = k0 + lane, j = j0 + w;
+    const int jl = w + 1, kl = lane + 1;
+    const bool inb = j < NY && k < NZ;
+    const long long si = (long long)NY * g.P;
+    const int i0 = blockIdx.z * a.lx;
+    const int i1 = min(i0 + a.lx, NX);
+    const int ia = i0 > 0 ? i0 - 1 : 0;
+    float eyc = 0.f, ezc = 0.f, hy_prev = 0.f, hz_prev = 0.f;
+    for (int i = ia; i < i1; ++i) {
+        const bool own = i >= i0;
+        const long long c = inb ? g.at(i, j, k) : 0;
+        const float exc = (inb && i < NX - 1) ? a.Ex0[c] : 0.f;
+        if (i == ia) {
+            eyc = (inb && j < NY - 1) ? a.Ey0[c] : 0.f;
+            ezc = (inb && k < NZ - 1) ? a.Ez0[c] : 0.f;
+        }
+        const bool nxt = i + 1 < NX;
+        const float eyn = (inb && nxt && j < NY - 1) ? a.Ey0[c + si] : 0.f;
+        const float ezn = (inb && nxt && k < NZ - 1) ? a.Ez0[c + si] : 0.f;
+        se[0][jl][kl] = exc;
+        se[1][jl][kl] = eyc;
+        se[2][jl][kl] = ezc;
+        for (int h = tid; h < NRIM; h += 64 * TY) {        // E rim of plane i
+            int jj, kk;
+            if (h < TZ + 2) { jj = 0; kk = h; }
+            else if (h < 2 * (TZ + 2)) { jj = TY + 1; kk = h - (TZ + 2); }
+            else if (h < 2 * (TZ + 2) + TY) { jj = 1 + h - 2 * (TZ + 2); kk = 0; }
+            else { jj = 1 + h - 2 * (TZ + 2) - TY; kk = TZ + 1; }
+            const int jg = j0 - 1 + jj, kg = k0 - 1 + kk;
+            const bool ok = jg >= 0 && jg < NY && kg >= 0 && kg < NZ;
+            const long long cc = ok ? g.at(i, jg, kg) : 0;
+            se[0][jj][kk] = (ok && i < NX - 1) ? a.Ex0[cc] : 0.f;
+            se[1][jj][kk] = (ok && jg < NY - 1) ? a.Ey0[cc] : 0.f;
+            se[2][jj][kk] = (ok && kg < NZ - 1) ? a.Ez0[cc] : 0.f;
+        }
+        __syncthreads();
+        float hx = 0.f, hy = 0.f, hz = 0.f;
+        if (inb) {
+            if (j < NY - 1 && k < NZ - 1) {                                  // Hx
+                hx = yf_hx(a, i, j, k, se[1][jl][kl + 1] - eyc, se[2][jl + 1][kl] - ezc, own);
+                if (own) a.Hx1[c] = hx;
+            }
+            if (i < NX - 1 && k < NZ - 1) {                                  // Hy
+                hy = yf_hy(a, i, j, k, ezn - ezc, se[0][jl][kl + 1] - exc, own);
+                if (own) a.Hy1[c] = hy;
+            }
+            if (i < NX - 1 && j < NY - 1) {                                  // Hz
+                hz = yf_hz(a, i, j, k, se[0][jl + 1][kl] - exc, eyn - eyc, own);
+                if (own) a.Hz1[c] = hz;
+            }
+        }
+        sh[0][jl][kl] = hx;
+        sh[1][jl][kl] = hy;
+        sh[2][jl][kl] = hz;
+        if (own) {                                         // H rim: Hx,Hz at j0-1; Hx,Hy at k0-1
+            for (int h = tid; h < TZ + TY; h += 64 * TY) {
+                if (h < TZ) {
+                    const int kk = h + 1, kg = k0 + h, jg = j0 - 1;
+                    float vx = 0.f, vz = 0.f;
+                    if (jg >= 0 && kg < NZ) {
+                        const float ex0 = se[0][0][kk], ey0 = se[1][0][kk], ez0 = se[2][0][kk];
+                        if (kg < NZ - 1)
+                            vx = yf_hx(a, i, jg, kg, se[1][0][kk + 1] - ey0, se[2][1][kk] - ez0, false);
+                        if (i < NX - 1) {
+                            const float eyn_r = a.Ey0[g.at(i + 1, jg, kg)];
+                            vz = yf_hz(a, i, jg, kg, se[0][1][kk] - ex0, eyn_r - ey0, false);
+                        }
+                    }
+                    sh[0][0][kk] = vx;
+                    sh[2][0][kk] = vz;
+                } else {
+                    const int jj = h - TZ + 1, jg = j0 + h - TZ, kg = k0 - 1;
+                    float vx = 0.f, vy = 0.f;
+                    if (kg >= 0 && jg < NY) {
+                        const float ex0 = se[0][jj][0], ey0 = se[1][jj][0], ez0 = se[2][jj][0];
+                        if (jg < NY - 1)
+                            vx = yf_hx(a, i, jg, kg, se[1][jj][1] - ey0, se[2][jj + 1][0] - ez0, false);
+                        if (i < NX - 1) {
+                            const float ezn_r = a.Ez0[g.at(i + 1, jg, kg)];
+                            vy = yf_hy(a, i, jg, kg, ezn_r - ez0, se[0][jj][1] - ex0, false);
+                        }
+                    }
+                    sh[0][jj][0] = vx;
+                    sh[1][jj][0] = vy;
+                }
+            }
+        }
+        __syncthreads();
+        if (own && inb) {
+            if (i < NX - 1) {                                                // Ex
+                float v = exc;
+                if (a.dax) v = v * a.dax[c];
+                if (j >= 1 && j <= NY - 2 && k >= 1 && k <= NZ - 2) {
+                    const float d1 = hz - sh[2][jl - 1][kl];
+                    const float d2 = hy - sh[1][jl][kl - 1];
+                    const float ce = a.cex[c];
+                    v = v + ce * (d1 - d2);
+                    int s = strip_pos(a.sE[0], j);
+                    if (s >= 0) v = v + ce * psi_term(a.sE[0], 1, s, a.pe[1], j, d1, i, j, k, g);
+                    s = strip_pos(a.sE[1], k);
+                    if (s >= 0) v = v + (-ce) * psi_term(a.sE[1], 2, s, a.pe[2], k, d2, i, j, k, g);
+                }
+                if ((a.pec[2] && j == 0) || (a.pec[3] && j == NY - 1) || (a.pec[4] && k == 0) ||
+                    (a.pec[5] && k == NZ - 1))
+                    v = 0.0f;
+                v = apply_drive(v, LF_EX, i, j * NZ + k, x, st);
+                a.Ex1[c] = v;
+            }
+            if (j < NY - 1) {                                                // Ey
+                float v = eyc;
+                if (a.day) v = v * a.day[c];
+                if (i >= 1 && i <= NX - 2 && k >= 1 && k <= NZ - 2) {
+                    const float d1 = hx - sh[0][jl][kl - 1];
+                    const float d2 = hz - hz_prev;
+                    const float ce = a.cey[c];
+                    v = v + ce * (d1 - d2);
+                    int s = strip_pos(a.sE[2], k);
+                    if (s >= 0) v = v + ce * psi_term(a.sE[2], 2, s, a.pe[2], k, d1, i, j, k, g);
+                    s = strip_pos(a.sE[3], i);
+                    if (s >= 0) v = v + (-ce) * psi_term(a.sE[3], 0, s, a.pe[0], i, d2, i, j, k, g);
+                }
+                if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) || (a.pec[4] && k == 0) ||
+                    (a.pec[5] && k == NZ - 1))
+                    v = 0.0f;
+                const int plane = j * NZ + k;
+                v = apply_source(v, LF_EY, i, plane, x, st, fp64);
+                v = apply_drive(v, LF_EY, i, plane, x, st);
+                a.Ey1[c] = v;
+            }
+            if (k < NZ - 1) {                                                // Ez
+                float v = ezc;
+                if (a.daz) v = v * a.daz[c];
+                if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2) {
+                    const float d1 = hy - hy_prev;
+                    const float d2 = hx - sh[0][jl - 1][kl];
+                    const float ce = a.cez[c];
+                    v = v + ce * (d1 - d2);
+                    int s = strip_pos(a.sE[4], i);
+                    if (s >= 0) v = v + ce * psi_term(a.sE[4], 0, s, a.pe[0], i, d1, i, j, k, g);
+                    s = strip_pos(a.sE[5], j);
+                    if (s >= 0) v = v + (-ce) * psi_term(a.sE[5], 1, s, a.pe[1], j, d2, i, j, k, g);
+                }
+                if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) || (a.pec[2] && j == 0) ||
+                    (a.pec[3] && j == NY - 1))
+                    v = 0.0f;
+                const int plane = j * (NZ - 1) + k;
+                v = apply_source(v, LF_EZ, i, plane, x, st, fp64);
+                v = apply_drive(v, LF_EZ, i, plane, x, st);
+                a.Ez1[c] = v;
+            }
+        }
+        hy_prev = hy;
+        hz_prev = hz;
+        eyc = eyn;
+        ezc = ezn;
+        __syncthreads();
+    }
+}
+
+// ============================================================================ monitors
+struct FieldView {          // one component in the padded grid
+    const float *p;
+    int nx, ny, nz;         // component extent (2D: ny = 1, nz = Ny)
+};
+
+struct MonArgs {
+    Grid g;
+    const float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+    int kind;               // LF_TE2D ...
+};
+
+// 2D node-grid helpers (fdtd2d_core.py:458-486); 2D stores (i, 0, j) with Grid.nz = Ny.
+__device__ __forceinline__ float node_ey(const MonArgs &m, int i, int j) {
+    const int Ny = m.g.nz;
+    const long long r = (long long)i * m.g.P;
+    if (j == 0) return m.Ey[r];
+    if (j == Ny - 1) return m.Ey[r + Ny - 2];
+    return 0.5f * (m.Ey[r + j - 1] + m.Ey[r + j]);
+}
+
+__device__ __forceinline__ float node_ex(const MonArgs &m, int i, int j) {
+    const int Nx = m.g.nx;
+    if (i == 0) return m.Ex[j];
+    if (i == Nx - 1) return m.Ex[(long long)(Nx - 2) * m.g.P + j];
+    return 0.5f * (m.Ex[(long long)(i - 1) * m.g.P + j] + m.Ex[(long long)i * m.g.P + j]);
+}
+
+__device__ __forceinline__ float node_ex2(const MonArgs &m, int i, int j) {
+    const int Nx = m.g.nx;
+    if (i == 0) { const float s = m.Ex[j]; return s * s; }
+    if (i == Nx - 1) { const float s = m.Ex[(long long)(Nx - 2) * m.g.P + j]; return s * s; }
+    const float a = m.Ex[(long long)(i - 1) * m.g.P + j];
+    const float b = m.Ex[(long long)i * m.g.P + j];
+    return 0.5f * (a * a + b * b);
+}
+
+__device__ __forceinline__ float intensity2d(const MonArgs &m, int i, int j) {
+    if (m.kind == LF_TE2D) {
+        const float e = m.Ez[(long long)i * m.g.P + j];
+        return e * e;
+    }
+    const float f = node_ey(m, i, j);
+    return f * f + node_ex2(m, i, j);                       // fdtd2d_core.py:812
+}
+
+// <|E|^2> accumulated on the driven component's grid (fdtd3d_core.py:438-474)
+__device__ float sq3d(const MonArgs &m, int form, int nxa, int i, int j, int k) {
+    const Grid &g = m.g;
+    const int NY = g.ny, NZ = g.nz;
+    const int nF = nxa - 1;
+    int a0 = i - 1; a0 = a0 < 0 ? 0 : (a0 > nF - 1 ? nF - 1 : a0);
+    int b0 = i;     b0 = b0 < 0 ? 0 : (b0 > nF - 1 ? nF - 1 : b0);
+    auto exs = [&](int jj, int kk) {                        // axis-0 averaged Ex^2
+        const float u = m.Ex[g.at(a0, jj, kk)];
+        const float w = m.Ex[g.at(b0, jj, kk)];
+        return 0.5f * (u * u + w * w);
+    };
+    auto sq = [&](const float *F, int jj, int kk) {
+        const float u = F[g.at(i, jj, kk)];
+        return u * u;
+    };
+    float tx, ty, tz;
+    if (form == LF_SQ_3D_S) {                               // target Ez grid (Nx, Ny, Nz-1)
+        tx = 0.5f * (exs(j, k) + exs(j, k + 1));
+        auto eyT = [&](int kk) {
+            if (j == 0) return sq(m.Ey, 0, kk);
+            if (j == NY - 1) return sq(m.Ey, NY - 2, kk);
+            return 0.5f * (sq(m.Ey, j - 1, kk) + sq(m.Ey, j, kk));
+        };
+        ty = 0.5f * (eyT(k) + eyT(k + 1));
+        tz = sq(m.Ez, j, k);
+    } else {                                                // target Ey grid (Nx, Ny-1, Nz)
+        tx = 0.5f * (exs(j, k) + exs(j + 1, k));
+        ty = sq(m.Ey, j, k);
+        auto ezM = [&](int kk) { return 0.5f * (sq(m.Ez, j, kk) + sq(m.Ez, j + 1, kk)); };
+        if (k == 0) tz = ezM(0);
+        else if (k == NZ - 1) tz = ezM(NZ - 2);
+        else tz = 0.5f * (ezM(k - 1) + ezM(k));
+    }
+    return (tx + ty) + tz;
+}
+
+__global__ void __launch_bounds__(256) sq_kernel(MonArgs m, float *acc, int form, int nxa, int n1,
+                                                 int n2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= n2 || j >= n1 || i >= nxa) return;
+    float add;
+    if (form == LF_SQ_TE || form == LF_SQ_TM) add = intensity2d(m, i, k);
+    else if (form == LF_SQ_TM_EX2) add = node_ex2(m, i, k);
+    else add = sq3d(m, form, nxa, i, j, k);
+    const long long o = ((long long)i * n1 + j) * n2 + k;
+    acc[o] = acc[o] + add;
+}
+
+__device__ __forceinline__ float dft_src(const MonArgs &m, int src, int i, int j, int k) {
+    if (src == LF_SRC_EY_NODE) return node_ey(m, i, k);
+    if (src == LF_SRC_EX_NODE) return node_ex(m, i, k);
+    const float *F = src == LF_SRC_EX ? m.Ex : (src == LF_SRC_EY ? m.Ey : m.Ez);
+    return F[m.g.at(i, j, k)];
+}
+
+struct DftArgs {
+    int src, x0, kplane, acc, phasor;
+    int n0, n1, n2;          // output extent (rows, mid, fast); kplane: (rows, 1, n2=ny)
+    void *buf;
+};
+
+__global__ void __launch_bounds__(256) dft_kernel(MonArgs m, DftArgs d, double pr, double pi,
+                                                  float fr, float fi) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int r = blockIdx.z;
+    if (q >= d.n2 || j >= d.n1 || r >= d.n0) return;
+    const int i = r + d.x0;
+    float F;
+    if (d.kplane >= 0) F = dft_src(m, d.src, i, q, d.kplane);   // plane (x, j) at fixed k
+    else F = dft_src(m, d.src, i, j, q);
+    const long long o = ((long long)r * d.n1 + j) * d.n2 + q;
+    if (d.acc == LF_ACC_C128) {
+        double2 *a = (double2 *)d.buf;
+        double2 v = a[o];
+        v.x = v.x + (double)F * pr;
+        v.y = v.y + (double)F * pi;
+        a[o] = v;
+    } else if (d.acc == LF_ACC_C64P) {                      // complex128 promotion (NumPy 2)
+        float2 *a = (float2 *)d.buf;
+        float2 v = a[o];
+        v.x = (float)((double)v.x + (double)F * pr);
+        v.y = (float)((double)v.y + (double)F * pi);
+        a[o] = v;
+    } else {                                                // complex64 arithmetic
+        float2 *a = (float2 *)d.buf;
+        float2 v = a[o];
+        v.x = v.x + F * fr;
+        v.y = v.y + F * fi;
+        a[o] = v;
+    }
+}
+
+struct FluxArgs {
+    int il, npair, wide;
+    int ecomp[2], hcomp[2];
+    int n1[2], n2[2];        // plane extents per pair (2D: n1 = 1)
+    void *aE[2], *aH[2];
+};
+
+__device__ __forceinline__ const float *comp_ptr(const MonArgs &m, int c) {
+    switch (c) {
+        case LF_EX: return m.Ex;
+        case LF_EY: return m.Ey;
+        case LF_EZ: return m.Ez;
+        case LF_HX: return m.Hx;
+        case LF_HY: return m.Hy;
+        default: return m.Hz;
+    }
+}
+
+__global__ void __launch_bounds__(256) flux_kernel(MonArgs m, FluxArgs f, StepArgs st, double pr,
+                                                   double pi, double hr, double hi, float pr32,
+                                                   float pi32, float hr32, float hi32) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int p = blockIdx.z;
+    if (p >= f.npair || j >= f.n1[p] || q >= f.n2[p]) return;
+    const float *E = comp_ptr(m, f.ecomp[p]);
+    const float *H = comp_ptr(m, f.hcomp[p]);
+    const float e = E[m.g.at(f.il, j, q)];
+    const float h = 0.5f * (H[m.g.at(f.il - 1, j, q)] + H[m.g.at(f.il, j, q)]);   // flux.py:42
+    const long long o = (long long)j * f.n2[p] + q;
+    if (f.wide) {
+        double2 *aE = (double2 *)f.aE[p];
+        double2 *aH = (double2 *)f.aH[p];
+        double2 u = aE[o], w = aH[o];
+        u.x = u.x + (double)e * pr; u.y = u.y + (double)e * pi;
+        w.x = w.x + (double)h * hr; w.y = w.y + (double)h * hi;
+        aE[o] = u; aH[o] = w;
+    } else {
+        float2 *aE = (float2 *)f.aE[p];
+        float2 *aH = (float2 *)f.aH[p];
+        float2 u = aE[o], w = aH[o];
+        u.x = u.x + e * pr32; u.y = u.y + e * pi32;
+        w.x = w.x + h * hr32; w.y = w.y + h * hi32;
+        aE[o] = u; aH[o] = w;
+    }
+    (void)st;
+}
+
+struct BoxArgs {
+    int ncomp;
+    int comp[3];
+    int lo[3][3], hi[3][3];
+    double *partial;         // kBoxBlocks running partial sums (deterministic order)
+};
+
+__global__ void __launch_bounds__(256) box_kernel(MonArgs m, BoxArgs b) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int cc = 0; cc < b.ncomp; ++cc) {
+        const int n0 = b.hi[cc][0] - b.lo[cc][0];
+        const int n1 = b.hi[cc][1] - b.lo[cc][1];
+        const int n2 = b.hi[cc][2] - b.lo[cc][2];
+        const long long tot = (long long)n0 * n1 * n2;
+        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+             t += (long long)gridDim.x * blockDim.x) {
+            const int q = (int)(t % n2);
+            const long long rest = t / n2;
+            const int j = (int)(rest % n1) + b.lo[cc][1];
+            const int i = (int)(rest / n1) + b.lo[cc][0];
+            const int k = q + b.lo[cc][2];
+            float v;
+            if (b.comp[cc] < 0) {
+                v = intensity2d(m, i, k);                  // 2D: I_E on the node grid
+            } else {
+                const float e = comp_ptr(m, b.comp[cc])[m.g.at(i, j, k)];
+                v = e * e;
+            }
+            s += (double)v;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) b.partial[blockIdx.x] += red[0];
+}
+
+__global__ void scatter_kernel(float *dst, const float *src, int nx, int ny, int nz, int NY, int P) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)nx * ny * nz;
+    if (t >= tot) return;
+    const int k = (int)(t % nz);
+    const long long r = t / nz;
+    const int j = (int)(r % ny);
+    const int i = (int)(r / ny);
+    dst[((long long)i * NY + j) * P + k] = src[t];
+}
+
+__global__ void gather_kernel(float *dst, const float *src, int nx, int ny, int nz, int NY, int P) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)nx * ny * nz;
+    if (t >= tot) return;
+    const int k = (int)(t % nz);
+    const long long r = t / nz;
+    const int j = (int)(r % ny);
+    const int i = (int)(r / ny);
+    dst[t] = src[((long long)i * NY + j) * P + k];
+}
+
+}  // namespace
+
+// ============================================================================ host context
+struct Monitor {
+    int type = -1;           // 0 box, 1 sq, 2 dft, 3 flux
+    void *buf = nullptr;
+    size_t bytes = 0;
+    int form = 0, nxa = 0, n0 = 0, n1 = 0, n2 = 0;
+    DftArgs dft{};
+    FluxArgs flux{};
+    BoxArgs box{};
+};
+
+struct lf_ctx {
+    int kind = 0, mode = 0;
+    int nx = 0, ny = 0, nz = 0;          // logical dims (2D: nz = 1)
+    Grid g{};                            // padded grid; 2D stored as (nx, 1, ny)
+    hipStream_t stream = nullptr;
+    float *arr[LF_NARRAYS] = {};
+    float *alt[LF_NARRAYS] = {};         // second buffer of ping-pong fields (fused path)
+    int variant = 0;                     // 0: single-pass kernels where available, 1: H then E
+    int lx = 32;                         // x rows per workgroup in the single-pass kernels
+    float *psi_alt[4] = {};              // ping-pong psi of the H slabs (2D fused)
+    float *psi_alt3[6] = {};             // ping-pong psi of the H slabs (3D fused)
+    int ty = 8;                          // y rows (waves) per workgroup of the 3D single pass
+    float ch = 0.f;
+    // profiles per axis: e (k,b,c,km1), h (k,b,c,km1)
+    float *prof_e[3][4] = {}, *prof_h[3][4] = {};
+    std::vector<float> host_prof_e[3][4], host_prof_h[3][4];
+    int have_prof[3] = {0, 0, 0};
+    int pec[3][2] = {};
+    std::vector<float *> psi_bufs;
+    Strip sH[6] = {}, sE[6] = {};        // 3D
+    Strip s2[4] = {};                    // 2D: TE Hxy Hyx Ezx Ezy / TM Hzx Hzy Exy Eyx
+    // excitation
+    Excite ex{};
+    std::vector<void *> ex_bufs;
+    int nk_src = 0;
+    Monitor mon[kMaxMon];
+    int nmon = 0;
+    bool finalized = false;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double last_update_ms = 0, last_total_ms = 0;
+    int last_steps = 0;
+};
+
+namespace {
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// component extents in the logical grid
+void comp_extent(const lf_ctx *c, int id, int &ex, int &ey, int &ez) {
+    const int NX = c->nx, NY = c->ny, NZ = c->nz;
+    ex = NX; ey = NY; ez = NZ;
+    if (c->kind == LF_YEE3D) {
+        switch (id) {
+            case LF_EX: case LF_CEX: case LF_DAX: ex = NX - 1; break;
+            case LF_EY: case LF_CEY: case LF_DAY: ey = NY - 1; break;
+            case LF_EZ: case LF_CEZ: case LF_DAZ: ez = NZ - 1; break;
+            case LF_HX: ey = NY - 1; ez = NZ - 1; break;
+            case LF_HY: ex = NX - 1; ez = NZ - 1; break;
+            case LF_HZ: ex = NX - 1; ey = NY - 1; break;
+            default: break;
+        }
+    } else {
+        // 2D stored as (NX, 1, NY): y extent is "ez"
+        int a = NX, b = NY;
+        switch (id) {
+            case LF_HX: b = NY - 1; break;                              // TE Hx (Nx, Ny-1)
+            case LF_HY: a = NX - 1; break;                              // TE Hy (Nx-1, Ny)
+            case LF_EX: case LF_CAX: case LF_DAX: a = NX - 1; break;    // TM Ex (Nx-1, Ny)
+            case LF_EY: case LF_CAY: case LF_DAY: b = NY - 1; break;    // TM Ey (Nx, Ny-1)
+            case LF_HZ: a = NX - 1; b = NY - 1; break;                  // TM Hz
+            default: break;
+        }
+        ex = a; ey = 1; ez = b;
+    }
+}
+
+size_t grid_elems(const lf_ctx *c) { return (size_t)c->g.nx * c->g.ny * c->g.P; }
+
+int ensure_array(lf_ctx *c, int id) {
+    if (c->arr[id]) return 0;
+    LF_HIP(hipMalloc(&c->arr[id], grid_elems(c) * sizeof(float)));
+    LF_HIP(hipMemsetAsync(c->arr[id], 0, grid_elems(c) * sizeof(float), c->stream));
+    return 0;
+}
+
+Prof prof_of(const lf_ctx *c, int axis, bool half) {
+    float *const *p = half ? c->prof_h[axis] : c->prof_e[axis];
+    return Prof{p[1], p[2], p[0], p[3]};
+}
+
+// Slab extent of a profile over the index range [first, last] that the update touches:
+// indices < lo or >= hi carry psi (c != 0 there); lo = hi = n when the slabs meet
+// (boundaries3d.py:216-224 scans the sliced profile the same way).
+void strip_range(const std::vector<float> &cprof, int first, int last, int &lo, int &hi) {
+    const int N = (int)cprof.size();
+    const int span = last - first + 1;
+    int nlo = 0, nhi = 0;
+    while (nlo < span && cprof[first + nlo] != 0.0f) ++nlo;
+    while (nhi < span && cprof[last - nhi] != 0.0f) ++nhi;
+    if (nlo + nhi >= span) { lo = N; hi = N; return; }
+    lo = first + nlo;
+    hi = last + 1 - nhi;
+}
+
+// allocate a psi slab compressed along `axis` over a field grid of dims (a0, a1, a2);
+// positions: v < lo -> v, v >= hi -> v - hi + lo
+int make_strip(lf_ctx *c, Strip &s, const std::vector<float> &cprof, int first, int last, int axis,
+               long long a0, long long a1, long long a2) {
+    int lo, hi;
+    strip_range(cprof, first, last, lo, hi);
+    const int N = (int)cprof.size();
+    const int len = lo + (N - hi);
+    s = Strip{nullptr, lo, hi, len};
+    if (lo == first && hi == last + 1) return 0;      // no PML cell along this axis
+    long long dims[3] = {a0, a1, a2};
+    dims[axis] = len;
+    const size_t n = (size_t)(dims[0] * dims[1] * dims[2]);
+    float *p = nullptr;
+    LF_HIP(hipMalloc(&p, n * sizeof(float)));
+    LF_HIP(hipMemsetAsync(p, 0, n * sizeof(float), c->stream));
+    c->psi_bufs.push_back(p);
+    s.psi = p;
+    return 0;
+}
+
+int upload_vec(lf_ctx *c, const void *host, size_t bytes, void **dev) {
+    LF_HIP(hipMalloc(dev, bytes));
+    LF_HIP(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+    c->ex_bufs.push_back(*dev);
+    return 0;
+}
+
+TEArgs te_args(lf_ctx *c) {
+    TEArgs a{};
+    a.g = c->g;
+    a.Ez = c->arr[LF_EZ]; a.Hx = c->arr[LF_HX]; a.Hy = c->arr[LF_HY];
+    a.ceE = c->arr[LF_CEE]; a.ceH = c->arr[LF_CEH];
+    a.ch = c->ch;
+    a.xe = prof_of(c, 0, false); a.xh = prof_of(c, 0, true);
+    a.ye = prof_of(c, 1, false); a.yh = prof_of(c, 1, true);
+    a.pHxy = c->s2[0]; a.pHyx = c->s2[1]; a.pEzx = c->s2[2]; a.pEzy = c->s2[3];
+    a.pec_xlo = c->pec[0][0]; a.pec_xhi = c->pec[0][1];
+    a.pec_ylo = c->pec[1][0]; a.pec_yhi = c->pec[1][1];
+    return a;
+}
+
+TMArgs tm_args(lf_ctx *c) {
+    TMArgs a{};
+    a.g = c->g;
+    a.Ex = c->arr[LF_EX]; a.Ey = c->arr[LF_EY]; a.Hz = c->arr[LF_HZ];
+    a.cax = c->arr[LF_CAX]; a.cay = c->arr[LF_CAY];
+    a.dax = c->arr[LF_DAX]; a.day = c->arr[LF_DAY];
+    a.ch = c->ch;
+    a.xe = prof_of(c, 0, false); a.xh = prof_of(c, 0, true);
+    a.ye = prof_of(c, 1, false); a.yh = prof_of(c, 1, true);
+    a.pHzx = c->s2[0]; a.pHzy = c->s2[1]; a.pExy = c->s2[2]; a.pEyx = c->s2[3];
+    a.pec_xlo = c->pec[0][0]; a.pec_xhi = c->pec[0][1];
+    a.pec_ylo = c->pec[1][0]; a.pec_yhi = c->pec[1][1];
+    return a;
+}
+
+YeeArgs yee_args(lf_ctx *c) {
+    YeeArgs a{};
+    a.g = c->g;
+    a.Ex = c->arr[LF_EX]; a.Ey = c->arr[LF_EY]; a.Ez = c->arr[LF_EZ];
+    a.Hx = c->arr[LF_HX]; a.Hy = c->arr[LF_HY]; a.Hz = c->arr[LF_HZ];
+    a.cex = c->arr[LF_CEX]; a.cey = c->arr[LF_CEY]; a.cez = c->arr[LF_CEZ];
+    a.dax = c->arr[LF_DAX]; a.day = c->arr[LF_DAY]; a.daz = c->arr[LF_DAZ];
+    a.ch = c->ch;
+    for (int ax = 0; ax < 3; ++ax) { a.pe[ax] = prof_of(c, ax, false); a.ph[ax] = prof_of(c, ax, true); }
+    for (int q = 0; q < 6; ++q) { a.sH[q] = c->sH[q]; a.sE[q] = c->sE[q]; }
+    for (int ax = 0; ax < 3; ++ax) { a.pec[2 * ax] = c->pec[ax][0]; a.pec[2 * ax + 1] = c->pec[ax][1]; }
+    return a;
+}
+
+MonArgs mon_args(lf_ctx *c) {
+    MonArgs m{};
+    m.g = c->g;
+    m.Ex = c->arr[LF_EX]; m.Ey = c->arr[LF_EY]; m.Ez = c->arr[LF_EZ];
+    m.Hx = c->arr[LF_HX]; m.Hy = c->arr[LF_HY]; m.Hz = c->arr[LF_HZ];
+    m.kind = c->kind;
+    return m;
+}
+
+int check_ctx(lf_ctx *c) {
+    if (!c) return fail(-1, "null context");
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+const char *lf_last_error(void) { return g_err.c_str(); }
+
+int lf_device_count(int *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { *count = 0; return fail(-2, hipGetErrorString(e)); }
+    *count = n;
+    return 0;
+}
+
+int lf_device_name(char *buf, int len) {
+    hipDeviceProp_t p;
+    LF_HIP(hipGetDeviceProperties(&p, 0));
+    std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return 0;
+}
+
+int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out) {
+    if (!out) return fail(-1, "null out");
+    if (kind < LF_TE2D || kind > LF_YEE3D) return fail(-1, "bad kind");
+    if (mode != LF_MODE_REF32 && mode != LF_MODE_FP64ACC) return fail(-1, "bad mode");
+    if (nx < 3 || ny < 3 || (kind == LF_YEE3D && nz < 3)) return fail(-1, "grid too small");
+    lf_ctx *c = new lf_ctx();
+    c->kind = kind; c->mode = mode;
+    c->nx = nx; c->ny = ny; c->nz = (kind == LF_YEE3D) ? nz : 1;
+    if (kind == LF_YEE3D) c->g = Grid{nx, ny, nz, round_up(nz, 64)};
+    else c->g = Grid{nx, 1, ny, round_up(ny, 64)};
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) { delete c; return fail(-2, hipGetErrorString(e)); }
+    const int comps3[6] = {LF_EX, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ};
+    const int compsTE[3] = {LF_EZ, LF_HX, LF_HY};
+    const int compsTM[3] = {LF_EX, LF_EY, LF_HZ};
+    const int *cs = kind == LF_YEE3D ? comps3 : (kind == LF_TE2D ? compsTE : compsTM);
+    const int nc = kind == LF_YEE3D ? 6 : 3;
+    for (int q = 0; q < nc; ++q) {
+        int r = ensure_array(c, cs[q]);
+        if (r) { lf_destroy(c); return r; }
+    }
+    *out = c;
+    return 0;
+}
+
+int lf_destroy(lf_ctx *c) {
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &p : c->arr) if (p) (void)hipFree(p);
+    for (auto &p : c->alt) if (p) (void)hipFree(p);
+    {   // psi buffers: originals, ping-pong partners (pointers swap every fused step)
+        std::set<float *> psi(c->psi_bufs.begin(), c->psi_bufs.end());
+        for (auto p : c->psi_alt) if (p) psi.insert(p);
+        for (auto p : c->psi_alt3) if (p) psi.insert(p);
+        for (auto &s : c->s2) if (s.psi) psi.insert(s.psi);
+        for (auto &s : c->sH) if (s.psi) psi.insert(s.psi);
+        for (auto &s : c->sE) if (s.psi) psi.insert(s.psi);
+        for (auto p : psi) (void)hipFree(p);
+        c->psi_bufs.clear();
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 4; ++q) {
+            if (c->prof_e[a][q]) (void)hipFree(c->prof_e[a][q]);
+            if (c->prof_h[a][q]) (void)hipFree(c->prof_h[a][q]);
+        }
+    for (auto p : c->ex_bufs) (void)hipFree(p);
+    for (int m = 0; m < c->nmon; ++m) {
+        if (c->mon[m].buf) (void)hipFree(c->mon[m].buf);
+        if (c->mon[m].type == 3)
+            for (int p = 0; p < c->mon[m].flux.npair; ++p) {
+                (void)hipFree(c->mon[m].flux.aE[p]);
+                (void)hipFree(c->mon[m].flux.aH[p]);
+            }
+    }
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    (void)hipGetLastError();   // teardown errors must not surface in a later call
+    return 0;
+}
+
+int lf_upload(lf_ctx *c, int id, const float *host, long long count) {
+    if (int r = check_ctx(c)) return r;
+    if (id < 0 || id >= LF_NARRAYS) return fail(-1, "bad array id");
+    int ex, ey, ez;
+    comp_extent(c, id, ex, ey, ez);
+    if (count != (long long)ex * ey * ez)
+        return fail(-1, "lf_upload: element count " + std::to_string(count) + " != expected " +
+                            std::to_string((long long)ex * ey * ez));
+    if (int r = ensure_array(c, id)) return r;
+    float *tmp = nullptr;
+    LF_HIP(hipMalloc(&tmp, (size_t)count * sizeof(float)));
+    LF_HIP(hipMemcpyAsync(tmp, host, (size_t)count * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const int nb = (int)((count + 255) / 256);
+    scatter_kernel<<<nb, 256, 0, c->stream>>>(c->arr[id], tmp, ex, ey, ez, c->g.ny, c->g.P);
+    LF_HIP(hipGetLastError());
+    LF_HIP(hipStreamSynchronize(c->stream));
+    LF_HIP(hipFree(tmp));
+    return 0;
+}
+
+int lf_download(lf_ctx *c, int id, float *host, long long count) {
+    if (int r = check_ctx(c)) return r;
+    if (id < 0 || id >= LF_NARRAYS || !c->arr[id]) return fail(-1, "array not present");
+    int ex, ey, ez;
+    comp_extent(c, id, ex, ey, ez);
+    if (count != (long long)ex * ey * ez) return fail(-1, "lf_download: element count mismatch");
+    float *tmp = nullptr;
+    LF_HIP(hipMalloc(&tmp, (size_t)count * sizeof(float)));
+    const int nb = (int)((count + 255) / 256);
+    gather_kernel<<<nb, 256, 0, c->stream>>>(tmp, c->arr[id], ex, ey, ez, c->g.ny, c->g.P);
+    LF_HIP(hipGetLastError());
+    LF_HIP(hipMemcpyAsync(host, tmp, (size_t)count * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    LF_HIP(hipStreamSynchronize(c->stream));
+    LF_HIP(hipFree(tmp));
+    return 0;
+}
+
+int lf_set_ch(lf_ctx *c, float ch) {
+    if (int r = check_ctx(c)) return r;
+    c->ch = ch;
+    return 0;
+}
+
+int lf_set_cpml(lf_ctx *c, int axis, const float *ek, const float *eb, const float *ec,
+                const float *hk, const float *hb, const float *hc, int pec_lo, int pec_hi) {
+    if (int r = check_ctx(c)) return r;
+    const int ndim = c->kind == LF_YEE3D ? 3 : 2;
+    if (axis < 0 || axis >= ndim) return fail(-1, "bad axis");
+    if (c->finalized) return fail(-1, "lf_set_cpml after lf_finalize");
+    const int N = axis == 0 ? c->nx : (axis == 1 ? c->ny : c->nz);
+    const float *src_e[3] = {ek, eb, ec}, *src_h[3] = {hk, hb, hc};
+    for (int q = 0; q < 3; ++q) {
+        c->host_prof_e[axis][q].assign(src_e[q], src_e[q] + N);
+        c->host_prof_h[axis][q].assign(src_h[q], src_h[q] + N - 1);
+    }
+    // km1 = 1/kappa - 1 in float32 (boundaries3d.py:231)
+    c->host_prof_e[axis][3].resize(N);
+    c->host_prof_h[axis][3].resize(N - 1);
+    for (int n = 0; n < N; ++n) c->host_prof_e[axis][3][n] = 1.0f / ek[n] - 1.0f;
+    for (int n = 0; n < N - 1; ++n) c->host_prof_h[axis][3][n] = 1.0f / hk[n] - 1.0f;
+    for (int q = 0; q < 4; ++q) {
+        if (c->prof_e[axis][q]) (void)hipFree(c->prof_e[axis][q]);
+        if (c->prof_h[axis][q]) (void)hipFree(c->prof_h[axis][q]);
+        LF_HIP(hipMalloc(&c->prof_e[axis][q], N * sizeof(float)));
+        LF_HIP(hipMalloc(&c->prof_h[axis][q], (N - 1) * sizeof(float)));
+        LF_HIP(hipMemcpyAsync(c->prof_e[axis][q], c->host_prof_e[axis][q].data(), N * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+        LF_HIP(hipMemcpyAsync(c->prof_h[axis][q], c->host_prof_h[axis][q].data(),
+                              (N - 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    LF_HIP(hipStreamSynchronize(c->stream));
+    c->have_prof[axis] = 1;
+    c->pec[axis][0] = pec_lo ? 1 : 0;
+    c->pec[axis][1] = pec_hi ? 1 : 0;
+    return 0;
+}
+
+int lf_finalize(lf_ctx *c) {
+    if (int r = check_ctx(c)) return r;
+    const int ndim = c->kind == LF_YEE3D ? 3 : 2;
+    for (int a = 0; a < ndim; ++a)
+        if (!c->have_prof[a]) return fail(-1, "lf_finalize: CPML profile missing for an axis");
+    if (c->finalized) return 0;
+    const long long NX = c->g.nx, NY = c->g.ny, P = c->g.P;
+    int r = 0;
+    if (c->kind == LF_YEE3D) {
+        // psi slabs compressed along the stretch axis.  H: half-node profiles over all
+        // n-1 nodes; E: full-node profiles over the interior [1, n-2] that the E update
+        // touches (boundaries3d.py:286-300).
+        const int N3[3] = {c->nx, c->ny, c->nz};
+        auto H = [&](Strip &s, int ax) {
+            return make_strip(c, s, c->host_prof_h[ax][2], 0, N3[ax] - 2, ax, NX, NY, P);
+        };
+        auto E = [&](Strip &s, int ax) {
+            return make_strip(c, s, c->host_prof_e[ax][2], 1, N3[ax] - 2, ax, NX, NY, P);
+        };
+        // Hx_z Hx_y Hy_x Hy_z Hz_y Hz_x
+        if ((r = H(c->sH[0], 2)) || (r = H(c->sH[1], 1)) || (r = H(c->sH[2], 0)) ||
+            (r = H(c->sH[3], 2)) || (r = H(c->sH[4], 1)) || (r = H(c->sH[5], 0)))
+            return r;
+        // Ex_y Ex_z Ey_z Ey_x Ez_x Ez_y
+        if ((r = E(c->sE[0], 1)) || (r = E(c->sE[1], 2)) || (r = E(c->sE[2], 2)) ||
+            (r = E(c->sE[3], 0)) || (r = E(c->sE[4], 0)) || (r = E(c->sE[5], 1)))
+            return r;
+    } else {
+        // 2D grid (NX, 1, P): x slabs compress rows (index s*P + j), y slabs compress
+        // the fast axis (index i*len + s)
+        const int N2[2] = {c->nx, c->ny};
+        auto H = [&](Strip &s, int ax) {
+            return make_strip(c, s, c->host_prof_h[ax][2], 0, N2[ax] - 2, ax == 0 ? 0 : 2, NX, 1, P);
+        };
+        auto E = [&](Strip &s, int ax) {
+            return make_strip(c, s, c->host_prof_e[ax][2], 1, N2[ax] - 2, ax == 0 ? 0 : 2, NX, 1, P);
+        };
+        if (c->kind == LF_TE2D) {   // psi_Hxy psi_Hyx psi_Ezx psi_Ezy (boundaries.py:180-184)
+            if ((r = H(c->s2[0], 1)) || (r = H(c->s2[1], 0)) || (r = E(c->s2[2], 0)) ||
+                (r = E(c->s2[3], 1)))
+                return r;
+        } else {                    // psi_Hzx psi_Hzy psi_Exy psi_Eyx (boundaries.py:235-238)
+            if ((r = H(c->s2[0], 0)) || (r = H(c->s2[1], 1)) || (r = E(c->s2[2], 1)) ||
+                (r = E(c->s2[3], 0)))
+                return r;
+        }
+    }
+    LF_HIP(hipStreamSynchronize(c->stream));
+    c->finalized = true;
+    return 0;
+}
+
+int lf_set_source(lf_ctx *c, int comp, int ix, int n, const float *amp, const float *ph1,
+                  const float *ph2) {
+    if (int r = check_ctx(c)) return r;
+    int ex, ey, ez;
+    comp_extent(c, comp, ex, ey, ez);
+    if (n != ey * ez) return fail(-1, "lf_set_source: plane size mismatch");
+    if (ix < 0 || ix + 1 >= ex) return fail(-1, "lf_set_source: plane index out of range");
+    std::vector<double> a1c(n), a1s(n), a2c(n), a2s(n);
+    for (int q = 0; q < n; ++q) {
+        const double a = amp[q], p1 = ph1[q], p2 = ph2[q];
+        a1c[q] = a * std::cos(p1); a1s[q] = a * std::sin(p1);
+        a2c[q] = a * std::cos(p2); a2s[q] = a * std::sin(p2);
+    }
+    void *d;
+    int r;
+    if ((r = upload_vec(c, amp, n * sizeof(float), &d))) return r; c->ex.amp = (const float *)d;
+    if ((r = upload_vec(c, ph1, n * sizeof(float), &d))) return r; c->ex.ph1 = (const float *)d;
+    if ((r = upload_vec(c, ph2, n * sizeof(float), &d))) return r; c->ex.ph2 = (const float *)d;
+    if ((r = upload_vec(c, a1c.data(), n * sizeof(double), &d))) return r; c->ex.a1c = (const double *)d;
+    if ((r = upload_vec(c, a1s.data(), n * sizeof(double), &d))) return r; c->ex.a1s = (const double *)d;
+    if ((r = upload_vec(c, a2c.data(), n * sizeof(double), &d))) return r; c->ex.a2c = (const double *)d;
+    if ((r = upload_vec(c, a2s.data(), n * sizeof(double), &d))) return r; c->ex.a2s = (const double *)d;
+    c->ex.src_comp = comp;
+    c->ex.src_ix = ix;
+    c->ex.src_n = n;
+    LF_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lf_set_drive(lf_ctx *c, int ncells, const float *wtap, int ncomp, const int *comps,
+                 const float *drives, int math) {
+    if (int r = check_ctx(c)) return r;
+    if (ncells <= 0) return fail(-1, "lf_set_drive: ncells must be > 0");
+    void *d;
+    int r;
+    if ((r = upload_vec(c, wtap, ncells * sizeof(float), &d))) return r;
+    c->ex.wtap = (const float *)d;
+    c->ex.drv_n = ncells;
+    c->ex.drv_math = math;
+    size_t off = 0;
+    for (int q = 0; q < ncomp; ++q) {
+        if (comps[q] < LF_EX || comps[q] > LF_EZ) return fail(-1, "lf_set_drive: not an E component");
+        int ex, ey, ez;
+        comp_extent(c, comps[q], ex, ey, ez);
+        if (ncells > ex) return fail(-1, "lf_set_drive: overlap larger than the component");
+        const int n = ey * ez;
+        if ((r = upload_vec(c, drives + off, (size_t)ncells * n * 2 * sizeof(float), &d))) return r;
+        off += (size_t)ncells * n * 2;
+        c->ex.drv[comps[q]] = (const float2 *)d;
+        c->ex.drv_stride[comps[q]] = n;
+    }
+    LF_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"
+
+
+namespace {
+
+int ensure_alt(lf_ctx *c, int id) {
+    if (c->alt[id]) return 0;
+    LF_HIP(hipMalloc(&c->alt[id], grid_elems(c) * sizeof(float)));
+    LF_HIP(hipMemsetAsync(c->alt[id], 0, grid_elems(c) * sizeof(float), c->stream));
+    return 0;
+}
+
+size_t strip_elems_2d(const lf_ctx *c, const Strip &s, bool x_slab) {
+    return x_slab ? (size_t)s.len * c->g.P : (size_t)c->g.nx * s.len;
+}
+
+// one single-pass TM step: reads arr[], writes alt[], then swaps
+int tm_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
+    if (int r = ensure_alt(c, LF_EX)) return r;
+    if (int r = ensure_alt(c, LF_EY)) return r;
+    if (int r = ensure_alt(c, LF_HZ)) return r;
+    for (int q = 0; q < 2; ++q)                       // psi_Hzx (x slab), psi_Hzy (y slab)
+        if (c->s2[q].psi && !c->psi_alt[q]) {
+            const size_t n = strip_elems_2d(c, c->s2[q], q == 0);
+            LF_HIP(hipMalloc(&c->psi_alt[q], n * sizeof(float)));
+            LF_HIP(hipMemsetAsync(c->psi_alt[q], 0, n * sizeof(float), c->stream));
+        }
+    TMFArgs f{};
+    f.g = c->g;
+    f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Hz0 = c->arr[LF_HZ];
+    f.Ex1 = c->alt[LF_EX]; f.Ey1 = c->alt[LF_EY]; f.Hz1 = c->alt[LF_HZ];
+    f.cax = c->arr[LF_CAX]; f.cay = c->arr[LF_CAY]; f.dax = c->arr[LF_DAX]; f.day = c->arr[LF_DAY];
+    f.ch = c->ch;
+    f.xe = prof_of(c, 0, false); f.xh = prof_of(c, 0, true);
+    f.ye = prof_of(c, 1, false); f.yh = prof_of(c, 1, true);
+    f.pHzx0 = c->s2[0]; f.pHzy0 = c->s2[1];
+    f.pHzx1 = c->s2[0]; f.pHzx1.psi = c->psi_alt[0];
+    f.pHzy1 = c->s2[1]; f.pHzy1.psi = c->psi_alt[1];
+    f.pExy = c->s2[2]; f.pEyx = c->s2[3];
+    f.pec_xlo = c->pec[0][0]; f.pec_xhi = c->pec[0][1];
+    f.pec_ylo = c->pec[1][0]; f.pec_yhi = c->pec[1][1];
+    f.lx = c->lx;
+    dim3 grd((c->g.nz + 255) / 256, (c->g.nx + c->lx - 1) / c->lx, 1);
+    tm_fused_kernel<<<grd, 256, 0, c->stream>>>(f, c->ex, st, fp64);
+    std::swap(c->arr[LF_EX], c->alt[LF_EX]);
+    std::swap(c->arr[LF_EY], c->alt[LF_EY]);
+    std::swap(c->arr[LF_HZ], c->alt[LF_HZ]);
+    for (int q = 0; q < 2; ++q)
+        if (c->s2[q].psi) std::swap(c->s2[q].psi, c->psi_alt[q]);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int lf_set_variant(lf_ctx *c, int variant, int lx) {
+    if (int e = check_ctx(c)) return e;
+    if (variant < 0 || variant > 1) return fail(-1, "lf_set_variant: variant 0 or 1");
+    if (lx < 1 || lx > 4096) return fail(-1, "lf_set_variant: lx out of range");
+    c->variant = variant;
+    c->lx = lx;
+    return 0;
+}
+
+
+namespace {
+
+size_t strip_elems_3d(const lf_ctx *c, const Strip &s, int axis) {
+    long long d[3] = {c->g.nx, c->g.ny, c->g.P};
+    d[axis] = s.len;
+    return (size_t)(d[0] * d[1] * d[2]);
+}
+
+int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
+    const int ids[6] = {LF_EX, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ};
+    for (int q = 0; q < 6; ++q)
+        if (int r = ensure_alt(c, ids[q])) return r;
+    const int hax[6] = {2, 1, 0, 2, 1, 0};            // stretch axis of Hx_z Hx_y Hy_x Hy_z Hz_y Hz_x
+    for (int q = 0; q < 6; ++q)
+        if (c->sH[q].psi && !c->psi_alt3[q]) {
+            const size_t n = strip_elems_3d(c, c->sH[q], hax[q]);
+            LF_HIP(hipMalloc(&c->psi_alt3[q], n * sizeof(float)));
+            LF_HIP(hipMemsetAsync(c->psi_alt3[q], 0, n * sizeof(float), c->stream));
+        }
+    YFArgs f{};
+    f.g = c->g;
+    f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Ez0 = c->arr[LF_EZ];
+    f.Hx0 = c->arr[LF_HX]; f.Hy0 = c->arr[LF_HY]; f.Hz0 = c->arr[LF_HZ];
+    f.Ex1 = c->alt[LF_EX]; f.Ey1 = c->alt[LF_EY]; f.Ez1 = c->alt[LF_EZ];
+    f.Hx1 = c->alt[LF_HX]; f.Hy1 = c->alt[LF_HY]; f.Hz1 = c->alt[LF_HZ];
+    f.cex = c->arr[LF_CEX]; f.cey = c->arr[LF_CEY]; f.cez = c->arr[LF_CEZ];
+    f.dax = c->arr[LF_DAX]; f.day = c->arr[LF_DAY]; f.daz = c->arr[LF_DAZ];
+    f.ch = c->ch;
+    for (int ax = 0; ax < 3; ++ax) { f.pe[ax] = prof_of(c, ax, false); f.ph[ax] = prof_of(c, ax, true); }
+    for (int q = 0; q < 6; ++q) {
+        f.sH0[q] = c->sH[q];
+        f.sH1[q] = c->sH[q];
+        f.sH1[q].psi = c->psi_alt3[q];
+        f.sE[q] = c->sE[q];
+    }
+    for (int ax = 0; ax < 3; ++ax) { f.pec[2 * ax] = c->pec[ax][0]; f.pec[2 * ax + 1] = c->pec[ax][1]; }
+    f.lx = c->lx;
+    const int ty = c->ty;
+    dim3 grd((c->g.nz + 63) / 64, (c->g.ny + ty - 1) / ty, (c->g.nx + c->lx - 1) / c->lx);
+    if (ty == 4) yee_fused_kernel<4><<<grd, dim3(64, 4, 1), 0, c->stream>>>(f, c->ex, st, fp64);
+    else if (ty == 2) yee_fused_kernel<2><<<grd, dim3(64, 2, 1), 0, c->stream>>>(f, c->ex, st, fp64);
+    else yee_fused_kernel<8><<<grd, dim3(64, 8, 1), 0, c->stream>>>(f, c->ex, st, fp64);
+    for (int q = 0; q < 6; ++q) std::swap(c->arr[ids[q]], c->alt[ids[q]]);
+    for (int q = 0; q < 6; ++q)
+        if (c->sH[q].psi) std::swap(c->sH[q].psi, c->psi_alt3[q]);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int lf_set_tile(lf_ctx *c, int ty) {
+    if (int e = check_ctx(c)) return e;
+    if (ty != 2 && ty != 4 && ty != 8) return fail(-1, "lf_set_tile: ty must be 2, 4 or 8");
+    c->ty = ty;
+    return 0;
+}
+
+// ============================================================================ monitors (C ABI)
+namespace {
+
+int new_monitor(lf_ctx *c, int *id, Monitor **m) {
+    if (c->nmon >= kMaxMon) return fail(-1, "at most 32 monitors per context");
+    *id = c->nmon;
+    *m = &c->mon[c->nmon++];
+    return 0;
+}
+
+int alloc_zero(lf_ctx *c, void **p, size_t bytes) {
+    LF_HIP(hipMalloc(p, bytes));
+    LF_HIP(hipMemsetAsync(*p, 0, bytes, c->stream));
+    return 0;
+}
+
+int comp_of_src(int src) {
+    return src == LF_SRC_EX ? LF_EX : (src == LF_SRC_EY ? LF_EY : LF_EZ);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lf_add_box(lf_ctx *c, const int *r, int *id) {
+    if (int e = check_ctx(c)) return e;
+    Monitor *m;
+    if (int e = new_monitor(c, id, &m)) return e;
+    m->type = 0;
+    BoxArgs &b = m->box;
+    if (c->kind == LF_YEE3D) {
+        b.ncomp = 3;
+        for (int q = 0; q < 3; ++q) {
+            b.comp[q] = LF_EX + q;
+            for (int a = 0; a < 3; ++a) { b.lo[q][a] = r[q * 6 + 2 * a]; b.hi[q][a] = r[q * 6 + 2 * a + 1]; }
+        }
+    } else {                                    // 2D: one node-grid box (x, y) -> (i, 0, k)
+        b.ncomp = 1;
+        b.comp[0] = -1;
+        b.lo[0][0] = r[0]; b.hi[0][0] = r[1];
+        b.lo[0][1] = 0;    b.hi[0][1] = 1;
+        b.lo[0][2] = r[2]; b.hi[0][2] = r[3];
+    }
+    for (int q = 0; q < b.ncomp; ++q)
+        for (int a = 0; a < 3; ++a)
+            if (b.hi[q][a] < b.lo[q][a] || b.lo[q][a] < 0) return fail(-1, "lf_add_box: bad range");
+    m->bytes = kBoxBlocks * sizeof(double);
+    if (int e = alloc_zero(c, &m->buf, m->bytes)) return e;
+    b.partial = (double *)m->buf;
+    return 0;
+}
+
+int lf_add_sq(lf_ctx *c, int form, int nx_acc, int *id) {
+    if (int e = check_ctx(c)) return e;
+    Monitor *m;
+    if (int e = new_monitor(c, id, &m)) return e;
+    m->type = 1;
+    m->form = form;
+    if (c->kind == LF_YEE3D) {
+        if (form != LF_SQ_3D_S && form != LF_SQ_3D_P) return fail(-1, "lf_add_sq: 3D form expected");
+        if (nx_acc < 2 || nx_acc > c->nx) return fail(-1, "lf_add_sq: bad nx_acc");
+        m->nxa = nx_acc;
+        m->n1 = form == LF_SQ_3D_S ? c->ny : c->ny - 1;
+        m->n2 = form == LF_SQ_3D_S ? c->nz - 1 : c->nz;
+    } else {
+        if (form > LF_SQ_TM_EX2) return fail(-1, "lf_add_sq: 2D form expected");
+        m->nxa = c->nx;
+        m->n1 = 1;
+        m->n2 = c->ny;
+    }
+    m->n0 = m->nxa;
+    m->bytes = (size_t)m->n0 * m->n1 * m->n2 * sizeof(float);
+    return alloc_zero(c, &m->buf, m->bytes);
+}
+
+int lf_add_dft(lf_ctx *c, int src, int x0, int x1, int kplane, int phasor, int acc, int *id) {
+    if (int e = check_ctx(c)) return e;
+    Monitor *m;
+    if (int e = new_monitor(c, id, &m)) return e;
+    m->type = 2;
+    DftArgs &d = m->dft;
+    d.src = src; d.x0 = x0; d.kplane = kplane; d.acc = acc; d.phasor = phasor;
+    int ex, ey, ez;
+    if (src == LF_SRC_EY_NODE || src == LF_SRC_EX_NODE) {
+        if (c->kind != LF_TM2D) return fail(-1, "node sources are 2D TM only");
+        ex = c->nx; ey = 1; ez = c->ny;
+    } else {
+        comp_extent(c, comp_of_src(src), ex, ey, ez);
+    }
+    if (x0 < 0 || x1 > ex || x1 <= x0) return fail(-1, "lf_add_dft: bad x range");
+    d.n0 = x1 - x0;
+    if (kplane >= 0) {
+        if (c->kind != LF_YEE3D || kplane >= ez) return fail(-1, "lf_add_dft: bad kplane");
+        d.n1 = 1; d.n2 = ey;                   // stored (rows, ey) ; kernel maps q -> j
+    } else {
+        d.n1 = ey; d.n2 = ez;
+    }
+    const size_t el = acc == LF_ACC_C128 ? sizeof(double2) : sizeof(float2);
+    m->bytes = (size_t)d.n0 * d.n1 * d.n2 * el;
+    if (int e = alloc_zero(c, &m->buf, m->bytes)) return e;
+    d.buf = m->buf;
+    return 0;
+}
+
+int lf_add_flux(lf_ctx *c, int il, int *id) {
+    if (int e = check_ctx(c)) return e;
+    if (il < 1 || il >= c->nx - 1) return fail(-1, "lf_add_flux: plane index out of range");
+    Monitor *m;
+    if (int e = new_monitor(c, id, &m)) return e;
+    m->type = 3;
+    FluxArgs &f = m->flux;
+    f.il = il;
+    f.wide = c->mode == LF_MODE_FP64ACC;
+    if (c->kind == LF_TE2D) { f.npair = 1; f.ecomp[0] = LF_EZ; f.hcomp[0] = LF_HY; }
+    else if (c->kind == LF_TM2D) { f.npair = 1; f.ecomp[0] = LF_EY; f.hcomp[0] = LF_HZ; }
+    else { f.npair = 2; f.ecomp[0] = LF_EY; f.hcomp[0] = LF_HZ; f.ecomp[1] = LF_EZ; f.hcomp[1] = LF_HY; }
+    const size_t el = f.wide ? sizeof(double2) : sizeof(float2);
+    m->bytes = 0;
+    for (int p = 0; p < f.npair; ++p) {
+        int ex, ey, ez;
+        comp_extent(c, f.ecomp[p], ex, ey, ez);
+        f.n1[p] = ey; f.n2[p] = ez;
+        const size_t b = (size_t)ey * ez * el;
+        if (int e = alloc_zero(c, &f.aE[p], b)) return e;
+        if (int e = alloc_zero(c, &f.aH[p], b)) return e;
+        m->bytes += 2 * b;
+    }
+    return 0;
+}
+
+int lf_monitor_bytes(lf_ctx *c, int id, long long *bytes) {
+    if (int e = check_ctx(c)) return e;
+    if (id < 0 || id >= c->nmon) return fail(-1, "bad monitor id");
+    *bytes = (long long)c->mon[id].bytes;
+    return 0;
+}
+
+int lf_monitor_read(lf_ctx *c, int id, void *dst, long long bytes) {
+    if (int e = check_ctx(c)) return e;
+    if (id < 0 || id >= c->nmon) return fail(-1, "bad monitor id");
+    Monitor &m = c->mon[id];
+    if ((size_t)bytes != m.bytes) return fail(-1, "lf_monitor_read: size mismatch");
+    LF_HIP(hipStreamSynchronize(c->stream));
+    if (m.type == 3) {                          // pairs: aE0 aH0 [aE1 aH1]
+        char *o = (char *)dst;
+        const size_t el = m.flux.wide ? sizeof(double2) : sizeof(float2);
+        for (int p = 0; p < m.flux.npair; ++p) {
+            const size_t b = (size_t)m.flux.n1[p] * m.flux.n2[p] * el;
+            LF_HIP(hipMemcpy(o, m.flux.aE[p], b, hipMemcpyDeviceToHost)); o += b;
+            LF_HIP(hipMemcpy(o, m.flux.aH[p], b, hipMemcpyDeviceToHost)); o += b;
+        }
+        return 0;
+    }
+    LF_HIP(hipMemcpy(dst, m.buf, m.bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lf_monitor_write(lf_ctx *c, int id, const void *src, long long bytes) {
+    if (int e = check_ctx(c)) return e;
+    if (id < 0 || id >= c->nmon) return fail(-1, "bad monitor id");
+    Monitor &m = c->mon[id];
+    if ((size_t)bytes != m.bytes) return fail(-1, "lf_monitor_write: size mismatch");
+    LF_HIP(hipStreamSynchronize(c->stream));
+    if (m.type == 3) {
+        const char *o = (const char *)src;
+        const size_t el = m.flux.wide ? sizeof(double2) : sizeof(float2);
+        for (int p = 0; p < m.flux.npair; ++p) {
+            const size_t b = (size_t)m.flux.n1[p] * m.flux.n2[p] * el;
+            LF_HIP(hipMemcpy(m.flux.aE[p], o, b, hipMemcpyHostToDevice)); o += b;
+            LF_HIP(hipMemcpy(m.flux.aH[p], o, b, hipMemcpyHostToDevice)); o += b;
+        }
+        return 0;
+    }
+    if (m.type == 0) {                          // box: the value goes to partial[0]
+        std::vector<double> part(kBoxBlocks, 0.0);
+        part[0] = *(const double *)src;
+        LF_HIP(hipMemcpy(m.buf, part.data(), m.bytes, hipMemcpyHostToDevice));
+        return 0;
+    }
+    LF_HIP(hipMemcpy(m.buf, src, m.bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int lf_monitor_zero(lf_ctx *c, int id) {
+    if (int e = check_ctx(c)) return e;
+    if (id < 0 || id >= c->nmon) return fail(-1, "bad monitor id");
+    Monitor &m = c->mon[id];
+    if (m.type == 3) {
+        const size_t el = m.flux.wide ? sizeof(double2) : sizeof(float2);
+        for (int p = 0; p < m.flux.npair; ++p) {
+            const size_t b = (size_t)m.flux.n1[p] * m.flux.n2[p] * el;
+            LF_HIP(hipMemsetAsync(m.flux.aE[p], 0, b, c->stream));
+            LF_HIP(hipMemsetAsync(m.flux.aH[p], 0, b, c->stream));
+        }
+        return 0;
+    }
+    LF_HIP(hipMemsetAsync(m.buf, 0, m.bytes, c->stream));
+    return 0;
+}
+
+int lf_set_timing(lf_ctx *c, int on) {
+    if (int e = check_ctx(c)) return e;
+    c->timing = on != 0;
+    return 0;
+}
+
+int lf_kernel_ms(lf_ctx *c, double *update_ms, double *total_ms, int *nsteps) {
+    if (int e = check_ctx(c)) return e;
+    *update_ms = c->last_update_ms;
+    *total_ms = c->last_total_ms;
+    *nsteps = c->last_steps;
+    return 0;
+}
+
+int lf_sync(lf_ctx *c) {
+    if (int e = check_ctx(c)) return e;
+    LF_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
+    if (int e = check_ctx(c)) return e;
+    if (!c->finalized) return fail(-1, "lf_run before lf_finalize");
+    if (nsteps <= 0) return 0;
+    if (c->ch == 0.0f) return fail(-1, "lf_run: Ch not set");
+    if (c->kind == LF_TE2D && (!c->arr[LF_CEE] || !c->arr[LF_CEH]))
+        return fail(-1, "lf_run: Ce_E/Ce_H not uploaded");
+    if (c->kind == LF_TM2D && (!c->arr[LF_CAX] || !c->arr[LF_CAY]))
+        return fail(-1, "lf_run: Ca_ex/Ca_ey not uploaded");
+    if (c->kind == LF_YEE3D && (!c->arr[LF_CEX] || !c->arr[LF_CEY] || !c->arr[LF_CEZ]))
+        return fail(-1, "lf_run: ce_x/ce_y/ce_z not uploaded");
+    const int fp64 = c->mode == LF_MODE_FP64ACC;
+    const Grid &g = c->g;
+    dim3 blk(64, 4, 1);
+    dim3 grd2((g.nz + 63) / 64, (g.nx + 3) / 4, 1);                    // 2D: (fast y, x)
+    dim3 grd3((g.nz + 63) / 64, (g.ny + 3) / 4, g.nx);                 // 3D: (z, y, x)
+    TEArgs te{}; TMArgs tm{}; YeeArgs yee{};
+    if (c->kind == LF_TE2D) te = te_args(c);
+    else if (c->kind == LF_TM2D) tm = tm_args(c);
+    else yee = yee_args(c);
+    MonArgs ma = mon_args(c);
+    const bool timing = c->timing;
+    if (timing) {
+        while ((int)c->ev.size() < 2 * nsteps + 2) {
+            hipEvent_t e;
+            LF_HIP(hipEventCreate(&e));
+            c->ev.push_back(e);
+        }
+        LF_HIP(hipEventRecord(c->ev[2 * nsteps], c->stream));
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        const lf_step &t = steps[s];
+        StepArgs st{t.e32, t.wt32, t.ssin, t.scos, t.env, t.dr32, t.di32, t.dr, t.di};
+        if (timing) LF_HIP(hipEventRecord(c->ev[2 * s], c->stream));
+        if (c->kind == LF_TE2D) {
+            te_h_kernel<<<grd2, blk, 0, c->stream>>>(te);
+            te_e_kernel<<<grd2, blk, 0, c->stream>>>(te, c->ex, st, fp64);
+        } else if (c->kind == LF_TM2D && c->variant == 0) {
+            if (int e = tm_fused_step(c, st, fp64)) return e;
+            ma = mon_args(c);
+        } else if (c->kind == LF_TM2D) {
+            tm_h_kernel<<<grd2, blk, 0, c->stream>>>(tm);
+            tm_e_kernel<<<grd2, blk, 0, c->stream>>>(tm, c->ex, st, fp64);
+        } else if (c->variant == 0) {
+            if (int e = yee_fused_step(c, st, fp64)) return e;
+            ma = mon_args(c);
+        } else {
+            yee_h_kernel<<<grd3, blk, 0, c->stream>>>(yee);
+            yee_e_kernel<<<grd3, blk, 0, c->stream>>>(yee, c->ex, st, fp64, 0);
+        }
+        if (timing) LF_HIP(hipEventRecord(c->ev[2 * s + 1], c->stream));
+        const unsigned mask = t.mask;
+        for (int id = 0; mask && id < c->nmon; ++id) {
+            if (!((mask >> id) & 1u)) continue;
+            Monitor &m = c->mon[id];
+            if (m.type == 0) {
+                box_kernel<<<kBoxBlocks, 256, 0, c->stream>>>(ma, m.box);
+            } else if (m.type == 1) {
+                dim3 gq((m.n2 + 63) / 64, (m.n1 + 3) / 4, m.n0);
+                sq_kernel<<<gq, blk, 0, c->stream>>>(ma, (float *)m.buf, m.form, m.nxa, m.n1, m.n2);
+            } else if (m.type == 2) {
+                const DftArgs &d = m.dft;
+                dim3 gq((d.n2 + 63) / 64, (d.n1 + 3) / 4, d.n0);
+                const float fr = d.phasor == LF_PH_Q ? t.qr32 : t.pr32;
+                const float fi = d.phasor == LF_PH_Q ? t.qi32 : t.pi32;
+                dft_kernel<<<gq, blk, 0, c->stream>>>(ma, d, t.pr, t.pi, fr, fi);
+            } else {
+                const FluxArgs &f = m.flux;
+                const int n1 = f.n1[0] > (f.npair > 1 ? f.n1[1] : 0) ? f.n1[0] : f.n1[1];
+                const int n2 = f.n2[0] > (f.npair > 1 ? f.n2[1] : 0) ? f.n2[0] : f.n2[1];
+                dim3 gq((n2 + 255) / 256, n1, f.npair);
+                flux_kernel<<<gq, 256, 0, c->stream>>>(ma, f, st, t.pr, t.pi, t.hr, t.hi, t.pr32,
+                                                       t.pi32, t.hr32, t.hi32);
+            }
+        }
+    }
+    LF_HIP(hipGetLastError());
+    if (timing) {
+        LF_HIP(hipEventRecord(c->ev[2 * nsteps + 1], c->stream));
+        LF_HIP(hipEventSynchronize(c->ev[2 * nsteps + 1]));
+        double upd = 0.0;
+        for (int s = 0; s < nsteps; ++s) {
+            float ms = 0.f;
+            LF_HIP(hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]));
+            upd += ms;
+        }
+        float tot = 0.f;
+        LF_HIP(hipEventElapsedTime(&tot, c->ev[2 * nsteps], c->ev[2 * nsteps + 1]));
+        c->last_update_ms = upd;
+        c->last_total_ms = tot;
+        c->last_steps = nsteps;
+    }
+    return 0;
+}
+
+}  // extern "C"

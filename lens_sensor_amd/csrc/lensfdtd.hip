@@ -107,12 +107,15 @@ __device__ __forceinline__ float apply_drive(float v, int comp, int i, int plane
     if (i >= x.drv_n || x.drv[comp] == nullptr) return v;
     const float2 D = x.drv[comp][(long long)i * x.drv_stride[comp] + plane];
     const float w = x.wtap[i];
+    // Re(D * e^{iwt}): NumPy forms complex products as fma(ar, br, -(ai*bi)) on FMA hosts
+    // (its AVX2/AVX-512 loops); the same single-rounding form is used here explicitly.
     if (x.drv_math == LF_DRIVE_C128) {       // fdtd2d_core.py:1426-1427 (float64 blend)
-        const double din = s.env * ((double)D.x * s.dr - (double)D.y * s.di);
+        const double re = fma((double)D.x, s.dr, -((double)D.y * s.di));
+        const double din = s.env * re;
         const float keep = (1.0f - w) * v;
         return (float)((double)w * din + (double)keep);
     }
-    const float re = D.x * s.dr32 - D.y * s.di32;  // fdtd3d_core.py:1627-1630 (float32)
+    const float re = fmaf(D.x, s.dr32, -(D.y * s.di32));   // fdtd3d_core.py:1627-1630 (float32)
     return w * (s.e32 * re) + (1.0f - w) * v;
 }
 

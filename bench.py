@@ -130,7 +130,7 @@ def bench_yee3d(args):
         r = f3d.run_3d('bench3d', 1.49, t_wg_um=t_wg, lam_nm=850.0, lx_um=nx * dx_um, air_um=air,
                        tear_um=tear, lz_um=nz * dx_um, dx_nm=30.0, vcsel_waist_um=1.5,
                        vcsel_waist_z_um=1.5, n_snapshots=1, bead=bead, stack=stack, n_co=1.376,
-                       npml=12, engine_mode='fp64acc', steps_override=steps)
+                       npml=12, engine_mode='fp64acc', steps_override=steps, cw_volume=False)
     wall = time.time() - t0
     e = records[0]
     cells = nx * ny * nz
@@ -226,7 +226,7 @@ def main():
                              kernel='Yee H+E update (per step, HIP events)',
                              bytes_per_cell_update=res['bytes_per_cell'],
                              update_ms_per_step=res['upd_ms_per_step']),
-               wall_s=res['wall'], transmission=res['transmission'])
+               wall_s=res['wall'])
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)

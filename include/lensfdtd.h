@@ -101,7 +101,7 @@ int lf_sync(lf_ctx *ctx);
  * HIP events on the context's stream (0 if timing is off); lf_set_timing(ctx, 1) on. */
 int lf_set_timing(lf_ctx *ctx, int on);
 /* Kernel variant (no reference counterpart: an engine tuning knob): 0 = single-pass
- * H+E kernels where the kind has one (default), 1 = separate H and E kernels; lx = x
+ * H+E kernels where the kind has one, 1 = separate H and E kernels (default); lx = x
  * rows per workgroup of the single-pass kernels. */
 int lf_set_variant(lf_ctx *ctx, int variant, int lx);
 /* y rows (wavefronts) per workgroup of the 3D single-pass kernel: 2, 4 or 8 (tuning

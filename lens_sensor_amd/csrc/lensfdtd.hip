@@ -41,7 +41,7 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 constexpr int kMaxMon = 32;
-constexpr int kBoxBlocks = 64;
+constexpr int kBoxBlocks = 1024;   // per-block FP64 partials (deterministic order)
 
 // ----------------------------------------------------------------------------- device structs
 struct Strip {            // psi slab of one stretch axis: v < lo -> v, v >= hi -> v-hi+lo
@@ -298,13 +298,14 @@ __global__ void __launch_bounds__(256) tm_e_kernel(TMArgs a, Excite x, StepArgs 
 
 
 // ---------------------------------------------------------------------------- fused TM step
-// One pass per step: each workgroup owns 256 consecutive y cells and an x chunk of
-// `lx` rows and marches along x.  Hz^{n+1/2}(i) is formed from E^n, then Ex/Ey^{n+1}(i)
-// from Hz^{n+1/2}(i), Hz^{n+1/2}(i, j-1) (LDS, neighbour lane; the lane below the block
-// is recomputed) and Hz^{n+1/2}(i-1) (register).  Old and new fields live in separate
-// buffers (ping-pong), so recomputed halo values always see E^n / H^{n-1/2}; psi of Hz is
-// ping-ponged for the same reason, psi of Ex/Ey is touched by its owner only.
-// Per cell the arithmetic is the one of tm_h_kernel + tm_e_kernel (same float32 order).
+// One pass per step, barrier free.  Every wavefront is independent: its 64 lanes hold
+// consecutive y cells j = jb - 1 + lane (lane 0 is the cell below the wave's 63 owned
+// cells) and it marches along an x chunk of `lx` rows.  For row i, Hz^{n+1/2}(i) is formed
+// from E^n (Ex(i, j+1) comes from lane+1 by a wavefront shuffle, lane 63 loads it),
+// then Ex^{n+1}(i) uses Hz(i, j-1) from lane-1 and Ey^{n+1}(i) uses Hz(i-1) kept in a
+// register from the previous row.  Old and new fields (and the psi of Hz) are separate
+// buffers (ping-pong): a value recomputed outside a lane's own cells always sees E^n and
+// H^{n-1/2}.  Per cell the arithmetic is exactly that of tm_h_kernel + tm_e_kernel.
 struct TMFArgs {
     Grid g;
     const float *Ex0, *Ey0, *Hz0;
@@ -314,7 +315,7 @@ struct TMFArgs {
     Prof xe, xh, ye, yh;
     Strip pHzx0, pHzy0, pHzx1, pHzy1, pExy, pEyx;
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
-    int lx;
+    int lx, nwy;                      // x rows per wave, waves along y
 };
 
 __device__ __forceinline__ float tm_hz_new(const TMFArgs &a, int i, int j, float ey_n, float ey_c,
@@ -345,46 +346,38 @@ __device__ __forceinline__ float tm_hz_new(const TMFArgs &a, int i, int j, float
 }
 
 __global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, StepArgs st, int fp64) {
-    __shared__ float ex_s[256];
-    __shared__ float hz_s[257];
-    const int t = threadIdx.x;
-    const int j0 = blockIdx.x * 256;
-    const int j = j0 + t;
+    const int lane = threadIdx.x & 63;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wy = wv % a.nwy, wx = wv / a.nwy;
     const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
-    const int i0 = blockIdx.y * a.lx;
+    const int j = wy * 63 + lane - 1;
+    const bool jin = j >= 0 && j < Ny;
+    const bool mine = lane >= 1;                 // lanes 1..63 own their cell
+    const int i0 = wx * a.lx;
+    if (i0 >= Nx) return;                        // whole wave leaves together
     const int i1 = min(i0 + a.lx, Nx);
     const int ia = i0 > 0 ? i0 - 1 : 0;
     float hz_prev = 0.0f;
+    float eyc = (jin && j < Ny - 1) ? a.Ey0[(long long)ia * P + j] : 0.0f;
     for (int i = ia; i < i1; ++i) {
-        const bool own = i >= i0;
-        const long long c = (long long)i * P + j;
-        const float exc = (j < Ny && i < Nx - 1) ? a.Ex0[c] : 0.0f;
-        const float eyc = (j < Ny - 1) ? a.Ey0[c] : 0.0f;
-        const float eyn = (j < Ny - 1 && i < Nx - 1) ? a.Ey0[c + P] : 0.0f;
-        ex_s[t] = exc;
-        __syncthreads();
+        const bool own = i >= i0 && mine && jin;
+        const long long c = (long long)i * P + (jin ? j : 0);
+        const float exc = (jin && i < Nx - 1) ? a.Ex0[c] : 0.0f;
+        const float eyn = (jin && j < Ny - 1 && i < Nx - 1) ? a.Ey0[c + P] : 0.0f;
+        float exn = __shfl_down(exc, 1);
+        if (lane == 63) exn = (jin && j + 1 < Ny && i < Nx - 1) ? a.Ex0[c + 1] : 0.0f;
         float hz = 0.0f;
-        if (i < Nx - 1 && j < Ny - 1) {
-            const float exn = t < 255 ? ex_s[t + 1] : a.Ex0[c + 1];
+        if (jin && i < Nx - 1 && j < Ny - 1) {
             hz = tm_hz_new(a, i, j, eyn, eyc, exn, exc, own);
             if (own) a.Hz1[c] = hz;
         }
-        if (t == 0) {
-            float h = 0.0f;
-            if (j0 >= 1 && i < Nx - 1) {              // lane below the block
-                const long long cm = c - 1;
-                h = tm_hz_new(a, i, j0 - 1, a.Ey0[cm + P], a.Ey0[cm], exc, a.Ex0[cm], false);
-            }
-            hz_s[0] = h;
-        }
-        hz_s[t + 1] = hz;
-        __syncthreads();
-        if (own && j < Ny) {
-            if (i < Nx - 1) {                          // Ex (Nx-1, Ny)
+        const float hzm = __shfl_up(hz, 1);      // Hz(i, j-1)
+        if (own) {
+            if (i < Nx - 1) {                    // Ex (Nx-1, Ny)
                 float v = exc;
                 if (a.dax) v = v * a.dax[c];
                 if (j >= 1 && j <= Ny - 2) {
-                    const float d = hz - hz_s[t];
+                    const float d = hz - hzm;
                     const int s = strip_pos(a.pExy, j);
                     float tt;
                     if (s >= 0) {
@@ -400,7 +393,7 @@ __global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, Step
                 if ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1)) v = 0.0f;
                 a.Ex1[c] = v;
             }
-            if (j < Ny - 1) {                          // Ey (Nx, Ny-1)
+            if (j < Ny - 1) {                    // Ey (Nx, Ny-1)
                 float v = eyc;
                 if (a.day) v = v * a.day[c];
                 if (i >= 1 && i <= Nx - 2) {
@@ -424,7 +417,7 @@ __global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, Step
             }
         }
         hz_prev = hz;
-        __syncthreads();
+        eyc = eyn;
     }
 }
 
@@ -636,172 +629,159 @@ __device__ __forceinline__ float yf_hz(const YFArgs &a, int i, int j, int k, flo
     return h;
 }
 
+// Barrier-free wavefront kernel.  Lane = z: k = kb - 1 + lane (lane 0 is the cell below
+// the 63 owned cells); the wavefront owns rows j0 .. j0+TY-1 of an x chunk and walks,
+// per plane i, the rows j0-1 .. j0+TY-1 (row j0-1 only forms H).  Hx/Hz(j-1) are carried
+// from the previous row, Hy/Hz(i-1) and Ey(i) from the previous plane (register arrays),
+// E/H at k+1 and H at k-1 come from wavefront shuffles.
 template <int TY>
-__global__ void __launch_bounds__(64 * TY) yee_fused_kernel(YFArgs a, Excite x, StepArgs st, int fp64) {
-    constexpr int TZ = 64;
-    constexpr int NRIM = 2 * (TZ + 2) + 2 * TY;
-    __shared__ float se[3][TY + 2][TZ + 2];      // E^n(i): local (j - j0 + 1, k - k0 + 1)
-    __shared__ float sh[3][TY + 1][TZ + 1];      // H^{n+1/2}(i): local (j - j0 + 1, k - k0 + 1)
-    const int lane = threadIdx.x, w = threadIdx.y, tid = w * 64 + lane;
+__global__ void __launch_bounds__(256) yee_fused_kernel(YFArgs a, Excite x, StepArgs st, int fp64,
+                                                        int nwz, int nwy) {
+    const int lane = threadIdx.x & 63;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wz = wv % nwz;
+    const int rest = wv / nwz;
+    const int wy = rest % nwy;
+    const int wx = rest / nwy;
     const Grid &g = a.g;
     const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int k0 = blockIdx.x * TZ, j0 = blockIdx.y * TY;
-    const int k = k0 + lane, j = j0 + w;
-    const int jl = w + 1, kl = lane + 1;
-    const bool inb = j < NY && k < NZ;
-    const long long si = (long long)NY * g.P;
-    const int i0 = blockIdx.z * a.lx;
+    const int i0 = wx * a.lx;
+    if (i0 >= NX) return;                         // whole wavefront leaves together
     const int i1 = min(i0 + a.lx, NX);
     const int ia = i0 > 0 ? i0 - 1 : 0;
-    float eyc = 0.f, ezc = 0.f, hy_prev = 0.f, hz_prev = 0.f;
+    const int k = wz * 63 + lane - 1;
+    const bool kin = k >= 0 && k < NZ;
+    const int j0 = wy * TY;
+    const long long sj = g.P, si = (long long)NY * g.P;
+    float hy_ip[TY], hz_ip[TY], ey_ip[TY + 1];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) { hy_ip[r] = 0.f; hz_ip[r] = 0.f; }
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) {               // Ey(ia, j0-1+r)
+        const int jr = j0 - 1 + r;
+        ey_ip[r] = (kin && jr >= 0 && jr < NY - 1) ? a.Ey0[g.at(ia, jr, k)] : 0.f;
+    }
     for (int i = ia; i < i1; ++i) {
-        const bool own = i >= i0;
-        const long long c = inb ? g.at(i, j, k) : 0;
-        const float exc = (inb && i < NX - 1) ? a.Ex0[c] : 0.f;
-        if (i == ia) {
-            eyc = (inb && j < NY - 1) ? a.Ey0[c] : 0.f;
-            ezc = (inb && k < NZ - 1) ? a.Ez0[c] : 0.f;
-        }
-        const bool nxt = i + 1 < NX;
-        const float eyn = (inb && nxt && j < NY - 1) ? a.Ey0[c + si] : 0.f;
-        const float ezn = (inb && nxt && k < NZ - 1) ? a.Ez0[c + si] : 0.f;
-        se[0][jl][kl] = exc;
-        se[1][jl][kl] = eyc;
-        se[2][jl][kl] = ezc;
-        for (int h = tid; h < NRIM; h += 64 * TY) {        // E rim of plane i
-            int jj, kk;
-            if (h < TZ + 2) { jj = 0; kk = h; }
-            else if (h < 2 * (TZ + 2)) { jj = TY + 1; kk = h - (TZ + 2); }
-            else if (h < 2 * (TZ + 2) + TY) { jj = 1 + h - 2 * (TZ + 2); kk = 0; }
-            else { jj = 1 + h - 2 * (TZ + 2) - TY; kk = TZ + 1; }
-            const int jg = j0 - 1 + jj, kg = k0 - 1 + kk;
-            const bool ok = jg >= 0 && jg < NY && kg >= 0 && kg < NZ;
-            const long long cc = ok ? g.at(i, jg, kg) : 0;
-            se[0][jj][kk] = (ok && i < NX - 1) ? a.Ex0[cc] : 0.f;
-            se[1][jj][kk] = (ok && jg < NY - 1) ? a.Ey0[cc] : 0.f;
-            se[2][jj][kk] = (ok && kg < NZ - 1) ? a.Ez0[cc] : 0.f;
-        }
-        __syncthreads();
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        if (inb) {
-            if (j < NY - 1 && k < NZ - 1) {                                  // Hx
-                hx = yf_hx(a, i, j, k, se[1][jl][kl + 1] - eyc, se[2][jl + 1][kl] - ezc, own);
-                if (own) a.Hx1[c] = hx;
+        const bool plane_own = i >= i0;
+        const bool inx = i < NX - 1;              // Ex, Hy, Hz exist on this plane
+        // row carry: E(i, j0-1) for Ex, Ez
+        int jr = j0 - 1;
+        bool jok = jr >= 0 && jr < NY;
+        long long c = (kin && jok) ? g.at(i, jr, k) : 0;
+        float ex = (kin && jok && inx) ? a.Ex0[c] : 0.f;
+        float ez = (kin && jok && k < NZ - 1) ? a.Ez0[c] : 0.f;
+        float hx_jm = 0.f, hz_jm = 0.f;
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            jr = j0 - 1 + r;
+            jok = jr >= 0 && jr < NY;
+            const bool cell = kin && jok;
+            c = cell ? g.at(i, jr, k) : 0;
+            const float ey = ey_ip[r];
+            const bool jn = jr + 1 < NY;
+            const float exj = (cell && jn && inx) ? a.Ex0[c + sj] : 0.f;
+            const float ezj = (cell && jn && k < NZ - 1) ? a.Ez0[c + sj] : 0.f;
+            const float eyi = (cell && inx && jr < NY - 1) ? a.Ey0[c + si] : 0.f;
+            const float ezi = (cell && inx && k < NZ - 1) ? a.Ez0[c + si] : 0.f;
+            float exk = __shfl_down(ex, 1);
+            float eyk = __shfl_down(ey, 1);
+            if (lane == 63) {
+                const bool kn = cell && k + 1 < NZ;
+                exk = (kn && inx) ? a.Ex0[c + 1] : 0.f;
+                eyk = (kn && jr < NY - 1) ? a.Ey0[c + 1] : 0.f;
             }
-            if (i < NX - 1 && k < NZ - 1) {                                  // Hy
-                hy = yf_hy(a, i, j, k, ezn - ezc, se[0][jl][kl + 1] - exc, own);
-                if (own) a.Hy1[c] = hy;
+            const bool own = plane_own && r >= 1 && lane >= 1 && cell;
+            float hx = 0.f, hy = 0.f, hz = 0.f;
+            if (cell) {
+                if (jr < NY - 1 && k < NZ - 1) {
+                    hx = yf_hx(a, i, jr, k, eyk - ey, ezj - ez, own);
+                    if (own) a.Hx1[c] = hx;
+                }
+                if (inx && k < NZ - 1) {
+                    hy = yf_hy(a, i, jr, k, ezi - ez, exk - ex, own);
+                    if (own) a.Hy1[c] = hy;
+                }
+                if (inx && jr < NY - 1) {
+                    hz = yf_hz(a, i, jr, k, exj - ex, eyi - ey, own);
+                    if (own) a.Hz1[c] = hz;
+                }
             }
-            if (i < NX - 1 && j < NY - 1) {                                  // Hz
-                hz = yf_hz(a, i, j, k, se[0][jl + 1][kl] - exc, eyn - eyc, own);
-                if (own) a.Hz1[c] = hz;
-            }
-        }
-        sh[0][jl][kl] = hx;
-        sh[1][jl][kl] = hy;
-        sh[2][jl][kl] = hz;
-        if (own) {                                         // H rim: Hx,Hz at j0-1; Hx,Hy at k0-1
-            for (int h = tid; h < TZ + TY; h += 64 * TY) {
-                if (h < TZ) {
-                    const int kk = h + 1, kg = k0 + h, jg = j0 - 1;
-                    float vx = 0.f, vz = 0.f;
-                    if (jg >= 0 && kg < NZ) {
-                        const float ex0 = se[0][0][kk], ey0 = se[1][0][kk], ez0 = se[2][0][kk];
-                        if (kg < NZ - 1)
-                            vx = yf_hx(a, i, jg, kg, se[1][0][kk + 1] - ey0, se[2][1][kk] - ez0, false);
-                        if (i < NX - 1) {
-                            const float eyn_r = a.Ey0[g.at(i + 1, jg, kg)];
-                            vz = yf_hz(a, i, jg, kg, se[0][1][kk] - ex0, eyn_r - ey0, false);
+            const float hxk = __shfl_up(hx, 1);   // Hx(k-1)
+            const float hyk = __shfl_up(hy, 1);   // Hy(k-1)
+            if (r >= 1) {
+                const int q = r - 1;
+                if (own) {
+                    if (inx) {                                               // Ex
+                        float v = ex;
+                        if (a.dax) v = v * a.dax[c];
+                        if (jr >= 1 && jr <= NY - 2 && k >= 1 && k <= NZ - 2) {
+                            const float d1 = hz - hz_jm;
+                            const float d2 = hy - hyk;
+                            const float ce = a.cex[c];
+                            v = v + ce * (d1 - d2);
+                            int s = strip_pos(a.sE[0], jr);
+                            if (s >= 0) v = v + ce * psi_term(a.sE[0], 1, s, a.pe[1], jr, d1, i, jr, k, g);
+                            s = strip_pos(a.sE[1], k);
+                            if (s >= 0) v = v + (-ce) * psi_term(a.sE[1], 2, s, a.pe[2], k, d2, i, jr, k, g);
                         }
+                        if ((a.pec[2] && jr == 0) || (a.pec[3] && jr == NY - 1) ||
+                            (a.pec[4] && k == 0) || (a.pec[5] && k == NZ - 1))
+                            v = 0.0f;
+                        v = apply_drive(v, LF_EX, i, jr * NZ + k, x, st);
+                        a.Ex1[c] = v;
                     }
-                    sh[0][0][kk] = vx;
-                    sh[2][0][kk] = vz;
-                } else {
-                    const int jj = h - TZ + 1, jg = j0 + h - TZ, kg = k0 - 1;
-                    float vx = 0.f, vy = 0.f;
-                    if (kg >= 0 && jg < NY) {
-                        const float ex0 = se[0][jj][0], ey0 = se[1][jj][0], ez0 = se[2][jj][0];
-                        if (jg < NY - 1)
-                            vx = yf_hx(a, i, jg, kg, se[1][jj][1] - ey0, se[2][jj + 1][0] - ez0, false);
-                        if (i < NX - 1) {
-                            const float ezn_r = a.Ez0[g.at(i + 1, jg, kg)];
-                            vy = yf_hy(a, i, jg, kg, ezn_r - ez0, se[0][jj][1] - ex0, false);
+                    if (jr < NY - 1) {                                       // Ey
+                        float v = ey;
+                        if (a.day) v = v * a.day[c];
+                        if (i >= 1 && i <= NX - 2 && k >= 1 && k <= NZ - 2) {
+                            const float d1 = hx - hxk;
+                            const float d2 = hz - hz_ip[q];
+                            const float ce = a.cey[c];
+                            v = v + ce * (d1 - d2);
+                            int s = strip_pos(a.sE[2], k);
+                            if (s >= 0) v = v + ce * psi_term(a.sE[2], 2, s, a.pe[2], k, d1, i, jr, k, g);
+                            s = strip_pos(a.sE[3], i);
+                            if (s >= 0) v = v + (-ce) * psi_term(a.sE[3], 0, s, a.pe[0], i, d2, i, jr, k, g);
                         }
+                        if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) ||
+                            (a.pec[4] && k == 0) || (a.pec[5] && k == NZ - 1))
+                            v = 0.0f;
+                        const int plane = jr * NZ + k;
+                        v = apply_source(v, LF_EY, i, plane, x, st, fp64);
+                        v = apply_drive(v, LF_EY, i, plane, x, st);
+                        a.Ey1[c] = v;
                     }
-                    sh[0][jj][0] = vx;
-                    sh[1][jj][0] = vy;
+                    if (k < NZ - 1) {                                        // Ez
+                        float v = ez;
+                        if (a.daz) v = v * a.daz[c];
+                        if (i >= 1 && i <= NX - 2 && jr >= 1 && jr <= NY - 2) {
+                            const float d1 = hy - hy_ip[q];
+                            const float d2 = hx - hx_jm;
+                            const float ce = a.cez[c];
+                            v = v + ce * (d1 - d2);
+                            int s = strip_pos(a.sE[4], i);
+                            if (s >= 0) v = v + ce * psi_term(a.sE[4], 0, s, a.pe[0], i, d1, i, jr, k, g);
+                            s = strip_pos(a.sE[5], jr);
+                            if (s >= 0) v = v + (-ce) * psi_term(a.sE[5], 1, s, a.pe[1], jr, d2, i, jr, k, g);
+                        }
+                        if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) ||
+                            (a.pec[2] && jr == 0) || (a.pec[3] && jr == NY - 1))
+                            v = 0.0f;
+                        const int plane = jr * (NZ - 1) + k;
+                        v = apply_source(v, LF_EZ, i, plane, x, st, fp64);
+                        v = apply_drive(v, LF_EZ, i, plane, x, st);
+                        a.Ez1[c] = v;
+                    }
                 }
+                hy_ip[q] = hy;
+                hz_ip[q] = hz;
             }
+            ey_ip[r] = eyi;
+            hx_jm = hx;
+            hz_jm = hz;
+            ex = exj;
+            ez = ezj;
         }
-        __syncthreads();
-        if (own && inb) {
-            if (i < NX - 1) {                                                // Ex
-                float v = exc;
-                if (a.dax) v = v * a.dax[c];
-                if (j >= 1 && j <= NY - 2 && k >= 1 && k <= NZ - 2) {
-                    const float d1 = hz - sh[2][jl - 1][kl];
-                    const float d2 = hy - sh[1][jl][kl - 1];
-                    const float ce = a.cex[c];
-                    v = v + ce * (d1 - d2);
-                    int s = strip_pos(a.sE[0], j);
-                    if (s >= 0) v = v + ce * psi_term(a.sE[0], 1, s, a.pe[1], j, d1, i, j, k, g);
-                    s = strip_pos(a.sE[1], k);
-                    if (s >= 0) v = v + (-ce) * psi_term(a.sE[1], 2, s, a.pe[2], k, d2, i, j, k, g);
-                }
-                if ((a.pec[2] && j == 0) || (a.pec[3] && j == NY - 1) || (a.pec[4] && k == 0) ||
-                    (a.pec[5] && k == NZ - 1))
-                    v = 0.0f;
-                v = apply_drive(v, LF_EX, i, j * NZ + k, x, st);
-                a.Ex1[c] = v;
-            }
-            if (j < NY - 1) {                                                // Ey
-                float v = eyc;
-                if (a.day) v = v * a.day[c];
-                if (i >= 1 && i <= NX - 2 && k >= 1 && k <= NZ - 2) {
-                    const float d1 = hx - sh[0][jl][kl - 1];
-                    const float d2 = hz - hz_prev;
-                    const float ce = a.cey[c];
-                    v = v + ce * (d1 - d2);
-                    int s = strip_pos(a.sE[2], k);
-                    if (s >= 0) v = v + ce * psi_term(a.sE[2], 2, s, a.pe[2], k, d1, i, j, k, g);
-                    s = strip_pos(a.sE[3], i);
-                    if (s >= 0) v = v + (-ce) * psi_term(a.sE[3], 0, s, a.pe[0], i, d2, i, j, k, g);
-                }
-                if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) || (a.pec[4] && k == 0) ||
-                    (a.pec[5] && k == NZ - 1))
-                    v = 0.0f;
-                const int plane = j * NZ + k;
-                v = apply_source(v, LF_EY, i, plane, x, st, fp64);
-                v = apply_drive(v, LF_EY, i, plane, x, st);
-                a.Ey1[c] = v;
-            }
-            if (k < NZ - 1) {                                                // Ez
-                float v = ezc;
-                if (a.daz) v = v * a.daz[c];
-                if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2) {
-                    const float d1 = hy - hy_prev;
-                    const float d2 = hx - sh[0][jl - 1][kl];
-                    const float ce = a.cez[c];
-                    v = v + ce * (d1 - d2);
-                    int s = strip_pos(a.sE[4], i);
-                    if (s >= 0) v = v + ce * psi_term(a.sE[4], 0, s, a.pe[0], i, d1, i, j, k, g);
-                    s = strip_pos(a.sE[5], j);
-                    if (s >= 0) v = v + (-ce) * psi_term(a.sE[5], 1, s, a.pe[1], j, d2, i, j, k, g);
-                }
-                if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) || (a.pec[2] && j == 0) ||
-                    (a.pec[3] && j == NY - 1))
-                    v = 0.0f;
-                const int plane = j * (NZ - 1) + k;
-                v = apply_source(v, LF_EZ, i, plane, x, st, fp64);
-                v = apply_drive(v, LF_EZ, i, plane, x, st);
-                a.Ez1[c] = v;
-            }
-        }
-        hy_prev = hy;
-        hz_prev = hz;
-        eyc = eyn;
-        ezc = ezn;
-        __syncthreads();
     }
 }
 
@@ -1036,6 +1016,10 @@ __global__ void __launch_bounds__(256) box_kernel(MonArgs m, BoxArgs b) {
     if (threadIdx.x == 0) b.partial[blockIdx.x] += red[0];
 }
 
+// Monitor launch shape: rows of 256 along the fast axis when the middle extent is 1
+// (2D planes), otherwise 64 x 4 tiles (3D volumes).
+static inline dim3 mon_block(int n1) { return n1 == 1 ? dim3(256, 1, 1) : dim3(64, 4, 1); }
+
 __global__ void scatter_kernel(float *dst, const float *src, int nx, int ny, int nz, int NY, int P) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)nx * ny * nz;
@@ -1078,7 +1062,7 @@ struct lf_ctx {
     hipStream_t stream = nullptr;
     float *arr[LF_NARRAYS] = {};
     float *alt[LF_NARRAYS] = {};         // second buffer of ping-pong fields (fused path)
-    int variant = 0;                     // 0: single-pass kernels where available, 1: H then E
+    int variant = 1;                     // 0: single-pass kernels where available, 1: H then E
     int lx = 32;                         // x rows per workgroup in the single-pass kernels
     float *psi_alt[4] = {};              // ping-pong psi of the H slabs (2D fused)
     float *psi_alt3[6] = {};             // ping-pong psi of the H slabs (3D fused)
@@ -1561,8 +1545,10 @@ int tm_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
     f.pec_xlo = c->pec[0][0]; f.pec_xhi = c->pec[0][1];
     f.pec_ylo = c->pec[1][0]; f.pec_yhi = c->pec[1][1];
     f.lx = c->lx;
-    dim3 grd((c->g.nz + 255) / 256, (c->g.nx + c->lx - 1) / c->lx, 1);
-    tm_fused_kernel<<<grd, 256, 0, c->stream>>>(f, c->ex, st, fp64);
+    f.nwy = (c->g.nz + 62) / 63;                      // 63 owned y cells per wavefront
+    const long long nwx = (c->g.nx + c->lx - 1) / c->lx;
+    const long long nwaves = (long long)f.nwy * nwx;
+    tm_fused_kernel<<<(unsigned)((nwaves + 3) / 4), 256, 0, c->stream>>>(f, c->ex, st, fp64);
     std::swap(c->arr[LF_EX], c->alt[LF_EX]);
     std::swap(c->arr[LF_EY], c->alt[LF_EY]);
     std::swap(c->arr[LF_HZ], c->alt[LF_HZ]);
@@ -1621,10 +1607,12 @@ int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
     for (int ax = 0; ax < 3; ++ax) { f.pec[2 * ax] = c->pec[ax][0]; f.pec[2 * ax + 1] = c->pec[ax][1]; }
     f.lx = c->lx;
     const int ty = c->ty;
-    dim3 grd((c->g.nz + 63) / 64, (c->g.ny + ty - 1) / ty, (c->g.nx + c->lx - 1) / c->lx);
-    if (ty == 4) yee_fused_kernel<4><<<grd, dim3(64, 4, 1), 0, c->stream>>>(f, c->ex, st, fp64);
-    else if (ty == 2) yee_fused_kernel<2><<<grd, dim3(64, 2, 1), 0, c->stream>>>(f, c->ex, st, fp64);
-    else yee_fused_kernel<8><<<grd, dim3(64, 8, 1), 0, c->stream>>>(f, c->ex, st, fp64);
+    const int nwz = (c->g.nz + 62) / 63, nwy = (c->g.ny + ty - 1) / ty;
+    const long long nwx = (c->g.nx + c->lx - 1) / c->lx;
+    const unsigned nblk = (unsigned)(((long long)nwz * nwy * nwx + 3) / 4);
+    if (ty == 4) yee_fused_kernel<4><<<nblk, 256, 0, c->stream>>>(f, c->ex, st, fp64, nwz, nwy);
+    else if (ty == 2) yee_fused_kernel<2><<<nblk, 256, 0, c->stream>>>(f, c->ex, st, fp64, nwz, nwy);
+    else yee_fused_kernel<8><<<nblk, 256, 0, c->stream>>>(f, c->ex, st, fp64, nwz, nwy);
     for (int q = 0; q < 6; ++q) std::swap(c->arr[ids[q]], c->alt[ids[q]]);
     for (int q = 0; q < 6; ++q)
         if (c->sH[q].psi) std::swap(c->sH[q].psi, c->psi_alt3[q]);
@@ -1918,14 +1906,16 @@ int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
             if (m.type == 0) {
                 box_kernel<<<kBoxBlocks, 256, 0, c->stream>>>(ma, m.box);
             } else if (m.type == 1) {
-                dim3 gq((m.n2 + 63) / 64, (m.n1 + 3) / 4, m.n0);
-                sq_kernel<<<gq, blk, 0, c->stream>>>(ma, (float *)m.buf, m.form, m.nxa, m.n1, m.n2);
+                const dim3 mb = mon_block(m.n1);
+                dim3 gq((m.n2 + mb.x - 1) / mb.x, (m.n1 + mb.y - 1) / mb.y, m.n0);
+                sq_kernel<<<gq, mb, 0, c->stream>>>(ma, (float *)m.buf, m.form, m.nxa, m.n1, m.n2);
             } else if (m.type == 2) {
                 const DftArgs &d = m.dft;
-                dim3 gq((d.n2 + 63) / 64, (d.n1 + 3) / 4, d.n0);
+                const dim3 mb = mon_block(d.n1);
+                dim3 gq((d.n2 + mb.x - 1) / mb.x, (d.n1 + mb.y - 1) / mb.y, d.n0);
                 const float fr = d.phasor == LF_PH_Q ? t.qr32 : t.pr32;
                 const float fi = d.phasor == LF_PH_Q ? t.qi32 : t.pi32;
-                dft_kernel<<<gq, blk, 0, c->stream>>>(ma, d, t.pr, t.pi, fr, fi);
+                dft_kernel<<<gq, mb, 0, c->stream>>>(ma, d, t.pr, t.pi, fr, fi);
             } else {
                 const FluxArgs &f = m.flux;
                 const int n1 = f.n1[0] > (f.npair > 1 ? f.n1[1] : 0) ? f.n1[0] : f.n1[1];

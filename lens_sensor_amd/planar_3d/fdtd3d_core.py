@@ -410,11 +410,12 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
            pec_faces=(), end_facet_um=0.0, vcsel_tilt_deg=0.0, vcsel_offset_y_um=0.0,
            vcsel_offset_z_um=0.0, input_gap_um=0.0, polarization='s', allow_large=False,
            boundary='cpml', cad=None, pulse_cycles=30.0, stack=None,
-           npml=None, engine_mode=None, steps_override=None):
+           npml=None, engine_mode=None, steps_override=None, cw_volume=True):
     """Full-domain 3D run (reference fdtd3d_core.py:793-1308).  Extra keywords of this
     build: ``npml`` (CPML cells, reference fixed 10), ``engine_mode`` ('ref32' for the
     reference's float32 semantics, 'fp64acc' for FP64 phase/accumulators),
-    ``steps_override`` (run exactly this many steps, no CW extension)."""
+    ``steps_override`` (run exactly this many steps, no CW extension), ``cw_volume``
+    (False: CW phasor only on the z-centre monitor plane, no volume accumulator)."""
     if cad is not None:
         raise NotImplementedError('CAD geometry import is outside the device path')
     if save_vector:
@@ -514,7 +515,8 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
     mon_pin = eng.add_box(((iin0, iin1), (j_wg0, j_wg1), zs))
     mon_pout = eng.add_box(((iout0, iout1), (j_wg0, j_wg1), zs))
     mon_I = eng.add_sq('3d_p' if pol == 'p' else '3d_s', Nx)
-    mon_cw = eng.add_dft(src_comp, 0, Nx, acc=acc_full)
+    mon_cw = (eng.add_dft(src_comp, 0, Nx, acc=acc_full) if cw_volume
+              else eng.add_dft(src_comp, 0, Nx, kplane=kc, acc=acc_full))
     mon_pa = eng.add_dft(src_comp, 0, Nx, kplane=kc, acc=acc_full) if conv else None
     mon_pb = eng.add_dft(src_comp, 0, Nx, kplane=kc, acc=acc_full) if conv else None
     book = PlaneFluxBook(eng, 'yee3d')

@@ -185,6 +185,27 @@ def cpu_baseline(workload, seconds=12.0):
                        f'host has {os.cpu_count()} CPUs')
 
 
+def copy_probe(mib):
+    """Plain device copy of known size (calibration of the HBM byte counters): ``mib`` MiB
+    read + ``mib`` MiB written per launch, 20 launches; prints one JSON line."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device='cuda:0')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(20):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / 20
+    print(json.dumps(dict(probe='copy', bytes_read=n * 4, bytes_written=n * 4, ms_per_copy=ms,
+                          gbs=2 * n * 4 / (ms * 1e-3) / 1e9)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -197,7 +218,12 @@ def main():
                     help='steps per window (reduced runs for profiling only)')
     ap.add_argument('--seed', type=int, default=2024)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-probe', type=int, default=None, metavar='MIB',
+                    help='only run a plain device copy of MIB MiB (byte-counter calibration)')
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print('bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not '
               'part of it', file=sys.stderr)

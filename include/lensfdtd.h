@@ -108,6 +108,14 @@ int lf_set_variant(lf_ctx *ctx, int variant, int lx);
  * knob, no reference counterpart). */
 int lf_set_tile(lf_ctx *ctx, int ty);
 int lf_kernel_ms(lf_ctx *ctx, double *update_ms, double *total_ms, int *nsteps);
+/* Temporal blocking (no reference counterpart: an engine tuning knob; the reference
+ * advances one step per call, boundaries.py:240-258).  depth = steps advanced per pass
+ * (1 = off, 2, 4, 6 or 8) over runs of steps whose monitor mask is 0; lx = x planes per
+ * wavefront chunk.  Results are bit-identical to depth 1.  Used by 2D TM. */
+int lf_set_temporal(lf_ctx *ctx, int depth, int lx);
+/* Blocked passes of the last lf_run with timing on: summed HIP-event time (ms), number of
+ * launches, current depth. */
+int lf_tb_stats(lf_ctx *ctx, double *ms, int *launches, int *depth);
 
 #ifdef __cplusplus
 }

@@ -421,6 +421,200 @@ __global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, Step
     }
 }
 
+// ---------------------------------------------------------------------------- temporal blocking (2D TM)
+// T time steps per pass.  A wavefront holds 64 consecutive y cells
+// j = wy*(64-2T) - T + lane and marches along its x chunk [X0, X1) from X0-T to X1+T-1.
+// Iteration p loads plane p (E^n, Hz^{n-1/2}, psi, coefficients) into pipeline slot 0;
+// level s (0 <= s < T) then advances plane q = p-1-s from step n+s to n+s+1: it needs
+// Ey^{n+s}(q+1) (this iteration's output of level s-1), E^{n+s}(q) and Hz^{n+s-1/2}(q)
+// (level s-1's output of the previous iteration) and its own Hz^{n+s+1/2}(q-1).  Every
+// step shrinks the valid y range by one cell on each side (Hz needs Ex(j+1), Ex needs
+// Hz(j-1)) and the valid x range by one plane on the low side, so after T steps lanes
+// T..63-T of planes X0..X1-1 are exact and are stored.  Inputs and outputs (fields and
+// all four psi slabs) are separate buffers.  Per cell and step the arithmetic is that of
+// tm_h_kernel + tm_e_kernel (boundaries.py:243-258, fdtd2d_core.py:796).
+constexpr int kMaxTB = 8;
+
+struct TBTMArgs {
+    Grid g;
+    const float *Ex0, *Ey0, *Hz0;
+    float *Ex1, *Ey1, *Hz1;
+    const float *cax, *cay, *dax, *day;
+    float ch;
+    Prof xe, xh, ye, yh;
+    Strip s0[4], s1[4];       // Hzx Hzy Exy Eyx: read / write buffers
+    int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
+    int lx, nwy;
+    StepArgs st[kMaxTB];
+};
+
+template <int T>
+__global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
+    const int lane = threadIdx.x & 63;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wy = wv % a.nwy, wx = wv / a.nwy;
+    const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
+    const int X0 = wx * a.lx;
+    if (X0 >= Nx) return;                          // whole wavefront leaves together
+    const int X1 = min(X0 + a.lx, Nx);
+    const int j = wy * (64 - 2 * T) - T + lane;
+    const bool jin = j >= 0 && j < Ny;
+    const bool jey = jin && j < Ny - 1;            // Ey and Hz exist in this column
+    const bool own = jin && lane >= T && lane <= 63 - T;
+    // psi slots exist only where the single-step kernels update psi: Hz slabs for
+    // j <= Ny-2 (H nodes), Ex slabs for 1 <= j <= Ny-2 (E nodes)
+    const int sHzy = jey ? strip_pos(a.s0[1], j) : -1;
+    const int sExy = (jin && j >= 1 && j <= Ny - 2) ? strip_pos(a.s0[2], j) : -1;
+    float yhb = 0.f, yhc = 0.f, yhk = 1.f, yeb = 0.f, yec = 0.f, yek = 1.f;
+    if (sHzy >= 0) { yhb = a.yh.b[j]; yhc = a.yh.c[j]; yhk = a.yh.k[j]; }
+    if (sExy >= 0) { yeb = a.ye.b[j]; yec = a.ye.c[j]; yek = a.ye.k[j]; }
+    const bool lossy_x = a.dax != nullptr, lossy_y = a.day != nullptr;
+
+    // slot L holds the output of level L-1 (slot 0: loaded plane); C = this iteration,
+    // P = previous iteration
+    float exC[T + 1], eyC[T + 1], hzC[T + 1], exP[T + 1], eyP[T + 1], hzP[T + 1];
+    float hxC[T + 1], hyC[T + 1], eyyC[T + 1], exxC[T + 1];    // psi Hzx Hzy Exy Eyx
+    float hxP[T + 1], hyP[T + 1], eyyP[T + 1], exxP[T + 1];
+    float cA[T + 1], cY[T + 1], dA[T + 1], dY[T + 1];         // coefficients, by plane age
+#pragma unroll
+    for (int L = 0; L <= T; ++L) {
+        exC[L] = eyC[L] = hzC[L] = exP[L] = eyP[L] = hzP[L] = 0.f;
+        hxC[L] = hyC[L] = eyyC[L] = exxC[L] = 0.f;
+        hxP[L] = hyP[L] = eyyP[L] = exxP[L] = 0.f;
+        cA[L] = cY[L] = dA[L] = dY[L] = 0.f;
+    }
+    const int ps = X0 - T > 0 ? X0 - T : 0;
+    const int pe = X1 + T;
+    for (int p = ps; p < pe; ++p) {
+        // ---- slot 0: plane p of step n
+        {
+            float ex = 0.f, ey = 0.f, hz = 0.f, ca = 0.f, cy = 0.f, da = 0.f, dy = 0.f;
+            float ph = 0.f, py = 0.f, qy = 0.f, qx = 0.f;
+            if (p < Nx) {
+                const long long c = (long long)p * P + j;
+                const bool xin = p < Nx - 1;
+                if (xin && jin) {
+                    ex = a.Ex0[c];
+                    ca = a.cax[c];
+                    if (lossy_x) da = a.dax[c];
+                }
+                if (jey) {
+                    ey = a.Ey0[c];
+                    cy = a.cay[c];
+                    if (lossy_y) dy = a.day[c];
+                }
+                if (xin && jey) hz = a.Hz0[c];
+                const int sx0 = xin ? strip_pos(a.s0[0], p) : -1;             // H nodes p <= Nx-2
+                if (sx0 >= 0 && jey) ph = a.s0[0].psi[(long long)sx0 * P + j];
+                if (sHzy >= 0 && xin) py = a.s0[1].psi[(long long)p * a.s0[1].len + sHzy];
+                if (sExy >= 0 && xin) qy = a.s0[2].psi[(long long)p * a.s0[2].len + sExy];
+                const int sx3 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[3], p) : -1;
+                if (sx3 >= 0 && jey) qx = a.s0[3].psi[(long long)sx3 * P + j];
+            }
+            exC[0] = ex; eyC[0] = ey; hzC[0] = hz;
+            hxC[0] = ph; hyC[0] = py; eyyC[0] = qy; exxC[0] = qx;
+            cA[0] = ca; cY[0] = cy; dA[0] = da; dY[0] = dy;
+        }
+        // ---- levels
+#pragma unroll
+        for (int s = 0; s < T; ++s) {
+            const int q = p - 1 - s;
+            if (q >= ps && q < Nx) {
+                const float ex_o = exP[s], ey_o = eyP[s], hz_o = hzP[s];
+                const float ex_jp = __shfl_down(ex_o, 1);           // Ex(q, j+1)
+                float psx = hxP[s], psy = hyP[s];
+                float hz = 0.f;
+                if (q < Nx - 1 && jey) {                            // Hz(q, j)
+                    const float dEyx = eyC[s] - ey_o;
+                    const float dExy = ex_jp - ex_o;
+                    float tx, ty;
+                    if (strip_pos(a.s0[0], q) >= 0) {
+                        psx = a.xh.b[q] * psx + a.xh.c[q] * dEyx;
+                        tx = dEyx / a.xh.k[q] + psx;
+                    } else {
+                        tx = dEyx;
+                    }
+                    if (sHzy >= 0) {
+                        psy = yhb * psy + yhc * dExy;
+                        ty = dExy / yhk + psy;
+                    } else {
+                        ty = dExy;
+                    }
+                    hz = hz_o - a.ch * (tx - ty);
+                }
+                const float hz_jm = __shfl_up(hz, 1);               // Hz(q, j-1)
+                float qy = eyyP[s];
+                float exn = 0.f;
+                if (q < Nx - 1 && jin) {                            // Ex(q, j)
+                    float v = ex_o;
+                    if (lossy_x) v = v * dA[s + 1];
+                    if (j >= 1 && j <= Ny - 2) {
+                        const float d = hz - hz_jm;
+                        float t;
+                        if (sExy >= 0) {
+                            qy = yeb * qy + yec * d;
+                            t = d / yek + qy;
+                        } else {
+                            t = d;
+                        }
+                        v = v + cA[s + 1] * t;
+                    }
+                    if ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1)) v = 0.0f;
+                    exn = v;
+                }
+                float qx = exxP[s];
+                float eyn = 0.f;
+                if (jey) {                                          // Ey(q, j)
+                    float v = ey_o;
+                    if (lossy_y) v = v * dY[s + 1];
+                    if (q >= 1 && q <= Nx - 2) {
+                        const float d = hz - hzP[s + 1];            // Hz(q) - Hz(q-1)
+                        float t;
+                        if (strip_pos(a.s0[3], q) >= 0) {
+                            qx = a.xe.b[q] * qx + a.xe.c[q] * d;
+                            t = d / a.xe.k[q] + qx;
+                        } else {
+                            t = d;
+                        }
+                        v = v + (-cY[s + 1]) * t;
+                    }
+                    v = apply_source(v, LF_EY, q, j, x, a.st[s], fp64);
+                    v = apply_drive(v, LF_EY, q, j, x, a.st[s]);
+                    if ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1)) v = 0.0f;
+                    eyn = v;
+                }
+                exC[s + 1] = exn; eyC[s + 1] = eyn; hzC[s + 1] = hz;
+                hxC[s + 1] = psx; hyC[s + 1] = psy; eyyC[s + 1] = qy; exxC[s + 1] = qx;
+            }
+            exP[s] = exC[s]; eyP[s] = eyC[s]; hzP[s] = hzC[s];
+            hxP[s] = hxC[s]; hyP[s] = hyC[s]; eyyP[s] = eyyC[s]; exxP[s] = exxC[s];
+        }
+        // ---- store plane p-T of step n+T
+        {
+            const int q = p - T;
+            if (q >= X0 && q < X1 && own) {
+                const long long c = (long long)q * P + j;
+                if (q < Nx - 1) {
+                    a.Ex1[c] = exC[T];
+                    if (jey) a.Hz1[c] = hzC[T];
+                }
+                if (jey) a.Ey1[c] = eyC[T];
+                const int sx0 = q < Nx - 1 ? strip_pos(a.s1[0], q) : -1;
+                if (sx0 >= 0 && jey) a.s1[0].psi[(long long)sx0 * P + j] = hxC[T];
+                if (sHzy >= 0 && q < Nx - 1) a.s1[1].psi[(long long)q * a.s1[1].len + sHzy] = hyC[T];
+                if (sExy >= 0 && q < Nx - 1) a.s1[2].psi[(long long)q * a.s1[2].len + sExy] = eyyC[T];
+                const int sx3 = (q >= 1 && q <= Nx - 2) ? strip_pos(a.s1[3], q) : -1;
+                if (sx3 >= 0 && jey) a.s1[3].psi[(long long)sx3 * P + j] = exxC[T];
+            }
+            exP[T] = exC[T]; eyP[T] = eyC[T]; hzP[T] = hzC[T];
+        }
+#pragma unroll
+        for (int L = T; L >= 1; --L) {
+            cA[L] = cA[L - 1]; cY[L] = cY[L - 1]; dA[L] = dA[L - 1]; dY[L] = dY[L - 1];
+        }
+    }
+}
+
 // ============================================================================ 3D kernels
 struct YeeArgs {
     Grid g;
@@ -652,7 +846,7 @@ __global__ void __launch_bounds__(256) yee_fused_kernel(YFArgs a, Excite x, Step
     const int k = wz * 63 + lane - 1;
     const bool kin = k >= 0 && k < NZ;
     const int j0 = wy * TY;
-    const long long sj = g.P, si = (long long)NY * g.P;
+    const long long si = (long long)NY * g.P;
     float hy_ip[TY], hz_ip[TY], ey_ip[TY + 1];
 #pragma unroll
     for (int r = 0; r < TY; ++r) { hy_ip[r] = 0.f; hz_ip[r] = 0.f; }
@@ -678,9 +872,11 @@ __global__ void __launch_bounds__(256) yee_fused_kernel(YFArgs a, Excite x, Step
             const bool cell = kin && jok;
             c = cell ? g.at(i, jr, k) : 0;
             const float ey = ey_ip[r];
-            const bool jn = jr + 1 < NY;
-            const float exj = (cell && jn && inx) ? a.Ex0[c + sj] : 0.f;
-            const float ezj = (cell && jn && k < NZ - 1) ? a.Ez0[c + sj] : 0.f;
+            // row jr+1 is loaded whenever it exists, also below the domain (jr = -1)
+            const bool nxt = kin && jr + 1 >= 0 && jr + 1 < NY;
+            const long long cn = nxt ? g.at(i, jr + 1, k) : 0;
+            const float exj = (nxt && inx) ? a.Ex0[cn] : 0.f;
+            const float ezj = (nxt && k < NZ - 1) ? a.Ez0[cn] : 0.f;
             const float eyi = (cell && inx && jr < NY - 1) ? a.Ey0[c + si] : 0.f;
             const float ezi = (cell && inx && k < NZ - 1) ? a.Ez0[c + si] : 0.f;
             float exk = __shfl_down(ex, 1);
@@ -1067,6 +1263,10 @@ struct lf_ctx {
     float *psi_alt[4] = {};              // ping-pong psi of the H slabs (2D fused)
     float *psi_alt3[6] = {};             // ping-pong psi of the H slabs (3D fused)
     int ty = 8;                          // y rows (waves) per workgroup of the 3D single pass
+    int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
+    int tb_lx = 32;                      // x planes per wavefront chunk of the blocked pass
+    double last_tb_ms = 0;               // blocked launches of the last lf_run: time, count
+    int last_tb_launches = 0;
     float ch = 0.f;
     // profiles per axis: e (k,b,c,km1), h (k,b,c,km1)
     float *prof_e[3][4] = {}, *prof_h[3][4] = {};
@@ -1265,6 +1465,7 @@ int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out) {
     if (nx < 3 || ny < 3 || (kind == LF_YEE3D && nz < 3)) return fail(-1, "grid too small");
     lf_ctx *c = new lf_ctx();
     c->kind = kind; c->mode = mode;
+    if (kind == LF_TM2D) c->tb = 4;              // blocked passes over monitor-free steps
     c->nx = nx; c->ny = ny; c->nz = (kind == LF_YEE3D) ? nz : 1;
     if (kind == LF_YEE3D) c->g = Grid{nx, ny, nz, round_up(nz, 64)};
     else c->g = Grid{nx, 1, ny, round_up(ny, 64)};
@@ -1558,6 +1759,81 @@ int tm_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
 }
 
 }  // namespace
+
+namespace {
+
+StepArgs step_args(const lf_step &t) {
+    return StepArgs{t.e32, t.wt32, t.ssin, t.scos, t.env, t.dr32, t.di32, t.dr, t.di};
+}
+
+// TM psi slabs compressed along x: Hzx (0) and Eyx (3); Hzy (1) and Exy (2) along y
+bool tm_x_slab(int q) { return q == 0 || q == 3; }
+
+// one temporally blocked TM pass over T steps: reads arr[], writes alt[], then swaps
+int tm_tb_chunk(lf_ctx *c, const lf_step *steps, int T, int fp64) {
+    if (int r = ensure_alt(c, LF_EX)) return r;
+    if (int r = ensure_alt(c, LF_EY)) return r;
+    if (int r = ensure_alt(c, LF_HZ)) return r;
+    for (int q = 0; q < 4; ++q)
+        if (c->s2[q].psi && !c->psi_alt[q]) {
+            const size_t n = strip_elems_2d(c, c->s2[q], tm_x_slab(q));
+            LF_HIP(hipMalloc(&c->psi_alt[q], n * sizeof(float)));
+            LF_HIP(hipMemsetAsync(c->psi_alt[q], 0, n * sizeof(float), c->stream));
+        }
+    TBTMArgs f{};
+    f.g = c->g;
+    f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Hz0 = c->arr[LF_HZ];
+    f.Ex1 = c->alt[LF_EX]; f.Ey1 = c->alt[LF_EY]; f.Hz1 = c->alt[LF_HZ];
+    f.cax = c->arr[LF_CAX]; f.cay = c->arr[LF_CAY]; f.dax = c->arr[LF_DAX]; f.day = c->arr[LF_DAY];
+    f.ch = c->ch;
+    f.xe = prof_of(c, 0, false); f.xh = prof_of(c, 0, true);
+    f.ye = prof_of(c, 1, false); f.yh = prof_of(c, 1, true);
+    for (int q = 0; q < 4; ++q) {
+        f.s0[q] = c->s2[q];
+        f.s1[q] = c->s2[q];
+        f.s1[q].psi = c->psi_alt[q];
+    }
+    f.pec_xlo = c->pec[0][0]; f.pec_xhi = c->pec[0][1];
+    f.pec_ylo = c->pec[1][0]; f.pec_yhi = c->pec[1][1];
+    f.lx = c->tb_lx;
+    f.nwy = (c->g.nz + (64 - 2 * T) - 1) / (64 - 2 * T);
+    for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
+    const long long nwx = (c->g.nx + f.lx - 1) / f.lx;
+    const unsigned nblk = (unsigned)(((long long)f.nwy * nwx + 3) / 4);
+    switch (T) {
+        case 2: tm_tb_kernel<2><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
+        case 4: tm_tb_kernel<4><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
+        case 6: tm_tb_kernel<6><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
+        case 8: tm_tb_kernel<8><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
+        default: return fail(-1, "temporal block depth must be 2, 4, 6 or 8");
+    }
+    std::swap(c->arr[LF_EX], c->alt[LF_EX]);
+    std::swap(c->arr[LF_EY], c->alt[LF_EY]);
+    std::swap(c->arr[LF_HZ], c->alt[LF_HZ]);
+    for (int q = 0; q < 4; ++q)
+        if (c->s2[q].psi) std::swap(c->s2[q].psi, c->psi_alt[q]);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int lf_set_temporal(lf_ctx *c, int depth, int lx) {
+    if (int e = check_ctx(c)) return e;
+    if (depth != 1 && depth != 2 && depth != 4 && depth != 6 && depth != 8)
+        return fail(-1, "lf_set_temporal: depth must be 1 (off), 2, 4, 6 or 8");
+    if (lx < 1 || lx > 1 << 20) return fail(-1, "lf_set_temporal: lx out of range");
+    c->tb = depth;
+    c->tb_lx = lx;
+    return 0;
+}
+
+extern "C" int lf_tb_stats(lf_ctx *c, double *ms, int *launches, int *depth) {
+    if (int e = check_ctx(c)) return e;
+    *ms = c->last_tb_ms;
+    *launches = c->last_tb_launches;
+    *depth = c->tb;
+    return 0;
+}
 
 extern "C" int lf_set_variant(lf_ctx *c, int variant, int lx) {
     if (int e = check_ctx(c)) return e;
@@ -1878,10 +2154,28 @@ int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
         }
         LF_HIP(hipEventRecord(c->ev[2 * nsteps], c->stream));
     }
+    int ngroup = 0;
+    std::vector<int> tb_groups;
     for (int s = 0; s < nsteps; ++s) {
         const lf_step &t = steps[s];
-        StepArgs st{t.e32, t.wt32, t.ssin, t.scos, t.env, t.dr32, t.di32, t.dr, t.di};
-        if (timing) LF_HIP(hipEventRecord(c->ev[2 * s], c->stream));
+        StepArgs st = step_args(t);
+        const int T = c->tb;
+        if (T > 1 && c->kind == LF_TM2D && s + T <= nsteps) {
+            bool quiet = true;                       // no monitor in the block's steps
+            for (int u = 0; u < T; ++u) quiet = quiet && steps[s + u].mask == 0;
+            if (quiet) {
+                if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
+                if (int e = tm_tb_chunk(c, steps + s, T, fp64)) return e;
+                if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
+                tb_groups.push_back(ngroup);
+                ++ngroup;
+                tm = tm_args(c);
+                ma = mon_args(c);
+                s += T - 1;
+                continue;
+            }
+        }
+        if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
         if (c->kind == LF_TE2D) {
             te_h_kernel<<<grd2, blk, 0, c->stream>>>(te);
             te_e_kernel<<<grd2, blk, 0, c->stream>>>(te, c->ex, st, fp64);
@@ -1898,7 +2192,8 @@ int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
             yee_h_kernel<<<grd3, blk, 0, c->stream>>>(yee);
             yee_e_kernel<<<grd3, blk, 0, c->stream>>>(yee, c->ex, st, fp64, 0);
         }
-        if (timing) LF_HIP(hipEventRecord(c->ev[2 * s + 1], c->stream));
+        if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
+        ++ngroup;
         const unsigned mask = t.mask;
         for (int id = 0; mask && id < c->nmon; ++id) {
             if (!((mask >> id) & 1u)) continue;
@@ -1930,12 +2225,19 @@ int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
     if (timing) {
         LF_HIP(hipEventRecord(c->ev[2 * nsteps + 1], c->stream));
         LF_HIP(hipEventSynchronize(c->ev[2 * nsteps + 1]));
-        double upd = 0.0;
-        for (int s = 0; s < nsteps; ++s) {
+        double upd = 0.0, tbms = 0.0;
+        size_t k = 0;
+        for (int gq = 0; gq < ngroup; ++gq) {
             float ms = 0.f;
-            LF_HIP(hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]));
+            LF_HIP(hipEventElapsedTime(&ms, c->ev[2 * gq], c->ev[2 * gq + 1]));
             upd += ms;
+            if (k < tb_groups.size() && tb_groups[k] == gq) {
+                tbms += ms;
+                ++k;
+            }
         }
+        c->last_tb_ms = tbms;
+        c->last_tb_launches = (int)tb_groups.size();
         float tot = 0.f;
         LF_HIP(hipEventElapsedTime(&tot, c->ev[2 * nsteps], c->ev[2 * nsteps + 1]));
         c->last_update_ms = upd;

@@ -28,7 +28,7 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_set_source', 'lf_set_drive', 'lf_add_box', 'lf_add_sq', 'lf_add_dft',
            'lf_add_flux', 'lf_monitor_bytes', 'lf_monitor_read', 'lf_monitor_write',
            'lf_monitor_zero', 'lf_run', 'lf_sync', 'lf_set_timing', 'lf_kernel_ms',
-           'lf_set_variant', 'lf_set_tile')
+           'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats')
 
 
 def library_path():
@@ -71,6 +71,8 @@ def load():
         'lf_set_timing': [P, I],
         'lf_set_variant': [P, I, I],
         'lf_set_tile': [P, I],
+        'lf_set_temporal': [P, I, I],
+        'lf_tb_stats': [P, ctypes.POINTER(ctypes.c_double), ip, ip],
         'lf_kernel_ms': [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ip],
     }
     for name, args in sig.items():
@@ -316,6 +318,15 @@ class HipEngine:
 
     def set_tile(self, ty):
         _check(self._lib.lf_set_tile(self.h, int(ty)), 'lf_set_tile')
+
+    def set_temporal(self, depth, lx=64):
+        _check(self._lib.lf_set_temporal(self.h, int(depth), int(lx)), 'lf_set_temporal')
+
+    def tb_stats(self):
+        ms, n, d = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.lf_tb_stats(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(d)),
+               'lf_tb_stats')
+        return ms.value, n.value, d.value
 
     def set_timing(self, on=True):
         _check(self._lib.lf_set_timing(self.h, int(bool(on))), 'lf_set_timing')

@@ -33,6 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # coefficient read once (psi slabs, sources and monitors not counted)
 BYTES_TM2D_LOSSY = 3 * 8 + 4 * 4      # Ex, Ey, Hz r/w + Ca_ex, Ca_ey, D_ex, D_ey = 40 B
 BYTES_YEE3D = 6 * 8 + 3 * 4           # 6 fields r/w + ce_x, ce_y, ce_z = 60 B
+# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch), scaled by the same counters on a plain 1 GiB copy
+# (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth).
+TRAFFIC_BYTES = {('tm2d_stitch', 6): 725.3e6}
 
 
 def _timed_factory(records):
@@ -45,6 +49,9 @@ def _timed_factory(records):
             self._upd = 0.0
             self._tot = 0.0
             self._n = 0
+            self._tb_ms = 0.0
+            self._tb_n = 0
+            self._tb_depth = 1
             records.append(self)
 
         def run(self, tab):
@@ -53,6 +60,10 @@ def _timed_factory(records):
             self._upd += u
             self._tot += t
             self._n += n
+            ms, nl, d = self.tb_stats()
+            self._tb_ms += ms
+            self._tb_n += nl
+            self._tb_depth = d
 
     return TimedHip
 
@@ -105,8 +116,10 @@ def bench_tm2d_stitch(args):
     n_upd = sum(e._n for e in records)
     return dict(cell_updates=cell_updates, step_seconds=step_s, wall=wall, steps=n_upd,
                 upd_ms_per_step=upd_ms / max(n_upd, 1), cells_per_step=float(np.mean(cells)),
-                bytes_per_cell=BYTES_TM2D_LOSSY, transmission=float(r['transmission']),
-                windows=len(info), grid=(nx_win, ny), steps_per_window=steps)
+                bytes_per_cell=BYTES_TM2D_LOSSY, windows=len(info), grid=(nx_win, ny),
+                steps_per_window=steps, tb_ms=sum(e._tb_ms for e in records),
+                tb_launches=sum(e._tb_n for e in records), tb_depth=records[0]._tb_depth,
+                kernel='tm_tb_kernel')
 
 
 def bench_yee3d(args):
@@ -136,8 +149,9 @@ def bench_yee3d(args):
     cells = nx * ny * nz
     return dict(cell_updates=cells * steps, step_seconds=e._tot / 1e3, wall=wall, steps=e._n,
                 upd_ms_per_step=e._upd / max(e._n, 1), cells_per_step=float(cells),
-                bytes_per_cell=BYTES_YEE3D, transmission=float(r['transmission']), grid=(nx, ny, nz),
-                steps_per_window=steps)
+                bytes_per_cell=BYTES_YEE3D, grid=(nx, ny, nz), steps_per_window=steps,
+                tb_ms=e._tb_ms, tb_launches=e._tb_n, tb_depth=e._tb_depth,
+                kernel='yee_h_kernel + yee_e_kernel')
 
 
 def cpu_baseline(workload, seconds=12.0):
@@ -242,14 +256,25 @@ def main():
                    phasors='fp64')
     value = res['cell_updates'] / res['step_seconds'] / 1e6
     ms_step = res['step_seconds'] * 1e3 / max(1, res['steps'])
-    achieved = res['bytes_per_cell'] * res['cells_per_step'] / (res['upd_ms_per_step'] * 1e-3) / 1e9
+    if res['tb_launches']:
+        # dominant kernel: the temporally blocked pass (its slab and interior launches),
+        # tb_depth cell updates per cell per launch, HIP events on the context stream
+        launch_ms = res['tb_ms'] / res['tb_launches']
+        units = res['cells_per_step'] * res['tb_depth']
+        kname = f"{res['kernel']}<{res['tb_depth']}> (one blocked pass: slab + interior launches)"
+    else:
+        launch_ms = res['upd_ms_per_step']
+        units = res['cells_per_step']
+        kname = f"{res['kernel']} (one step)"
+    achieved = res['bytes_per_cell'] * units / (launch_ms * 1e-3) / 1e9
+    traffic = TRAFFIC_BYTES.get((args.workload, res['tb_depth']))
     out = dict(metric='Mcell-updates/s', value=value, unit='Mcell-updates/s', n_gpus=1,
                steps=res['steps'], warmup=args.warmup, ms_per_step=ms_step,
                higher_is_better=True, scaling='weak', vs_baseline=None, dtype='fp32',
                data='synthetic', config=cfg, device=dev,
                roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s',
-                             frac=achieved / HBM_PEAK_GBS, traffic=None,
-                             kernel='Yee H+E update (per step, HIP events)',
+                             frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=kname,
+                             launch_ms=launch_ms, cell_updates_per_launch=units,
                              bytes_per_cell_update=res['bytes_per_cell'],
                              update_ms_per_step=res['upd_ms_per_step']),
                wall_s=res['wall'])

@@ -435,6 +435,18 @@ __global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, Step
 // tm_h_kernel + tm_e_kernel (boundaries.py:243-258, fdtd2d_core.py:796).
 constexpr int kMaxTB = 8;
 
+// Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1, one VALU op each, no
+// LDS round trip).  Lane 63 (shl) and lane 0 (shr) have no source lane and keep their
+// own value, as __shfl_down / __shfl_up return there.
+__device__ __forceinline__ float lane_next(float v) {          // value of lane + 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {          // value of lane - 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x138, 0xf, 0xf, false));
+}
+
 struct TBTMArgs {
     Grid g;
     const float *Ex0, *Ey0, *Hz0;
@@ -444,31 +456,104 @@ struct TBTMArgs {
     Prof xe, xh, ye, yh;
     Strip s0[4], s1[4];       // Hzx Hzy Exy Eyx: read / write buffers
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
-    int lx, nwy;
+    int lx;
+    // wavefronts of this launch: up to 4 rectangles of (x chunk, y wave) indices; wave
+    // wv of segment g (seg_end[g-1] <= wv < seg_end[g]) is
+    // (seg_wx0 + (wv - start) / seg_nwy, seg_wy0 + (wv - start) % seg_nwy)
+    int nseg;
+    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4];
     StepArgs st[kMaxTB];
 };
 
-template <int T>
+struct TMPlane {              // one plane of one column, as loaded from the step-n buffers
+    float ex, ey, hz, ca, cy, da, dy, ph, py, qy, qx;
+};
+
+// MODE bit 1: wavefront lanes may lie in a y slab or on a y edge (psi_Hzy, psi_Exy,
+// lane tests); bit 2: planes may lie in an x slab or on an x edge (psi_Hzx, psi_Eyx,
+// plane tests).  MODE 0 is the slab-free interior: no psi registers, no edge tests.
+template <int MODE>
+__device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j, bool jin, bool jey,
+                                                 int sHzy, int sExy, bool lossy_x, bool lossy_y) {
+    constexpr bool PX = MODE & 2, PY = MODE & 1;
+    TMPlane v{};
+    const int Nx = a.g.nx, P = a.g.P;
+    if (p < Nx) {
+        const long long c = (long long)p * P + j;
+        const bool xin = !PX || p < Nx - 1;
+        if (xin && jin) {
+            v.ex = a.Ex0[c];
+            v.ca = a.cax[c];
+            if (lossy_x) v.da = a.dax[c];
+        }
+        if (jey) {
+            v.ey = a.Ey0[c];
+            v.cy = a.cay[c];
+            if (lossy_y) v.dy = a.day[c];
+        }
+        if (xin && jey) v.hz = a.Hz0[c];
+        if (PX) {
+            const int sx0 = xin ? strip_pos(a.s0[0], p) : -1;           // H nodes p <= Nx-2
+            if (sx0 >= 0 && jey) v.ph = a.s0[0].psi[(long long)sx0 * P + j];
+            const int sx3 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[3], p) : -1;
+            if (sx3 >= 0 && jey) v.qx = a.s0[3].psi[(long long)sx3 * P + j];
+        }
+        if (PY) {
+            if (sHzy >= 0 && xin) v.py = a.s0[1].psi[(long long)p * a.s0[1].len + sHzy];
+            if (sExy >= 0 && xin) v.qy = a.s0[2].psi[(long long)p * a.s0[2].len + sExy];
+        }
+    }
+    return v;
+}
+
+template <int T, int MODE>
 __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
+    constexpr bool PX = MODE & 2, PY = MODE & 1;
+    constexpr bool XIN = !PX, YIN = !PY;            // planes / lanes strictly interior
+    // slab wavefronts are few long serial chains that run beside the interior launch:
+    // they take issue priority on their SIMD so that they finish with it
+    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int lane = threadIdx.x & 63;
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int wy = wv % a.nwy, wx = wv / a.nwy;
+    // wave-uniform index held in a scalar register: every x/plane test below is a
+    // scalar branch
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wv >= a.seg_end[a.nseg - 1]) return;       // surplus waves of the last workgroup
+    int g = 0;
+    while (g < a.nseg - 1 && wv >= a.seg_end[g]) ++g;
+    const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
+    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
     const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
     const int X0 = wx * a.lx;
     if (X0 >= Nx) return;                          // whole wavefront leaves together
     const int X1 = min(X0 + a.lx, Nx);
     const int j = wy * (64 - 2 * T) - T + lane;
-    const bool jin = j >= 0 && j < Ny;
-    const bool jey = jin && j < Ny - 1;            // Ey and Hz exist in this column
+    // the host launches a MODE only on wavefronts whose loaded lanes (YIN) / planes (XIN)
+    // lie in 1..Ny-2 / 1..Nx-2 and outside every slab, so those tests are constant
+    const bool jin = YIN || (j >= 0 && j < Ny);
+    const bool jey = YIN || (jin && j < Ny - 1);   // Ey and Hz exist in this column
     const bool own = jin && lane >= T && lane <= 63 - T;
     // psi slots exist only where the single-step kernels update psi: Hz slabs for
     // j <= Ny-2 (H nodes), Ex slabs for 1 <= j <= Ny-2 (E nodes)
-    const int sHzy = jey ? strip_pos(a.s0[1], j) : -1;
-    const int sExy = (jin && j >= 1 && j <= Ny - 2) ? strip_pos(a.s0[2], j) : -1;
+    const int sHzy = (PY && jey) ? strip_pos(a.s0[1], j) : -1;
+    const int sExy = (PY && jin && j >= 1 && j <= Ny - 2) ? strip_pos(a.s0[2], j) : -1;
     float yhb = 0.f, yhc = 0.f, yhk = 1.f, yeb = 0.f, yec = 0.f, yek = 1.f;
     if (sHzy >= 0) { yhb = a.yh.b[j]; yhc = a.yh.c[j]; yhk = a.yh.k[j]; }
     if (sExy >= 0) { yeb = a.ye.b[j]; yec = a.ye.c[j]; yek = a.ye.k[j]; }
     const bool lossy_x = a.dax != nullptr, lossy_y = a.day != nullptr;
+    const int ps = X0 - T > 0 ? X0 - T : 0;
+    const int pe = X1 + T;
+    // x profiles of the wavefront's planes, plane ps + lane in lane `lane`, read per level
+    // with v_readlane (no dependent scalar loads inside the level chain)
+    const bool xpre = PX && pe - ps <= 64;
+    float xhb = 0.f, xhc = 0.f, xhk = 1.f, xeb = 0.f, xec = 0.f, xek = 1.f;
+    if (xpre) {
+        const int pl = ps + lane;
+        if (pl < Nx - 1) { xhb = a.xh.b[pl]; xhc = a.xh.c[pl]; xhk = a.xh.k[pl]; }
+        if (pl < Nx) { xeb = a.xe.b[pl]; xec = a.xe.c[pl]; xek = a.xe.k[pl]; }
+    }
+    auto plane_val = [&](float reg, const float *arr, int q) {
+        return xpre ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps)) : arr[q];
+    };
 
     // slot L holds the output of level L-1 (slot 0: loaded plane); C = this iteration,
     // P = previous iteration
@@ -483,54 +568,29 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
         hxP[L] = hyP[L] = eyyP[L] = exxP[L] = 0.f;
         cA[L] = cY[L] = dA[L] = dY[L] = 0.f;
     }
-    const int ps = X0 - T > 0 ? X0 - T : 0;
-    const int pe = X1 + T;
+    TMPlane nxt = tm_load_plane<MODE>(a, ps, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
     for (int p = ps; p < pe; ++p) {
-        // ---- slot 0: plane p of step n
-        {
-            float ex = 0.f, ey = 0.f, hz = 0.f, ca = 0.f, cy = 0.f, da = 0.f, dy = 0.f;
-            float ph = 0.f, py = 0.f, qy = 0.f, qx = 0.f;
-            if (p < Nx) {
-                const long long c = (long long)p * P + j;
-                const bool xin = p < Nx - 1;
-                if (xin && jin) {
-                    ex = a.Ex0[c];
-                    ca = a.cax[c];
-                    if (lossy_x) da = a.dax[c];
-                }
-                if (jey) {
-                    ey = a.Ey0[c];
-                    cy = a.cay[c];
-                    if (lossy_y) dy = a.day[c];
-                }
-                if (xin && jey) hz = a.Hz0[c];
-                const int sx0 = xin ? strip_pos(a.s0[0], p) : -1;             // H nodes p <= Nx-2
-                if (sx0 >= 0 && jey) ph = a.s0[0].psi[(long long)sx0 * P + j];
-                if (sHzy >= 0 && xin) py = a.s0[1].psi[(long long)p * a.s0[1].len + sHzy];
-                if (sExy >= 0 && xin) qy = a.s0[2].psi[(long long)p * a.s0[2].len + sExy];
-                const int sx3 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[3], p) : -1;
-                if (sx3 >= 0 && jey) qx = a.s0[3].psi[(long long)sx3 * P + j];
-            }
-            exC[0] = ex; eyC[0] = ey; hzC[0] = hz;
-            hxC[0] = ph; hyC[0] = py; eyyC[0] = qy; exxC[0] = qx;
-            cA[0] = ca; cY[0] = cy; dA[0] = da; dY[0] = dy;
-        }
+        // ---- slot 0: plane p of step n (loaded one iteration ahead)
+        exC[0] = nxt.ex; eyC[0] = nxt.ey; hzC[0] = nxt.hz;
+        hxC[0] = nxt.ph; hyC[0] = nxt.py; eyyC[0] = nxt.qy; exxC[0] = nxt.qx;
+        cA[0] = nxt.ca; cY[0] = nxt.cy; dA[0] = nxt.da; dY[0] = nxt.dy;
+        nxt = tm_load_plane<MODE>(a, p + 1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
         // ---- levels
 #pragma unroll
         for (int s = 0; s < T; ++s) {
             const int q = p - 1 - s;
             if (q >= ps && q < Nx) {
                 const float ex_o = exP[s], ey_o = eyP[s], hz_o = hzP[s];
-                const float ex_jp = __shfl_down(ex_o, 1);           // Ex(q, j+1)
+                const float ex_jp = lane_next(ex_o);                // Ex(q, j+1)
                 float psx = hxP[s], psy = hyP[s];
                 float hz = 0.f;
-                if (q < Nx - 1 && jey) {                            // Hz(q, j)
+                if ((XIN || q < Nx - 1) && jey) {                   // Hz(q, j)
                     const float dEyx = eyC[s] - ey_o;
                     const float dExy = ex_jp - ex_o;
                     float tx, ty;
-                    if (strip_pos(a.s0[0], q) >= 0) {
-                        psx = a.xh.b[q] * psx + a.xh.c[q] * dEyx;
-                        tx = dEyx / a.xh.k[q] + psx;
+                    if (PX && strip_pos(a.s0[0], q) >= 0) {
+                        psx = plane_val(xhb, a.xh.b, q) * psx + plane_val(xhc, a.xh.c, q) * dEyx;
+                        tx = dEyx / plane_val(xhk, a.xh.k, q) + psx;
                     } else {
                         tx = dEyx;
                     }
@@ -542,13 +602,13 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
                     }
                     hz = hz_o - a.ch * (tx - ty);
                 }
-                const float hz_jm = __shfl_up(hz, 1);               // Hz(q, j-1)
+                const float hz_jm = lane_prev(hz);                  // Hz(q, j-1)
                 float qy = eyyP[s];
                 float exn = 0.f;
-                if (q < Nx - 1 && jin) {                            // Ex(q, j)
+                if ((XIN || q < Nx - 1) && jin) {                   // Ex(q, j)
                     float v = ex_o;
                     if (lossy_x) v = v * dA[s + 1];
-                    if (j >= 1 && j <= Ny - 2) {
+                    if (YIN || (j >= 1 && j <= Ny - 2)) {
                         const float d = hz - hz_jm;
                         float t;
                         if (sExy >= 0) {
@@ -559,7 +619,7 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
                         }
                         v = v + cA[s + 1] * t;
                     }
-                    if ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1)) v = 0.0f;
+                    if (!YIN && ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1))) v = 0.0f;
                     exn = v;
                 }
                 float qx = exxP[s];
@@ -567,12 +627,12 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
                 if (jey) {                                          // Ey(q, j)
                     float v = ey_o;
                     if (lossy_y) v = v * dY[s + 1];
-                    if (q >= 1 && q <= Nx - 2) {
+                    if (XIN || (q >= 1 && q <= Nx - 2)) {
                         const float d = hz - hzP[s + 1];            // Hz(q) - Hz(q-1)
                         float t;
-                        if (strip_pos(a.s0[3], q) >= 0) {
-                            qx = a.xe.b[q] * qx + a.xe.c[q] * d;
-                            t = d / a.xe.k[q] + qx;
+                        if (PX && strip_pos(a.s0[3], q) >= 0) {
+                            qx = plane_val(xeb, a.xe.b, q) * qx + plane_val(xec, a.xe.c, q) * d;
+                            t = d / plane_val(xek, a.xe.k, q) + qx;
                         } else {
                             t = d;
                         }
@@ -580,7 +640,7 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
                     }
                     v = apply_source(v, LF_EY, q, j, x, a.st[s], fp64);
                     v = apply_drive(v, LF_EY, q, j, x, a.st[s]);
-                    if ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1)) v = 0.0f;
+                    if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     eyn = v;
                 }
                 exC[s + 1] = exn; eyC[s + 1] = eyn; hzC[s + 1] = hz;
@@ -594,17 +654,23 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
             const int q = p - T;
             if (q >= X0 && q < X1 && own) {
                 const long long c = (long long)q * P + j;
-                if (q < Nx - 1) {
+                if (XIN || q < Nx - 1) {
                     a.Ex1[c] = exC[T];
                     if (jey) a.Hz1[c] = hzC[T];
                 }
                 if (jey) a.Ey1[c] = eyC[T];
-                const int sx0 = q < Nx - 1 ? strip_pos(a.s1[0], q) : -1;
-                if (sx0 >= 0 && jey) a.s1[0].psi[(long long)sx0 * P + j] = hxC[T];
-                if (sHzy >= 0 && q < Nx - 1) a.s1[1].psi[(long long)q * a.s1[1].len + sHzy] = hyC[T];
-                if (sExy >= 0 && q < Nx - 1) a.s1[2].psi[(long long)q * a.s1[2].len + sExy] = eyyC[T];
-                const int sx3 = (q >= 1 && q <= Nx - 2) ? strip_pos(a.s1[3], q) : -1;
-                if (sx3 >= 0 && jey) a.s1[3].psi[(long long)sx3 * P + j] = exxC[T];
+                if (PX) {
+                    const int sx0 = q < Nx - 1 ? strip_pos(a.s1[0], q) : -1;
+                    if (sx0 >= 0 && jey) a.s1[0].psi[(long long)sx0 * P + j] = hxC[T];
+                    const int sx3 = (q >= 1 && q <= Nx - 2) ? strip_pos(a.s1[3], q) : -1;
+                    if (sx3 >= 0 && jey) a.s1[3].psi[(long long)sx3 * P + j] = exxC[T];
+                }
+                if (PY) {
+                    if (sHzy >= 0 && (XIN || q < Nx - 1))
+                        a.s1[1].psi[(long long)q * a.s1[1].len + sHzy] = hyC[T];
+                    if (sExy >= 0 && (XIN || q < Nx - 1))
+                        a.s1[2].psi[(long long)q * a.s1[2].len + sExy] = eyyC[T];
+                }
             }
             exP[T] = exC[T]; eyP[T] = eyC[T]; hzP[T] = hzC[T];
         }
@@ -1256,6 +1322,8 @@ struct lf_ctx {
     int nx = 0, ny = 0, nz = 0;          // logical dims (2D: nz = 1)
     Grid g{};                            // padded grid; 2D stored as (nx, 1, ny)
     hipStream_t stream = nullptr;
+    hipStream_t side[2] = {};            // side streams (default priority) for slab launches
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {};
     float *arr[LF_NARRAYS] = {};
     float *alt[LF_NARRAYS] = {};         // second buffer of ping-pong fields (fused path)
     int variant = 1;                     // 0: single-pass kernels where available, 1: H then E
@@ -1264,7 +1332,7 @@ struct lf_ctx {
     float *psi_alt3[6] = {};             // ping-pong psi of the H slabs (3D fused)
     int ty = 8;                          // y rows (waves) per workgroup of the 3D single pass
     int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
-    int tb_lx = 32;                      // x planes per wavefront chunk of the blocked pass
+    int tb_lx = 64;                      // x planes per wavefront chunk of the blocked pass
     double last_tb_ms = 0;               // blocked launches of the last lf_run: time, count
     int last_tb_launches = 0;
     float ch = 0.f;
@@ -1465,7 +1533,7 @@ int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out) {
     if (nx < 3 || ny < 3 || (kind == LF_YEE3D && nz < 3)) return fail(-1, "grid too small");
     lf_ctx *c = new lf_ctx();
     c->kind = kind; c->mode = mode;
-    if (kind == LF_TM2D) c->tb = 4;              // blocked passes over monitor-free steps
+    if (kind == LF_TM2D) c->tb = 6;              // blocked passes over monitor-free steps
     c->nx = nx; c->ny = ny; c->nz = (kind == LF_YEE3D) ? nz : 1;
     if (kind == LF_YEE3D) c->g = Grid{nx, ny, nz, round_up(nz, 64)};
     else c->g = Grid{nx, 1, ny, round_up(ny, 64)};
@@ -1514,6 +1582,10 @@ int lf_destroy(lf_ctx *c) {
             }
     }
     for (auto e : c->ev) (void)hipEventDestroy(e);
+    for (auto st : c->side) if (st) (void)hipStreamSynchronize(st);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    for (auto e : c->ev_join) if (e) (void)hipEventDestroy(e);
+    for (auto st : c->side) if (st) (void)hipStreamDestroy(st);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     (void)hipGetLastError();   // teardown errors must not surface in a later call
@@ -1796,16 +1868,105 @@ int tm_tb_chunk(lf_ctx *c, const lf_step *steps, int T, int fp64) {
     f.pec_xlo = c->pec[0][0]; f.pec_xhi = c->pec[0][1];
     f.pec_ylo = c->pec[1][0]; f.pec_yhi = c->pec[1][1];
     f.lx = c->tb_lx;
-    f.nwy = (c->g.nz + (64 - 2 * T) - 1) / (64 - 2 * T);
+    const int W = 64 - 2 * T;
+    const int Nx = c->g.nx, Ny = c->g.nz;
+    const int nwy = (Ny + W - 1) / W, nwx = (Nx + f.lx - 1) / f.lx;
     for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
-    const long long nwx = (c->g.nx + f.lx - 1) / f.lx;
-    const unsigned nblk = (unsigned)(((long long)f.nwy * nwx + 3) / 4);
-    switch (T) {
-        case 2: tm_tb_kernel<2><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
-        case 4: tm_tb_kernel<4><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
-        case 6: tm_tb_kernel<6><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
-        case 8: tm_tb_kernel<8><<<nblk, 256, 0, c->stream>>>(f, c->ex, fp64); break;
-        default: return fail(-1, "temporal block depth must be 2, 4, 6 or 8");
+    // planes / columns with psi: strip positions below lo or at/above hi
+    int xlo = 0, xhi = Nx, ylo = 0, yhi = Ny;
+    for (int q = 0; q < 4; ++q) {
+        if (!c->s2[q].psi) continue;
+        if (tm_x_slab(q)) { xlo = std::max(xlo, c->s2[q].lo); xhi = std::min(xhi, c->s2[q].hi); }
+        else { ylo = std::max(ylo, c->s2[q].lo); yhi = std::min(yhi, c->s2[q].hi); }
+    }
+    // slab-free rectangle of wavefronts: every loaded plane and lane outside all slabs
+    // and strictly inside the domain
+    int ix0 = nwx, ix1 = -1, iy0 = nwy, iy1 = -1;
+    for (int wx = 0; wx < nwx; ++wx) {
+        const int X0 = wx * f.lx, X1 = std::min(X0 + f.lx, Nx);
+        if (X0 - T >= std::max(xlo, 1) && X1 + T - 1 < std::min(xhi, Nx - 2) + 1) {
+            ix0 = std::min(ix0, wx); ix1 = std::max(ix1, wx);
+        }
+    }
+    for (int wy = 0; wy < nwy; ++wy) {
+        const int j0 = wy * W - T;
+        if (j0 >= std::max(ylo, 1) && j0 + 63 < std::min(yhi, Ny - 2) + 1) {
+            iy0 = std::min(iy0, wy); iy1 = std::max(iy1, wy);
+        }
+    }
+    const bool interior = ix1 >= ix0 && iy1 >= iy0;
+    if (T != 2 && T != 4 && T != 6 && T != 8)
+        return fail(-1, "temporal block depth must be 2, 4, 6 or 8");
+    // Three wavefront classes, each launched with only the psi registers and edge tests
+    // it needs: the slab-free rectangle (MODE 0, context stream), the rows below / above
+    // it (MODE 1, y slabs only) and the x chunks left / right of it over all rows (MODE 3,
+    // x and y slabs).  All read the step-n buffers and write disjoint cells of the
+    // step-n+T buffers, so the slab classes run on two side streams concurrently with the
+    // rectangle.  Slab launches use x chunks of lx/k planes (shorter serial chains).
+    int k = 1;
+    for (int cand : {4, 2})
+        if (f.lx % cand == 0 && f.lx / cand >= 4) { k = cand; break; }
+    const int lxs = f.lx / k;
+    const int nwxp = (Nx + lxs - 1) / lxs;
+    auto add_seg = [](TBTMArgs &q, int wx0, int nwx_s, int wy0, int nwy_s) {
+        if (nwx_s <= 0 || nwy_s <= 0) return;
+        const int start = q.nseg ? q.seg_end[q.nseg - 1] : 0;
+        q.seg_wx0[q.nseg] = wx0; q.seg_wy0[q.nseg] = wy0; q.seg_nwy[q.nseg] = nwy_s;
+        q.seg_end[q.nseg] = start + nwx_s * nwy_s;
+        ++q.nseg;
+    };
+    auto launch = [&](TBTMArgs &q, int mode, hipStream_t st) {
+        if (q.nseg == 0) return;
+        const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
+#define LF_TB_CASE(TT)                                                                      \
+    case TT:                                                                              \
+        if (mode == 0) tm_tb_kernel<TT, 0><<<nb, 256, 0, st>>>(q, c->ex, fp64);            \
+        else if (mode == 1) tm_tb_kernel<TT, 1><<<nb, 256, 0, st>>>(q, c->ex, fp64);       \
+        else if (mode == 2) tm_tb_kernel<TT, 2><<<nb, 256, 0, st>>>(q, c->ex, fp64);       \
+        else tm_tb_kernel<TT, 3><<<nb, 256, 0, st>>>(q, c->ex, fp64);                       \
+        break;
+        switch (T) { LF_TB_CASE(2) LF_TB_CASE(4) LF_TB_CASE(6) LF_TB_CASE(8) }
+#undef LF_TB_CASE
+    };
+    if (!interior) {                                 // small grid: one launch, all tests on
+        TBTMArgs q = f;
+        q.lx = lxs; q.nseg = 0;
+        add_seg(q, 0, nwxp, 0, nwy);
+        launch(q, 3, c->stream);
+    } else {
+        for (int u = 0; u < 2; ++u)
+            if (!c->side[u]) LF_HIP(hipStreamCreate(&c->side[u]));
+        if (!c->ev_fork) {
+            LF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            for (int u = 0; u < 2; ++u)
+                LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
+        }
+        LF_HIP(hipEventRecord(c->ev_fork, c->stream));
+        for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
+        const int rx0 = ix0 * k, rx1 = (ix1 + 1) * k - 1;    // rectangle in slab-launch chunks
+        const int nri = iy1 - iy0 + 1;
+        TBTMArgs qy = f, qc = f;
+        qy.lx = lxs;
+        qy.nseg = qc.nseg = 0;
+        add_seg(qy, rx0, rx1 - rx0 + 1, 0, iy0);                     // rows below
+        add_seg(qy, rx0, rx1 - rx0 + 1, iy1 + 1, nwy - 1 - iy1);     // rows above
+        // x-slab chunks (all rows): the longest chains per cell, so the finest chunks
+        int kc = 1;
+        for (int cand : {16, 8, 4, 2})
+            if (f.lx % cand == 0 && f.lx / cand >= 4) { kc = cand; break; }
+        qc.lx = f.lx / kc;
+        const int cx0 = ix0 * kc, cx1 = (ix1 + 1) * kc - 1, nwxc = (Nx + qc.lx - 1) / qc.lx;
+        add_seg(qc, 0, cx0, 0, nwy);                                 // chunks left
+        add_seg(qc, cx1 + 1, nwxc - 1 - cx1, 0, nwy);                // chunks right
+        launch(qy, 1, c->side[0]);
+        launch(qc, 3, c->side[1]);
+        f.nseg = 0;
+        add_seg(f, ix0, ix1 - ix0 + 1, iy0, nri);
+        launch(f, 0, c->stream);
+        for (int u = 0; u < 2; ++u) {
+            LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
+            LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
+        }
     }
     std::swap(c->arr[LF_EX], c->alt[LF_EX]);
     std::swap(c->arr[LF_EY], c->alt[LF_EY]);

@@ -101,11 +101,11 @@ int lf_sync(lf_ctx *ctx);
  * HIP events on the context's stream (0 if timing is off); lf_set_timing(ctx, 1) on. */
 int lf_set_timing(lf_ctx *ctx, int on);
 /* Kernel variant (no reference counterpart: an engine tuning knob): 0 = single-pass
- * H+E kernels where the kind has one, 1 = separate H and E kernels (default); lx = x
- * rows per workgroup of the single-pass kernels. */
+ * H+E kernels where the kind has one (2D TM, 3D), 1 = separate H and E kernels (default);
+ * lx = x planes per wavefront chunk of the single-pass kernels.  Results are bit-identical. */
 int lf_set_variant(lf_ctx *ctx, int variant, int lx);
-/* y rows (wavefronts) per workgroup of the 3D single-pass kernel: 2, 4 or 8 (tuning
- * knob, no reference counterpart). */
+/* Owned y rows per wavefront of the 3D single-pass kernel: 2 or 4 (tuning knob, no
+ * reference counterpart). */
 int lf_set_tile(lf_ctx *ctx, int ty);
 int lf_kernel_ms(lf_ctx *ctx, double *update_ms, double *total_ms, int *nsteps);
 /* Temporal blocking (no reference counterpart: an engine tuning knob; the reference

@@ -827,25 +827,31 @@ __global__ void __launch_bounds__(256) yee_e_kernel(YeeArgs a, Excite x, StepArg
 }
 
 
-// ---------------------------------------------------------------------------- fused 3D step
-// One pass per step.  Workgroup = TY waves; lane = z, wave = y: owns a 64 x TY (z, y)
-// tile of an x chunk of `lx` planes and marches along x.  For plane i: E^n(i) of the tile
-// plus a one-cell rim is staged in LDS, H^{n+1/2}(i) of the tile is formed (Ey/Ez of plane
-// i+1 come from registers loaded for the next plane), the rim H values the E update
-// needs (Hx/Hz at j0-1, Hx/Hy at k0-1) are recomputed, then E^{n+1}(i) is formed with
-// Hy/Hz(i-1) kept in registers.  Old and new fields (and H psi) are separate buffers, so
-// recomputed rim values always read E^n / H^{n-1/2}.  Arithmetic per cell is that of
-// yee_h_kernel + yee_e_kernel (boundaries3d.py:311-362 operation order).
-struct YFArgs {
+// ---------------------------------------------------------------------------- single-pass 3D step
+// One pass per step.  A wavefront holds 64 consecutive z cells k = wz*63 - 1 + lane
+// (lane 0 is the cell below the 63 owned cells), owns R y rows j0..j0+R-1 (row slot r
+// holds j = j0 - 1 + r; slot 0 is the row below) and marches along an x chunk [X0, X1)
+// from X0 - 1.  For plane i it forms H^{n+1/2}(i) on slots 0..R from E^n(i), E^n(i+1) and
+// H^{n-1/2}(i), then E^{n+1}(i) on the owned slots from H(i) (row below from the same
+// pass, k-1 by a DPP lane shift) and the Hy/Hz of plane i-1 kept in registers.  Old and
+// new fields and the H psi slabs are separate buffers (a recomputed halo value always
+// reads step-n data); E psi is updated in place by the owner.  Per cell the arithmetic is
+// that of yee_h_kernel + yee_e_kernel (boundaries3d.py:311-362, same operation order).
+// MODE bits: 1 = lanes may touch a z slab or z edge, 2 = rows may touch a y slab or y
+// edge, 4 = planes may touch an x slab or x edge.  A cleared bit drops those psi terms and
+// edge tests; the host only gives such waves that provably need neither.
+struct SPArgs {
     Grid g;
     const float *Ex0, *Ey0, *Ez0, *Hx0, *Hy0, *Hz0;
     float *Ex1, *Ey1, *Ez1, *Hx1, *Hy1, *Hz1;
     const float *cex, *cey, *cez, *dax, *day, *daz;
     float ch;
     Prof pe[3], ph[3];
-    Strip sH0[6], sH1[6], sE[6];
+    Strip sH0[6], sH1[6], sE[6];   // H: Hx_z Hx_y Hy_x Hy_z Hz_y Hz_x ; E: Ex_y Ex_z Ey_z Ey_x Ez_x Ez_y
     int pec[6];
     int lx;
+    int nbox;                      // wavefront boxes in (wz, wy, wx) units, wy fastest
+    int bz0[8], bnz[8], by0[8], bny[8], bx0[8], bend[8];
 };
 
 // psi update of an H slab with separate old/new buffers (store only when owned)
@@ -859,191 +865,324 @@ __device__ __forceinline__ float psi_term_pp(const Strip &s0, const Strip &s1, i
     return p + pr.km1[n] * d;
 }
 
-__device__ __forceinline__ float yf_hx(const YFArgs &a, int i, int j, int k, float d1, float d2,
-                                       bool store) {
-    float h = a.Hx0[a.g.at(i, j, k)] + a.ch * (d1 - d2);
-    int s = strip_pos(a.sH0[0], k);
-    if (s >= 0) h = h + a.ch * psi_term_pp(a.sH0[0], a.sH1[0], 2, s, a.ph[2], k, d1, i, j, k, a.g, store);
-    s = strip_pos(a.sH0[1], j);
-    if (s >= 0) h = h + (-a.ch) * psi_term_pp(a.sH0[1], a.sH1[1], 1, s, a.ph[1], j, d2, i, j, k, a.g, store);
-    return h;
+// psi slot of a slab compressed along `axis`, 32-bit element offset (grids < 2^31)
+__device__ __forceinline__ unsigned psi_off32(const Strip &s, int axis, int pos, int i, int j, int k,
+                                              const Grid &g) {
+    if (axis == 0) return ((unsigned)pos * g.ny + (unsigned)j) * g.P + (unsigned)k;
+    if (axis == 1) return ((unsigned)i * s.len + (unsigned)pos) * g.P + (unsigned)k;
+    return ((unsigned)i * g.ny + (unsigned)j) * s.len + (unsigned)pos;
 }
 
-__device__ __forceinline__ float yf_hy(const YFArgs &a, int i, int j, int k, float d1, float d2,
-                                       bool store) {
-    float h = a.Hy0[a.g.at(i, j, k)] + a.ch * (d1 - d2);
-    int s = strip_pos(a.sH0[2], i);
-    if (s >= 0) h = h + a.ch * psi_term_pp(a.sH0[2], a.sH1[2], 0, s, a.ph[0], i, d1, i, j, k, a.g, store);
-    s = strip_pos(a.sH0[3], k);
-    if (s >= 0) h = h + (-a.ch) * psi_term_pp(a.sH0[3], a.sH1[3], 2, s, a.ph[2], k, d2, i, j, k, a.g, store);
-    return h;
+// profile triple of one node of one axis
+struct PV { float b, c, m; };
+__device__ __forceinline__ PV prof_at(const Prof &p, int n) { return PV{p.b[n], p.c[n], p.km1[n]}; }
+// psi <- b psi + c d ; returns psi + (1/kappa - 1) d   (boundaries3d.py:240-251 order)
+__device__ __forceinline__ float psi_step(float &psi, const PV &q, float d) {
+    float p = psi * q.b;
+    p = p + q.c * d;
+    psi = p;
+    return p + q.m * d;
 }
 
-__device__ __forceinline__ float yf_hz(const YFArgs &a, int i, int j, int k, float d1, float d2,
-                                       bool store) {
-    float h = a.Hz0[a.g.at(i, j, k)] + a.ch * (d1 - d2);
-    int s = strip_pos(a.sH0[4], j);
-    if (s >= 0) h = h + a.ch * psi_term_pp(a.sH0[4], a.sH1[4], 1, s, a.ph[1], j, d1, i, j, k, a.g, store);
-    s = strip_pos(a.sH0[5], i);
-    if (s >= 0) h = h + (-a.ch) * psi_term_pp(a.sH0[5], a.sH1[5], 0, s, a.ph[0], i, d2, i, j, k, a.g, store);
-    return h;
-}
+// one x plane of a single-pass wavefront, as loaded from the step-n buffers: Ex and the
+// old H of plane i, the lane-63 k+1 values, coefficients, psi of every slab term, and
+// Ey / Ez of plane i+1 (the x derivatives of H(i))
+template <int R>
+struct SPPlane {
+    float exA[R + 2], eyB[R + 1], ezB[R + 2], exK[R + 1], eyK[R + 1];
+    float hxO[R + 1], hyO[R + 1], hzO[R + 1];
+    float cx[R + 1], cy[R + 1], cz[R + 1], dx[R + 1], dy[R + 1], dz[R + 1];
+    float pHxz[R + 1], pHxy[R + 1], pHyx[R + 1], pHyz[R + 1], pHzy[R + 1], pHzx[R + 1];
+    float pExy[R + 1], pExz[R + 1], pEyz[R + 1], pEyx[R + 1], pEzx[R + 1], pEzy[R + 1];
+    int xHy, xHz, xEy, xEz;
+    PV xh, xe;
+};
 
-// Barrier-free wavefront kernel.  Lane = z: k = kb - 1 + lane (lane 0 is the cell below
-// the 63 owned cells); the wavefront owns rows j0 .. j0+TY-1 of an x chunk and walks,
-// per plane i, the rows j0-1 .. j0+TY-1 (row j0-1 only forms H).  Hx/Hz(j-1) are carried
-// from the previous row, Hy/Hz(i-1) and Ey(i) from the previous plane (register arrays),
-// E/H at k+1 and H at k-1 come from wavefront shuffles.
-template <int TY>
-__global__ void __launch_bounds__(256) yee_fused_kernel(YFArgs a, Excite x, StepArgs st, int fp64,
-                                                        int nwz, int nwy) {
+template <int R, int MODE>
+__global__ void __launch_bounds__(256) yee_sp_kernel(SPArgs a, Excite x, StepArgs st, int fp64) {
+    constexpr bool PZ = MODE & 1, PY = MODE & 2, PX = MODE & 4;
     const int lane = threadIdx.x & 63;
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int wz = wv % nwz;
-    const int rest = wv / nwz;
-    const int wy = rest % nwy;
-    const int wx = rest / nwy;
-    const Grid &g = a.g;
-    const int NX = g.nx, NY = g.ny, NZ = g.nz;
-    const int i0 = wx * a.lx;
-    if (i0 >= NX) return;                         // whole wavefront leaves together
-    const int i1 = min(i0 + a.lx, NX);
-    const int ia = i0 > 0 ? i0 - 1 : 0;
-    const int k = wz * 63 + lane - 1;
-    const bool kin = k >= 0 && k < NZ;
-    const int j0 = wy * TY;
-    const long long si = (long long)NY * g.P;
-    float hy_ip[TY], hz_ip[TY], ey_ip[TY + 1];
-#pragma unroll
-    for (int r = 0; r < TY; ++r) { hy_ip[r] = 0.f; hz_ip[r] = 0.f; }
-#pragma unroll
-    for (int r = 0; r <= TY; ++r) {               // Ey(ia, j0-1+r)
-        const int jr = j0 - 1 + r;
-        ey_ip[r] = (kin && jr >= 0 && jr < NY - 1) ? a.Ey0[g.at(ia, jr, k)] : 0.f;
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wv >= a.bend[a.nbox - 1]) return;                 // surplus waves of the last block
+    int g = 0;
+    while (g < a.nbox - 1 && wv >= a.bend[g]) ++g;
+    const int loc = wv - (g > 0 ? a.bend[g - 1] : 0);
+    const int wy = a.by0[g] + loc % a.bny[g];
+    const int rest = loc / a.bny[g];
+    const int wz = a.bz0[g] + rest % a.bnz[g];
+    const int wx = a.bx0[g] + rest / a.bnz[g];
+    const Grid &G = a.g;
+    const int NX = G.nx, NY = G.ny, NZ = G.nz;
+    const int X0 = wx * a.lx;
+    if (X0 >= NX) return;                                 // whole wavefront leaves together
+    const int X1 = min(X0 + a.lx, NX);
+    const int ia = X0 > 0 ? X0 - 1 : 0;
+    const int j0 = wy * R;
+    const int k = wz * 63 - 1 + lane;
+    const bool kin = !PZ || (k >= 0 && k < NZ);
+    const bool kz1 = kin && (!PZ || k < NZ - 1);          // Ez, Hx, Hy exist at this k
+    const bool kint = !PZ || (k >= 1 && k <= NZ - 2);      // E curl along z
+    const bool kown = kin && lane >= 1;
+    const int kc = kin ? k : 0;
+    const bool lx_ = a.dax != nullptr, ly_ = a.day != nullptr, lz_ = a.daz != nullptr;
+    // 32-bit element offsets (the host launches this kernel only for grids of < 2^31
+    // elements): global loads take the SGPR base + one VGPR offset form
+    const unsigned P = G.P, si = (unsigned)NY * P;
+    auto off = [&](int i, int j) { return ((unsigned)i * NY + (unsigned)j) * P + (unsigned)kc; };
+    auto rowok = [&](int j) { return !PY || (j >= 0 && j < NY); };
+    const float ch = a.ch;
+    // z slabs: positions and profiles of this lane, fixed over the wave's life
+    int zHx = -1, zHy = -1, zEx = -1, zEy = -1;
+    PV zh{0.f, 0.f, 0.f}, ze{0.f, 0.f, 0.f};
+    if (PZ && kin) {
+        zHx = k < NZ - 1 ? strip_pos(a.sH0[0], k) : -1;
+        zHy = k < NZ - 1 ? strip_pos(a.sH0[3], k) : -1;
+        zEx = (k >= 1 && k <= NZ - 2) ? strip_pos(a.sE[1], k) : -1;
+        zEy = (k >= 1 && k <= NZ - 2) ? strip_pos(a.sE[2], k) : -1;
+        if (zHx >= 0 || zHy >= 0) zh = prof_at(a.ph[2], k);
+        if (zEx >= 0 || zEy >= 0) ze = prof_at(a.pe[2], k);
     }
-    for (int i = ia; i < i1; ++i) {
-        const bool plane_own = i >= i0;
-        const bool inx = i < NX - 1;              // Ex, Hy, Hz exist on this plane
-        // row carry: E(i, j0-1) for Ex, Ez
-        int jr = j0 - 1;
-        bool jok = jr >= 0 && jr < NY;
-        long long c = (kin && jok) ? g.at(i, jr, k) : 0;
-        float ex = (kin && jok && inx) ? a.Ex0[c] : 0.f;
-        float ez = (kin && jok && k < NZ - 1) ? a.Ez0[c] : 0.f;
-        float hx_jm = 0.f, hz_jm = 0.f;
+    // y slabs: positions and profiles per row slot, fixed over the wave's life
+    int yHx[R + 1], yHz[R + 1], yEx[R + 1], yEz[R + 1];
+    PV yh[R + 1], ye[R + 1];
 #pragma unroll
-        for (int r = 0; r <= TY; ++r) {
-            jr = j0 - 1 + r;
-            jok = jr >= 0 && jr < NY;
-            const bool cell = kin && jok;
-            c = cell ? g.at(i, jr, k) : 0;
-            const float ey = ey_ip[r];
-            // row jr+1 is loaded whenever it exists, also below the domain (jr = -1)
-            const bool nxt = kin && jr + 1 >= 0 && jr + 1 < NY;
-            const long long cn = nxt ? g.at(i, jr + 1, k) : 0;
-            const float exj = (nxt && inx) ? a.Ex0[cn] : 0.f;
-            const float ezj = (nxt && k < NZ - 1) ? a.Ez0[cn] : 0.f;
-            const float eyi = (cell && inx && jr < NY - 1) ? a.Ey0[c + si] : 0.f;
-            const float ezi = (cell && inx && k < NZ - 1) ? a.Ez0[c + si] : 0.f;
-            float exk = __shfl_down(ex, 1);
-            float eyk = __shfl_down(ey, 1);
-            if (lane == 63) {
-                const bool kn = cell && k + 1 < NZ;
-                exk = (kn && inx) ? a.Ex0[c + 1] : 0.f;
-                eyk = (kn && jr < NY - 1) ? a.Ey0[c + 1] : 0.f;
-            }
-            const bool own = plane_own && r >= 1 && lane >= 1 && cell;
-            float hx = 0.f, hy = 0.f, hz = 0.f;
-            if (cell) {
-                if (jr < NY - 1 && k < NZ - 1) {
-                    hx = yf_hx(a, i, jr, k, eyk - ey, ezj - ez, own);
-                    if (own) a.Hx1[c] = hx;
-                }
-                if (inx && k < NZ - 1) {
-                    hy = yf_hy(a, i, jr, k, ezi - ez, exk - ex, own);
-                    if (own) a.Hy1[c] = hy;
-                }
-                if (inx && jr < NY - 1) {
-                    hz = yf_hz(a, i, jr, k, exj - ex, eyi - ey, own);
-                    if (own) a.Hz1[c] = hz;
-                }
-            }
-            const float hxk = __shfl_up(hx, 1);   // Hx(k-1)
-            const float hyk = __shfl_up(hy, 1);   // Hy(k-1)
-            if (r >= 1) {
-                const int q = r - 1;
-                if (own) {
-                    if (inx) {                                               // Ex
-                        float v = ex;
-                        if (a.dax) v = v * a.dax[c];
-                        if (jr >= 1 && jr <= NY - 2 && k >= 1 && k <= NZ - 2) {
-                            const float d1 = hz - hz_jm;
-                            const float d2 = hy - hyk;
-                            const float ce = a.cex[c];
-                            v = v + ce * (d1 - d2);
-                            int s = strip_pos(a.sE[0], jr);
-                            if (s >= 0) v = v + ce * psi_term(a.sE[0], 1, s, a.pe[1], jr, d1, i, jr, k, g);
-                            s = strip_pos(a.sE[1], k);
-                            if (s >= 0) v = v + (-ce) * psi_term(a.sE[1], 2, s, a.pe[2], k, d2, i, jr, k, g);
-                        }
-                        if ((a.pec[2] && jr == 0) || (a.pec[3] && jr == NY - 1) ||
-                            (a.pec[4] && k == 0) || (a.pec[5] && k == NZ - 1))
-                            v = 0.0f;
-                        v = apply_drive(v, LF_EX, i, jr * NZ + k, x, st);
-                        a.Ex1[c] = v;
-                    }
-                    if (jr < NY - 1) {                                       // Ey
-                        float v = ey;
-                        if (a.day) v = v * a.day[c];
-                        if (i >= 1 && i <= NX - 2 && k >= 1 && k <= NZ - 2) {
-                            const float d1 = hx - hxk;
-                            const float d2 = hz - hz_ip[q];
-                            const float ce = a.cey[c];
-                            v = v + ce * (d1 - d2);
-                            int s = strip_pos(a.sE[2], k);
-                            if (s >= 0) v = v + ce * psi_term(a.sE[2], 2, s, a.pe[2], k, d1, i, jr, k, g);
-                            s = strip_pos(a.sE[3], i);
-                            if (s >= 0) v = v + (-ce) * psi_term(a.sE[3], 0, s, a.pe[0], i, d2, i, jr, k, g);
-                        }
-                        if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) ||
-                            (a.pec[4] && k == 0) || (a.pec[5] && k == NZ - 1))
-                            v = 0.0f;
-                        const int plane = jr * NZ + k;
-                        v = apply_source(v, LF_EY, i, plane, x, st, fp64);
-                        v = apply_drive(v, LF_EY, i, plane, x, st);
-                        a.Ey1[c] = v;
-                    }
-                    if (k < NZ - 1) {                                        // Ez
-                        float v = ez;
-                        if (a.daz) v = v * a.daz[c];
-                        if (i >= 1 && i <= NX - 2 && jr >= 1 && jr <= NY - 2) {
-                            const float d1 = hy - hy_ip[q];
-                            const float d2 = hx - hx_jm;
-                            const float ce = a.cez[c];
-                            v = v + ce * (d1 - d2);
-                            int s = strip_pos(a.sE[4], i);
-                            if (s >= 0) v = v + ce * psi_term(a.sE[4], 0, s, a.pe[0], i, d1, i, jr, k, g);
-                            s = strip_pos(a.sE[5], jr);
-                            if (s >= 0) v = v + (-ce) * psi_term(a.sE[5], 1, s, a.pe[1], jr, d2, i, jr, k, g);
-                        }
-                        if ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1) ||
-                            (a.pec[2] && jr == 0) || (a.pec[3] && jr == NY - 1))
-                            v = 0.0f;
-                        const int plane = jr * (NZ - 1) + k;
-                        v = apply_source(v, LF_EZ, i, plane, x, st, fp64);
-                        v = apply_drive(v, LF_EZ, i, plane, x, st);
-                        a.Ez1[c] = v;
-                    }
-                }
-                hy_ip[q] = hy;
-                hz_ip[q] = hz;
-            }
-            ey_ip[r] = eyi;
-            hx_jm = hx;
-            hz_jm = hz;
-            ex = exj;
-            ez = ezj;
+    for (int r = 0; r <= R; ++r) {
+        const int j = j0 - 1 + r;
+        yHx[r] = yHz[r] = yEx[r] = yEz[r] = -1;
+        yh[r] = PV{0.f, 0.f, 0.f}; ye[r] = PV{0.f, 0.f, 0.f};
+        if (PY && rowok(j)) {
+            if (j < NY - 1) { yHx[r] = strip_pos(a.sH0[1], j); yHz[r] = strip_pos(a.sH0[4], j); }
+            if (r >= 1 && j >= 1 && j <= NY - 2) { yEx[r] = strip_pos(a.sE[0], j); yEz[r] = strip_pos(a.sE[5], j); }
+            if (yHx[r] >= 0 || yHz[r] >= 0) yh[r] = prof_at(a.ph[1], j);
+            if (yEx[r] >= 0 || yEz[r] >= 0) ye[r] = prof_at(a.pe[1], j);
         }
+    }
+
+    auto load_plane = [&](int i, SPPlane<R> &L) {
+        const bool inside = i < X1;
+        const bool xin1 = !PX || i < NX - 1;
+        const bool xcurl = !PX || (i >= 1 && i <= NX - 2);
+        const bool own_i = i >= X0;
+        L.xHy = L.xHz = L.xEy = L.xEz = -1;
+        L.xh = PV{0.f, 0.f, 0.f}; L.xe = PV{0.f, 0.f, 0.f};
+        if (PX && inside) {
+            if (i < NX - 1) { L.xHy = strip_pos(a.sH0[2], i); L.xHz = strip_pos(a.sH0[5], i); }
+            if (xcurl) { L.xEy = strip_pos(a.sE[3], i); L.xEz = strip_pos(a.sE[4], i); }
+            if (L.xHy >= 0 || L.xHz >= 0) L.xh = prof_at(a.ph[0], i);
+            if (L.xEy >= 0 || L.xEz >= 0) L.xe = prof_at(a.pe[0], i);
+        }
+#pragma unroll
+        for (int r = 0; r <= R + 1; ++r) {
+            const int j = j0 - 1 + r;
+            const bool ro = inside && rowok(j);
+            const unsigned c = off(i, rowok(j) ? j : 0);
+            L.exA[r] = (kin && ro && xin1) ? a.Ex0[c] : 0.f;
+            L.ezB[r] = (kz1 && ro && xin1) ? a.Ez0[c + si] : 0.f;
+            if (r <= R) {
+                const bool jh = ro && j < NY - 1;
+                L.eyB[r] = (kin && jh && xin1) ? a.Ey0[c + si] : 0.f;
+                const bool k1 = lane == 63 && kin && (!PZ || k + 1 < NZ);
+                L.exK[r] = (k1 && ro && xin1) ? a.Ex0[c + 1] : 0.f;
+                L.eyK[r] = (k1 && jh) ? a.Ey0[c + 1] : 0.f;
+                L.hxO[r] = (kz1 && jh) ? a.Hx0[c] : 0.f;
+                L.hyO[r] = (kz1 && ro && xin1) ? a.Hy0[c] : 0.f;
+                L.hzO[r] = (kin && jh && xin1) ? a.Hz0[c] : 0.f;
+                const bool oe = ro && own_i && r >= 1 && kown;   // owned E cells
+                L.cx[r] = (oe && xin1) ? a.cex[c] : 0.f;
+                L.cy[r] = (oe && j < NY - 1) ? a.cey[c] : 0.f;
+                L.cz[r] = (oe && kz1) ? a.cez[c] : 0.f;
+                L.dx[r] = (oe && xin1 && lx_) ? a.dax[c] : 0.f;
+                L.dy[r] = (oe && j < NY - 1 && ly_) ? a.day[c] : 0.f;
+                L.dz[r] = (oe && kz1 && lz_) ? a.daz[c] : 0.f;
+                L.pHxz[r] = L.pHxy[r] = L.pHyx[r] = L.pHyz[r] = L.pHzy[r] = L.pHzx[r] = 0.f;
+                L.pExy[r] = L.pExz[r] = L.pEyz[r] = L.pEyx[r] = L.pEzx[r] = L.pEzy[r] = 0.f;
+                const int jj = rowok(j) ? j : 0;
+                const bool ycurl = !PY || (j >= 1 && j <= NY - 2);
+                if (kz1 && jh && zHx >= 0) L.pHxz[r] = a.sH0[0].psi[psi_off32(a.sH0[0], 2, zHx, i, jj, kc, G)];
+                if (kz1 && jh && yHx[r] >= 0) L.pHxy[r] = a.sH0[1].psi[psi_off32(a.sH0[1], 1, yHx[r], i, jj, kc, G)];
+                if (kz1 && ro && L.xHy >= 0) L.pHyx[r] = a.sH0[2].psi[psi_off32(a.sH0[2], 0, L.xHy, i, jj, kc, G)];
+                if (kz1 && ro && xin1 && zHy >= 0) L.pHyz[r] = a.sH0[3].psi[psi_off32(a.sH0[3], 2, zHy, i, jj, kc, G)];
+                if (kin && jh && xin1 && yHz[r] >= 0) L.pHzy[r] = a.sH0[4].psi[psi_off32(a.sH0[4], 1, yHz[r], i, jj, kc, G)];
+                if (kin && jh && L.xHz >= 0) L.pHzx[r] = a.sH0[5].psi[psi_off32(a.sH0[5], 0, L.xHz, i, jj, kc, G)];
+                if (oe) {                                   // E psi of the owned cells
+                    if (xin1 && yEx[r] >= 0 && kint) L.pExy[r] = a.sE[0].psi[psi_off32(a.sE[0], 1, yEx[r], i, jj, kc, G)];
+                    if (xin1 && zEx >= 0 && ycurl) L.pExz[r] = a.sE[1].psi[psi_off32(a.sE[1], 2, zEx, i, jj, kc, G)];
+                    if (jh && zEy >= 0 && xcurl) L.pEyz[r] = a.sE[2].psi[psi_off32(a.sE[2], 2, zEy, i, jj, kc, G)];
+                    if (jh && L.xEy >= 0 && kint) L.pEyx[r] = a.sE[3].psi[psi_off32(a.sE[3], 0, L.xEy, i, jj, kc, G)];
+                    if (kz1 && L.xEz >= 0 && ycurl) L.pEzx[r] = a.sE[4].psi[psi_off32(a.sE[4], 0, L.xEz, i, jj, kc, G)];
+                    if (kz1 && yEz[r] >= 0 && xcurl) L.pEzy[r] = a.sE[5].psi[psi_off32(a.sE[5], 1, yEz[r], i, jj, kc, G)];
+                }
+            }
+        }
+    };
+
+    float eyA[R + 1], ezA[R + 2], hyM[R + 1], hzM[R + 1];
+#pragma unroll
+    for (int r = 0; r <= R; ++r) { hyM[r] = 0.f; hzM[r] = 0.f; }
+#pragma unroll
+    for (int r = 0; r <= R + 1; ++r) {
+        const int j = j0 - 1 + r;
+        const unsigned c = off(ia, rowok(j) ? j : 0);
+        if (r <= R) eyA[r] = (kin && rowok(j) && j < NY - 1) ? a.Ey0[c] : 0.f;
+        ezA[r] = (kz1 && rowok(j)) ? a.Ez0[c] : 0.f;
+    }
+    SPPlane<R> cur;
+    load_plane(ia, cur);
+    for (int i = ia; i < X1; ++i) {
+        SPPlane<R> nxt;
+        load_plane(i + 1, nxt);                            // in flight while plane i runs
+        const bool own_i = i >= X0;
+        const bool xin1 = !PX || i < NX - 1;               // Ex, Hy, Hz exist on plane i
+        const bool xcurl = !PX || (i >= 1 && i <= NX - 2);
+        // ---- H^{n+1/2}(i) on slots 0..R
+        float hxN[R + 1], hyN[R + 1], hzN[R + 1];
+#pragma unroll
+        for (int r = 0; r <= R; ++r) {
+            const int j = j0 - 1 + r;
+            const bool ro = rowok(j);
+            const bool jh = ro && (!PY || j < NY - 1);
+            const bool st_h = own_i && r >= 1 && kown && ro;
+            float ey1 = lane_next(eyA[r]);
+            float ex1 = lane_next(cur.exA[r]);
+            if (lane == 63) { ey1 = cur.eyK[r]; ex1 = cur.exK[r]; }
+            float hx = 0.f, hy = 0.f, hz = 0.f;
+            const int jj = ro ? j : 0;
+            if (kz1 && jh) {                                                // Hx
+                const float d1 = ey1 - eyA[r];
+                const float d2 = ezA[r + 1] - ezA[r];
+                hx = cur.hxO[r] + ch * (d1 - d2);
+                if (zHx >= 0) {
+                    hx = hx + ch * psi_step(cur.pHxz[r], zh, d1);
+                    if (st_h) a.sH1[0].psi[psi_off32(a.sH1[0], 2, zHx, i, jj, kc, G)] = cur.pHxz[r];
+                }
+                if (PY && yHx[r] >= 0) {
+                    hx = hx + (-ch) * psi_step(cur.pHxy[r], yh[r], d2);
+                    if (st_h) a.sH1[1].psi[psi_off32(a.sH1[1], 1, yHx[r], i, jj, kc, G)] = cur.pHxy[r];
+                }
+            }
+            if (kz1 && ro && xin1) {                                        // Hy
+                const float d1 = cur.ezB[r] - ezA[r];
+                const float d2 = ex1 - cur.exA[r];
+                hy = cur.hyO[r] + ch * (d1 - d2);
+                if (PX && cur.xHy >= 0) {
+                    hy = hy + ch * psi_step(cur.pHyx[r], cur.xh, d1);
+                    if (st_h) a.sH1[2].psi[psi_off32(a.sH1[2], 0, cur.xHy, i, jj, kc, G)] = cur.pHyx[r];
+                }
+                if (zHy >= 0) {
+                    hy = hy + (-ch) * psi_step(cur.pHyz[r], zh, d2);
+                    if (st_h) a.sH1[3].psi[psi_off32(a.sH1[3], 2, zHy, i, jj, kc, G)] = cur.pHyz[r];
+                }
+            }
+            if (kin && jh && xin1) {                                        // Hz
+                const float d1 = cur.exA[r + 1] - cur.exA[r];
+                const float d2 = cur.eyB[r] - eyA[r];
+                hz = cur.hzO[r] + ch * (d1 - d2);
+                if (PY && yHz[r] >= 0) {
+                    hz = hz + ch * psi_step(cur.pHzy[r], yh[r], d1);
+                    if (st_h) a.sH1[4].psi[psi_off32(a.sH1[4], 1, yHz[r], i, jj, kc, G)] = cur.pHzy[r];
+                }
+                if (PX && cur.xHz >= 0) {
+                    hz = hz + (-ch) * psi_step(cur.pHzx[r], cur.xh, d2);
+                    if (st_h) a.sH1[5].psi[psi_off32(a.sH1[5], 0, cur.xHz, i, jj, kc, G)] = cur.pHzx[r];
+                }
+            }
+            if (st_h) {
+                const unsigned c = off(i, j);
+                if (kz1 && jh) a.Hx1[c] = hx;
+                if (kz1 && xin1) a.Hy1[c] = hy;
+                if (xin1 && jh) a.Hz1[c] = hz;
+            }
+            hxN[r] = hx; hyN[r] = hy; hzN[r] = hz;
+        }
+        // ---- E^{n+1}(i) on the owned slots 1..R
+        if (own_i) {
+#pragma unroll
+            for (int r = 1; r <= R; ++r) {
+                const int j = j0 - 1 + r;
+                if (!rowok(j)) continue;
+                const unsigned c = off(i, j);
+                const float hxk = lane_prev(hxN[r]);                       // Hx(k-1)
+                const float hyk = lane_prev(hyN[r]);                       // Hy(k-1)
+                if (!kown) continue;
+                const bool ycurl = !PY || (j >= 1 && j <= NY - 2);
+                if (xin1) {                                                // Ex
+                    float v = cur.exA[r];
+                    if (lx_) v = v * cur.dx[r];
+                    if (ycurl && kint) {
+                        const float d1 = hzN[r] - hzN[r - 1];
+                        const float d2 = hyN[r] - hyk;
+                        const float ce = cur.cx[r];
+                        v = v + ce * (d1 - d2);
+                        if (PY && yEx[r] >= 0) {
+                            v = v + ce * psi_step(cur.pExy[r], ye[r], d1);
+                            a.sE[0].psi[psi_off32(a.sE[0], 1, yEx[r], i, j, k, G)] = cur.pExy[r];
+                        }
+                        if (zEx >= 0) {
+                            v = v + (-ce) * psi_step(cur.pExz[r], ze, d2);
+                            a.sE[1].psi[psi_off32(a.sE[1], 2, zEx, i, j, k, G)] = cur.pExz[r];
+                        }
+                    }
+                    if (PY && ((a.pec[2] && j == 0) || (a.pec[3] && j == NY - 1))) v = 0.0f;
+                    if (PZ && ((a.pec[4] && k == 0) || (a.pec[5] && k == NZ - 1))) v = 0.0f;
+                    v = apply_drive(v, LF_EX, i, j * NZ + k, x, st);
+                    a.Ex1[c] = v;
+                }
+                if (!PY || j < NY - 1) {                                   // Ey
+                    float v = eyA[r];
+                    if (ly_) v = v * cur.dy[r];
+                    if (xcurl && kint) {
+                        const float d1 = hxN[r] - hxk;
+                        const float d2 = hzN[r] - hzM[r];
+                        const float ce = cur.cy[r];
+                        v = v + ce * (d1 - d2);
+                        if (zEy >= 0) {
+                            v = v + ce * psi_step(cur.pEyz[r], ze, d1);
+                            a.sE[2].psi[psi_off32(a.sE[2], 2, zEy, i, j, k, G)] = cur.pEyz[r];
+                        }
+                        if (PX && cur.xEy >= 0) {
+                            v = v + (-ce) * psi_step(cur.pEyx[r], cur.xe, d2);
+                            a.sE[3].psi[psi_off32(a.sE[3], 0, cur.xEy, i, j, k, G)] = cur.pEyx[r];
+                        }
+                    }
+                    if (PX && ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1))) v = 0.0f;
+                    if (PZ && ((a.pec[4] && k == 0) || (a.pec[5] && k == NZ - 1))) v = 0.0f;
+                    const int plane = j * NZ + k;
+                    v = apply_source(v, LF_EY, i, plane, x, st, fp64);
+                    v = apply_drive(v, LF_EY, i, plane, x, st);
+                    a.Ey1[c] = v;
+                }
+                if (kz1) {                                                 // Ez
+                    float v = ezA[r];
+                    if (lz_) v = v * cur.dz[r];
+                    if (xcurl && ycurl) {
+                        const float d1 = hyN[r] - hyM[r];
+                        const float d2 = hxN[r] - hxN[r - 1];
+                        const float ce = cur.cz[r];
+                        v = v + ce * (d1 - d2);
+                        if (PX && cur.xEz >= 0) {
+                            v = v + ce * psi_step(cur.pEzx[r], cur.xe, d1);
+                            a.sE[4].psi[psi_off32(a.sE[4], 0, cur.xEz, i, j, k, G)] = cur.pEzx[r];
+                        }
+                        if (PY && yEz[r] >= 0) {
+                            v = v + (-ce) * psi_step(cur.pEzy[r], ye[r], d2);
+                            a.sE[5].psi[psi_off32(a.sE[5], 1, yEz[r], i, j, k, G)] = cur.pEzy[r];
+                        }
+                    }
+                    if (PX && ((a.pec[0] && i == 0) || (a.pec[1] && i == NX - 1))) v = 0.0f;
+                    if (PY && ((a.pec[2] && j == 0) || (a.pec[3] && j == NY - 1))) v = 0.0f;
+                    const int plane = j * (NZ - 1) + k;
+                    v = apply_source(v, LF_EZ, i, plane, x, st, fp64);
+                    v = apply_drive(v, LF_EZ, i, plane, x, st);
+                    a.Ez1[c] = v;
+                }
+            }
+        }
+        // ---- carry to plane i+1
+#pragma unroll
+        for (int r = 0; r <= R; ++r) { hyM[r] = hyN[r]; hzM[r] = hzN[r]; eyA[r] = cur.eyB[r]; }
+#pragma unroll
+        for (int r = 0; r <= R + 1; ++r) ezA[r] = cur.ezB[r];
+        cur = nxt;
     }
 }
 
@@ -1330,7 +1469,7 @@ struct lf_ctx {
     int lx = 32;                         // x rows per workgroup in the single-pass kernels
     float *psi_alt[4] = {};              // ping-pong psi of the H slabs (2D fused)
     float *psi_alt3[6] = {};             // ping-pong psi of the H slabs (3D fused)
-    int ty = 8;                          // y rows (waves) per workgroup of the 3D single pass
+    int ty = 4;                          // owned y rows per wavefront of the 3D single pass
     int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
     int tb_lx = 64;                      // x planes per wavefront chunk of the blocked pass
     double last_tb_ms = 0;               // blocked launches of the last lf_run: time, count
@@ -2025,7 +2164,7 @@ int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
             LF_HIP(hipMalloc(&c->psi_alt3[q], n * sizeof(float)));
             LF_HIP(hipMemsetAsync(c->psi_alt3[q], 0, n * sizeof(float), c->stream));
         }
-    YFArgs f{};
+    SPArgs f{};
     f.g = c->g;
     f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Ez0 = c->arr[LF_EZ];
     f.Hx0 = c->arr[LF_HX]; f.Hy0 = c->arr[LF_HY]; f.Hz0 = c->arr[LF_HZ];
@@ -2043,13 +2182,91 @@ int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
     }
     for (int ax = 0; ax < 3; ++ax) { f.pec[2 * ax] = c->pec[ax][0]; f.pec[2 * ax + 1] = c->pec[ax][1]; }
     f.lx = c->lx;
-    const int ty = c->ty;
-    const int nwz = (c->g.nz + 62) / 63, nwy = (c->g.ny + ty - 1) / ty;
-    const long long nwx = (c->g.nx + c->lx - 1) / c->lx;
-    const unsigned nblk = (unsigned)(((long long)nwz * nwy * nwx + 3) / 4);
-    if (ty == 4) yee_fused_kernel<4><<<nblk, 256, 0, c->stream>>>(f, c->ex, st, fp64, nwz, nwy);
-    else if (ty == 2) yee_fused_kernel<2><<<nblk, 256, 0, c->stream>>>(f, c->ex, st, fp64, nwz, nwy);
-    else yee_fused_kernel<8><<<nblk, 256, 0, c->stream>>>(f, c->ex, st, fp64, nwz, nwy);
+    if ((long long)c->g.nx * c->g.ny * c->g.P >= (1LL << 31))
+        return fail(-1, "single-pass 3D kernel: grid exceeds 2^31 elements (use variant 1)");
+    const int R = c->ty == 2 ? 2 : 4;                 // owned y rows per wavefront
+    const int NX = c->g.nx, NY = c->g.ny, NZ = c->g.nz;
+    const int nw[3] = {(NZ + 62) / 63, (NY + R - 1) / R, (NX + f.lx - 1) / f.lx};   // z, y, x
+    // per axis (z, y, x): positions outside every psi slab of that axis and inside 1..N-2
+    const int N3[3] = {NZ, NY, NX};
+    const int axs[3][4] = {{0, 3, 1, 2}, {1, 4, 0, 5}, {2, 5, 3, 4}};   // sH, sH, sE, sE
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = 1; hi[d] = N3[d] - 2;
+        for (int u = 0; u < 4; ++u) {
+            const Strip &sp = u < 2 ? c->sH[axs[d][u]] : c->sE[axs[d][u]];
+            if (!sp.psi) continue;
+            lo[d] = std::max(lo[d], sp.lo);
+            hi[d] = std::min(hi[d], sp.hi - 1);
+        }
+    }
+    // interval [ia, ib] of wave indices per axis whose touched positions are all interior
+    int ia3[3], ib3[3];
+    for (int d = 0; d < 3; ++d) {
+        ia3[d] = nw[d]; ib3[d] = -1;
+        for (int w = 0; w < nw[d]; ++w) {
+            int u0, u1;
+            if (d == 0) { u0 = w * 63 - 1; u1 = w * 63 + 63; }
+            else if (d == 1) { u0 = w * R - 1; u1 = w * R + R; }
+            else { u0 = w * f.lx - 1; u1 = std::min(w * f.lx + f.lx, NX); }
+            if (u0 >= lo[d] && u1 <= hi[d]) { ia3[d] = std::min(ia3[d], w); ib3[d] = std::max(ib3[d], w); }
+        }
+    }
+    if (!c->side[0]) {
+        for (int u = 0; u < 2; ++u) LF_HIP(hipStreamCreate(&c->side[u]));
+        LF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        for (int u = 0; u < 2; ++u) LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
+    }
+    LF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
+    for (int mode = 7; mode >= 0; --mode) {
+        // intervals per axis: bit clear -> interior interval, bit set -> below / above it
+        int iv[3][2][2], niv[3];
+        for (int d = 0; d < 3; ++d) {
+            const bool edge = mode & (1 << d);
+            niv[d] = 0;
+            const bool has_int = ib3[d] >= ia3[d];
+            if (!edge) {
+                if (has_int) { iv[d][0][0] = ia3[d]; iv[d][0][1] = ib3[d] - ia3[d] + 1; niv[d] = 1; }
+            } else if (!has_int) {
+                iv[d][0][0] = 0; iv[d][0][1] = nw[d]; niv[d] = 1;
+            } else {
+                if (ia3[d] > 0) { iv[d][niv[d]][0] = 0; iv[d][niv[d]][1] = ia3[d]; ++niv[d]; }
+                if (ib3[d] < nw[d] - 1) {
+                    iv[d][niv[d]][0] = ib3[d] + 1; iv[d][niv[d]][1] = nw[d] - 1 - ib3[d]; ++niv[d];
+                }
+            }
+        }
+        SPArgs q = f;
+        q.nbox = 0;
+        for (int az = 0; az < niv[0]; ++az)
+            for (int ay = 0; ay < niv[1]; ++ay)
+                for (int ax = 0; ax < niv[2]; ++ax) {
+                    const int b = q.nbox++;
+                    q.bz0[b] = iv[0][az][0]; q.bnz[b] = iv[0][az][1];
+                    q.by0[b] = iv[1][ay][0]; q.bny[b] = iv[1][ay][1];
+                    q.bx0[b] = iv[2][ax][0];
+                    const int cnt = iv[0][az][1] * iv[1][ay][1] * iv[2][ax][1];
+                    q.bend[b] = (b ? q.bend[b - 1] : 0) + cnt;
+                }
+        if (q.nbox == 0) continue;
+        const unsigned nb = (unsigned)(((long long)q.bend[q.nbox - 1] + 3) / 4);
+        hipStream_t sm = mode == 0 ? c->stream : (mode == 1 ? c->side[0] : c->side[1]);
+#define LF_SP_CASE(MM)                                                                        \
+    case MM:                                                                                \
+        if (R == 2) yee_sp_kernel<2, MM><<<nb, 256, 0, sm>>>(q, c->ex, st, fp64);             \
+        else yee_sp_kernel<4, MM><<<nb, 256, 0, sm>>>(q, c->ex, st, fp64);                    \
+        break;
+        switch (mode) {
+            LF_SP_CASE(0) LF_SP_CASE(1) LF_SP_CASE(2) LF_SP_CASE(3)
+            LF_SP_CASE(4) LF_SP_CASE(5) LF_SP_CASE(6) LF_SP_CASE(7)
+        }
+#undef LF_SP_CASE
+    }
+    for (int u = 0; u < 2; ++u) {
+        LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
+        LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
+    }
     for (int q = 0; q < 6; ++q) std::swap(c->arr[ids[q]], c->alt[ids[q]]);
     for (int q = 0; q < 6; ++q)
         if (c->sH[q].psi) std::swap(c->sH[q].psi, c->psi_alt3[q]);
@@ -2060,7 +2277,7 @@ int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
 
 extern "C" int lf_set_tile(lf_ctx *c, int ty) {
     if (int e = check_ctx(c)) return e;
-    if (ty != 2 && ty != 4 && ty != 8) return fail(-1, "lf_set_tile: ty must be 2, 4 or 8");
+    if (ty != 2 && ty != 4) return fail(-1, "lf_set_tile: rows per wavefront must be 2 or 4");
     c->ty = ty;
     return 0;
 }

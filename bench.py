@@ -154,6 +154,24 @@ def bench_yee3d(args):
                 kernel='yee_h_kernel + yee_e_kernel')
 
 
+def bench_sweep(args):
+    """64-instance batched 2D sweep vs the same 64 solves one after another with one step
+    per launch; both over the same seeded instances and the same steps per solve."""
+    from lens_sensor_amd.planar_beads import sweep
+    ps = sweep.sweep_instances(64, seed=args.seed)
+    cap = args.steps_override or 12288
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    rb, tb, sb = sweep.run_sweep(ps, batched=True, steps_cap=cap, log=log)
+    rs, ts, ss = sweep.run_sweep(ps, batched=False, steps_cap=cap, log=log)
+    same = all(np.array_equal([a['transmission'], a['S_in'], a['S_out'], a['ecw_norm']],
+                              [b['transmission'], b['S_in'], b['S_out'], b['ecw_norm']], equal_nan=True)
+               for a, b in zip(rb, rs))
+    cell_updates = sum(r['cells'] * r['steps'] for r in rb)
+    return dict(solves=len(ps), batched_s=tb, sequential_s=ts, setup_s=(sb + ss) / 2,
+                steps_cap=cap, identical=same, cell_updates=cell_updates,
+                full_steps=[int(x) for x in [min(r['steps'] for r in rb), max(r['steps'] for r in rb)]])
+
+
 def cpu_baseline(workload, seconds=12.0):
     """NumPy oracle (one core) on a bounded sample: the same grid and coefficients, as many
     steps as fit in ~``seconds``; Mcell-updates/s."""
@@ -226,7 +244,7 @@ def main():
     ap.add_argument('--steps', type=int, default=None,
                     help='accepted for the driver contract: the workload fixes its steps')
     ap.add_argument('--warmup', type=int, default=0)
-    ap.add_argument('--workload', default='tm2d_stitch', choices=('tm2d_stitch', 'yee3d'))
+    ap.add_argument('--workload', default='tm2d_stitch', choices=('tm2d_stitch', 'yee3d', 'sweep'))
     ap.add_argument('--windows', type=int, default=None, help='tm2d_stitch: windows to run')
     ap.add_argument('--steps-override', type=int, default=None,
                     help='steps per window (reduced runs for profiling only)')
@@ -244,6 +262,24 @@ def main():
         sys.exit(2)
     from lens_sensor_amd.engine import hip
     dev = hip.device_name()
+    if args.workload == 'sweep':
+        r = bench_sweep(args)
+        out = dict(metric='solves/s', value=r['solves'] / r['batched_s'], unit='solves/s',
+                   n_gpus=1, steps=r['steps_cap'], warmup=0,
+                   ms_per_step=r['batched_s'] * 1e3 / r['steps_cap'], higher_is_better=True,
+                   scaling='weak', vs_baseline=None, dtype='fp32', data='synthetic',
+                   config=dict(workload='sweep_64x2d_te_tm', instances=r['solves'],
+                               steps_per_solve=r['steps_cap'], dx_nm=25.0, npml=24,
+                               fields='fp32', phasors='fp64'),
+                   device=dev,
+                   sequential=dict(value=r['solves'] / r['sequential_s'], unit='solves/s',
+                                   note='same solves, one instance after another, one step per launch'),
+                   speedup=r['sequential_s'] / r['batched_s'], identical_results=r['identical'],
+                   mcups_batched=r['cell_updates'] / r['batched_s'] / 1e6,
+                   mcups_sequential=r['cell_updates'] / r['sequential_s'] / 1e6,
+                   setup_s_per_pass=r['setup_s'])
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == 'tm2d_stitch':
         res = bench_tm2d_stitch(args)
         cfg = dict(workload='tm2d_stitch_10x8192x1536', windows=res['windows'],

@@ -117,6 +117,27 @@ int lf_set_temporal(lf_ctx *ctx, int depth, int lx);
  * launches, current depth. */
 int lf_tb_stats(lf_ctx *ctx, double *ms, int *launches, int *depth);
 
+/* ---- Batch engine: many independent 2D domains advanced together.  No reference
+ * counterpart: the reference runs sweep instances and requests one after another, each
+ * through run_beads / run_beads_stitched (run_simulation.py:290-431).  Instances keep
+ * their own fields, coefficients, sources and monitors (set up with the calls above);
+ * every temporally blocked pass is ONE launch per wavefront class covering all instances
+ * that take it, the others step one step per launch.  Results are bit-identical to
+ * running each context alone. */
+typedef struct lf_batch lf_batch;
+int lf_batch_create(lf_batch **out);
+/* finalized 2D context, same precision mode for all instances */
+int lf_batch_add(lf_batch *b, lf_ctx *ctx);
+int lf_batch_set_temporal(lf_batch *b, int depth, int lx);
+int lf_batch_set_timing(lf_batch *b, int on);
+/* steps: shared step table; masks: ninst x nsteps monitor masks or NULL (steps[].mask);
+ * nact: steps per instance in this call (<= nsteps) or NULL (all nsteps) */
+int lf_batch_run(lf_batch *b, const lf_step *steps, int nsteps, const unsigned int *masks,
+                 const int *nact);
+/* last lf_batch_run with timing on: HIP-event time (ms) and blocked passes */
+int lf_batch_stats(lf_batch *b, double *ms, int *passes);
+int lf_batch_destroy(lf_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -506,17 +506,14 @@ __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j
     return v;
 }
 
+// Body of one wavefront (index wv within its instance's launch segments); st[s] are the
+// per-step scalars of step n+s.
 template <int T, int MODE>
-__global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
+__device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, const StepArgs *st,
+                                           int wv, int fp64) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
     constexpr bool XIN = !PX, YIN = !PY;            // planes / lanes strictly interior
-    // slab wavefronts are few long serial chains that run beside the interior launch:
-    // they take issue priority on their SIMD so that they finish with it
-    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int lane = threadIdx.x & 63;
-    // wave-uniform index held in a scalar register: every x/plane test below is a
-    // scalar branch
-    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wv >= a.seg_end[a.nseg - 1]) return;       // surplus waves of the last workgroup
     int g = 0;
     while (g < a.nseg - 1 && wv >= a.seg_end[g]) ++g;
@@ -638,8 +635,8 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
                         }
                         v = v + (-cY[s + 1]) * t;
                     }
-                    v = apply_source(v, LF_EY, q, j, x, a.st[s], fp64);
-                    v = apply_drive(v, LF_EY, q, j, x, a.st[s]);
+                    v = apply_source(v, LF_EY, q, j, x, st[s], fp64);
+                    v = apply_drive(v, LF_EY, q, j, x, st[s]);
                     if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     eyn = v;
                 }
@@ -679,6 +676,275 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
             cA[L] = cA[L - 1]; cY[L] = cY[L - 1]; dA[L] = dA[L - 1]; dY[L] = dY[L - 1];
         }
     }
+}
+
+// slab wavefronts are few long serial chains that run beside the interior launch: they
+// take issue priority on their SIMD so that they finish with it.  The wave index is read
+// into a scalar register, so every x/plane test is a scalar branch.
+template <int T, int MODE>
+__global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
+    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    tm_tb_body<T, MODE>(a, x, a.st, wv, fp64);
+}
+
+// ---------------------------------------------------------------------------- temporal blocking (2D TE)
+// Same pipeline as the TM pass with the roles of E and H exchanged: level s advances
+// plane q = p-1-s from step n+s to n+s+1: Hx^{n+s+1/2}(q) needs Ez(q, j+1) (lane above),
+// Hy^{n+s+1/2}(q) needs Ez(q+1) (this iteration's output of level s-1), and
+// Ez^{n+s+1}(q) needs Hy(q-1) (own previous output) and Hx(q, j-1) (lane below).  Per
+// cell and step the arithmetic is that of te_h_kernel + te_e_kernel
+// (boundaries.py:189-215).  psi slabs: Hxy (y), Hyx (x), Ezx (x), Ezy (y).
+struct TBTEArgs {
+    Grid g;
+    const float *Ez0, *Hx0, *Hy0;
+    float *Ez1, *Hx1, *Hy1;
+    const float *ceE, *ceH;
+    float ch;
+    Prof xe, xh, ye, yh;
+    Strip s0[4], s1[4];       // Hxy Hyx Ezx Ezy: read / write buffers
+    int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
+    int lx;
+    int nseg;
+    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4];
+    StepArgs st[kMaxTB];
+};
+
+struct TEPlane {
+    float ez, hx, hy, ce, cH, pxy, pyx, pzx, pzy;
+};
+
+template <int MODE>
+__device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int j, bool jin, bool jhx,
+                                                 int sHxy, int sEzy) {
+    constexpr bool PX = MODE & 2, PY = MODE & 1;
+    TEPlane v{};
+    const int Nx = a.g.nx, P = a.g.P;
+    if (p < Nx) {
+        const long long c = (long long)p * P + j;
+        const bool xin = !PX || p < Nx - 1;            // Hy exists on this plane
+        if (jin) {
+            v.ez = a.Ez0[c];
+            v.ce = a.ceE[c];
+            v.cH = a.ceH[c];
+            if (xin) v.hy = a.Hy0[c];
+        }
+        if (jhx) v.hx = a.Hx0[c];
+        if (PX) {
+            const int sx1 = xin ? strip_pos(a.s0[1], p) : -1;           // H nodes p <= Nx-2
+            if (sx1 >= 0 && jin) v.pyx = a.s0[1].psi[(long long)sx1 * P + j];
+            const int sx2 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[2], p) : -1;
+            if (sx2 >= 0 && jin) v.pzx = a.s0[2].psi[(long long)sx2 * P + j];
+        }
+        if (PY) {
+            if (sHxy >= 0) v.pxy = a.s0[0].psi[(long long)p * a.s0[0].len + sHxy];
+            if (sEzy >= 0 && (!PX || (p >= 1 && p <= Nx - 2)))
+                v.pzy = a.s0[3].psi[(long long)p * a.s0[3].len + sEzy];
+        }
+    }
+    return v;
+}
+
+template <int T, int MODE>
+__device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, const StepArgs *st,
+                                           int wv, int fp64) {
+    constexpr bool PX = MODE & 2, PY = MODE & 1;
+    constexpr bool XIN = !PX, YIN = !PY;
+    const int lane = threadIdx.x & 63;
+    if (wv >= a.seg_end[a.nseg - 1]) return;
+    int g = 0;
+    while (g < a.nseg - 1 && wv >= a.seg_end[g]) ++g;
+    const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
+    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
+    const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
+    const int X0 = wx * a.lx;
+    if (X0 >= Nx) return;
+    const int X1 = min(X0 + a.lx, Nx);
+    const int j = wy * (64 - 2 * T) - T + lane;
+    const bool jin = YIN || (j >= 0 && j < Ny);
+    const bool jhx = YIN || (jin && j < Ny - 1);   // Hx exists in this column
+    const bool jcurl = YIN || (j >= 1 && j <= Ny - 2);
+    const bool own = jin && lane >= T && lane <= 63 - T;
+    const int sHxy = (PY && jhx) ? strip_pos(a.s0[0], j) : -1;
+    const int sEzy = (PY && jin && j >= 1 && j <= Ny - 2) ? strip_pos(a.s0[3], j) : -1;
+    float yhb = 0.f, yhc = 0.f, yhk = 1.f, yeb = 0.f, yec = 0.f, yek = 1.f;
+    if (sHxy >= 0) { yhb = a.yh.b[j]; yhc = a.yh.c[j]; yhk = a.yh.k[j]; }
+    if (sEzy >= 0) { yeb = a.ye.b[j]; yec = a.ye.c[j]; yek = a.ye.k[j]; }
+    const int ps = X0 - T > 0 ? X0 - T : 0;
+    const int pe = X1 + T;
+    const bool xpre = PX && pe - ps <= 64;
+    float xhb = 0.f, xhc = 0.f, xhk = 1.f, xeb = 0.f, xec = 0.f, xek = 1.f;
+    if (xpre) {
+        const int pl = ps + lane;
+        if (pl < Nx - 1) { xhb = a.xh.b[pl]; xhc = a.xh.c[pl]; xhk = a.xh.k[pl]; }
+        if (pl < Nx) { xeb = a.xe.b[pl]; xec = a.xe.c[pl]; xek = a.xe.k[pl]; }
+    }
+    auto plane_val = [&](float reg, const float *arr, int q) {
+        return xpre ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps)) : arr[q];
+    };
+
+    float ezC[T + 1], hxC[T + 1], hyC[T + 1], ezP[T + 1], hxP[T + 1], hyP[T + 1];
+    float axyC[T + 1], ayxC[T + 1], azxC[T + 1], azyC[T + 1];   // psi Hxy Hyx Ezx Ezy
+    float axyP[T + 1], ayxP[T + 1], azxP[T + 1], azyP[T + 1];
+    float cE[T + 1], cH[T + 1];
+#pragma unroll
+    for (int L = 0; L <= T; ++L) {
+        ezC[L] = hxC[L] = hyC[L] = ezP[L] = hxP[L] = hyP[L] = 0.f;
+        axyC[L] = ayxC[L] = azxC[L] = azyC[L] = 0.f;
+        axyP[L] = ayxP[L] = azxP[L] = azyP[L] = 0.f;
+        cE[L] = cH[L] = 0.f;
+    }
+    TEPlane nxt = te_load_plane<MODE>(a, ps, j, jin, jhx, sHxy, sEzy);
+    for (int p = ps; p < pe; ++p) {
+        ezC[0] = nxt.ez; hxC[0] = nxt.hx; hyC[0] = nxt.hy;
+        axyC[0] = nxt.pxy; ayxC[0] = nxt.pyx; azxC[0] = nxt.pzx; azyC[0] = nxt.pzy;
+        cE[0] = nxt.ce; cH[0] = nxt.cH;
+        nxt = te_load_plane<MODE>(a, p + 1, j, jin, jhx, sHxy, sEzy);
+#pragma unroll
+        for (int s = 0; s < T; ++s) {
+            const int q = p - 1 - s;
+            if (q >= ps && q < Nx) {
+                const float ez_o = ezP[s];
+                const float ez_jp = lane_next(ez_o);                // Ez(q, j+1)
+                float qxy = axyP[s], qyx = ayxP[s];
+                float hx = 0.f, hy = 0.f;
+                if (jhx) {                                          // Hx(q, j)
+                    const float d = ez_jp - ez_o;
+                    float t;
+                    if (sHxy >= 0) {
+                        qxy = yhb * qxy + yhc * d;
+                        t = d / yhk + qxy;
+                    } else {
+                        t = d;
+                    }
+                    hx = hxP[s] - a.ch * t;
+                }
+                if ((XIN || q < Nx - 1) && jin) {                   // Hy(q, j)
+                    const float d = ezC[s] - ez_o;
+                    float t;
+                    if (PX && strip_pos(a.s0[1], q) >= 0) {
+                        qyx = plane_val(xhb, a.xh.b, q) * qyx + plane_val(xhc, a.xh.c, q) * d;
+                        t = d / plane_val(xhk, a.xh.k, q) + qyx;
+                    } else {
+                        t = d;
+                    }
+                    hy = hyP[s] + a.ch * t;
+                }
+                const float hx_jm = lane_prev(hx);                  // Hx(q, j-1)
+                float qzx = azxP[s], qzy = azyP[s];
+                float ezn = 0.f;
+                if (jin) {                                          // Ez(q, j)
+                    float v = ez_o;
+                    if ((XIN || (q >= 1 && q <= Nx - 2)) && jcurl) {
+                        const float dHy = hy - hyP[s + 1];          // Hy(q) - Hy(q-1)
+                        const float dHx = hx - hx_jm;
+                        float t;
+                        if (PX && strip_pos(a.s0[2], q) >= 0) {
+                            qzx = plane_val(xeb, a.xe.b, q) * qzx + plane_val(xec, a.xe.c, q) * dHy;
+                            t = dHy / plane_val(xek, a.xe.k, q) + qzx;
+                        } else {
+                            t = dHy;
+                        }
+                        if (sEzy >= 0) {
+                            qzy = yeb * qzy + yec * dHx;
+                            t = (t - dHx / yek) - qzy;
+                        } else {
+                            t = t - dHx;
+                        }
+                        v = cE[s + 1] * v + cH[s + 1] * t;
+                    }
+                    v = apply_source(v, LF_EZ, q, j, x, st[s], fp64);
+                    v = apply_drive(v, LF_EZ, q, j, x, st[s]);
+                    if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
+                    if (!YIN && ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1))) v = 0.0f;
+                    ezn = v;
+                }
+                ezC[s + 1] = ezn; hxC[s + 1] = hx; hyC[s + 1] = hy;
+                axyC[s + 1] = qxy; ayxC[s + 1] = qyx; azxC[s + 1] = qzx; azyC[s + 1] = qzy;
+            }
+            ezP[s] = ezC[s]; hxP[s] = hxC[s]; hyP[s] = hyC[s];
+            axyP[s] = axyC[s]; ayxP[s] = ayxC[s]; azxP[s] = azxC[s]; azyP[s] = azyC[s];
+        }
+        {
+            const int q = p - T;
+            if (q >= X0 && q < X1 && own) {
+                const long long c = (long long)q * P + j;
+                a.Ez1[c] = ezC[T];
+                if (jhx) a.Hx1[c] = hxC[T];
+                if (XIN || q < Nx - 1) a.Hy1[c] = hyC[T];
+                if (PX) {
+                    const int sx1 = q < Nx - 1 ? strip_pos(a.s1[1], q) : -1;
+                    if (sx1 >= 0) a.s1[1].psi[(long long)sx1 * P + j] = ayxC[T];
+                    const int sx2 = (q >= 1 && q <= Nx - 2) ? strip_pos(a.s1[2], q) : -1;
+                    if (sx2 >= 0) a.s1[2].psi[(long long)sx2 * P + j] = azxC[T];
+                }
+                if (PY) {
+                    if (sHxy >= 0) a.s1[0].psi[(long long)q * a.s1[0].len + sHxy] = axyC[T];
+                    if (sEzy >= 0 && (XIN || (q >= 1 && q <= Nx - 2)))
+                        a.s1[3].psi[(long long)q * a.s1[3].len + sEzy] = azyC[T];
+                }
+            }
+            ezP[T] = ezC[T]; hxP[T] = hxC[T]; hyP[T] = hyC[T];
+        }
+#pragma unroll
+        for (int L = T; L >= 1; --L) { cE[L] = cE[L - 1]; cH[L] = cH[L - 1]; }
+    }
+}
+
+template <int T, int MODE>
+__global__ void __launch_bounds__(256) te_tb_kernel(TBTEArgs a, Excite x, int fp64) {
+    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    te_tb_body<T, MODE>(a, x, a.st, wv, fp64);
+}
+
+// ---------------------------------------------------------------------------- batched passes
+// Many independent 2D domains (sweep instances, served requests) advance T steps in ONE
+// launch per wavefront class: a device table holds each instance's pass arguments, a
+// prefix table maps the launch's wavefront index to (instance, wavefront within it), and
+// the per-step scalars come from a device step table shared by the instances.
+__host__ __device__ __forceinline__ StepArgs step_args_of(const lf_step &t) {
+    return StepArgs{t.e32, t.wt32, t.ssin, t.scos, t.env, t.dr32, t.di32, t.dr, t.di};
+}
+
+struct TBRecTM { TBTMArgs a; Excite x; };
+struct TBRecTE { TBTEArgs a; Excite x; };
+
+__device__ __forceinline__ int batch_instance(const int *start, int ninst, int wv) {
+    int lo = 0, hi = ninst - 1;                    // largest b with start[b] <= wv
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= wv) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int T, int MODE>
+__global__ void __launch_bounds__(256) tm_tb_batch_kernel(const TBRecTM *__restrict__ rec,
+                                                          const int *__restrict__ start, int ninst,
+                                                          const lf_step *__restrict__ steps, int fp64) {
+    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wv >= start[ninst]) return;
+    const int b = batch_instance(start, ninst, wv);
+    StepArgs st[T];
+#pragma unroll
+    for (int u = 0; u < T; ++u) st[u] = step_args_of(steps[u]);
+    tm_tb_body<T, MODE>(rec[b].a, rec[b].x, st, wv - start[b], fp64);
+}
+
+template <int T, int MODE>
+__global__ void __launch_bounds__(256) te_tb_batch_kernel(const TBRecTE *__restrict__ rec,
+                                                          const int *__restrict__ start, int ninst,
+                                                          const lf_step *__restrict__ steps, int fp64) {
+    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wv >= start[ninst]) return;
+    const int b = batch_instance(start, ninst, wv);
+    StepArgs st[T];
+#pragma unroll
+    for (int u = 0; u < T; ++u) st[u] = step_args_of(steps[u]);
+    te_tb_body<T, MODE>(rec[b].a, rec[b].x, st, wv - start[b], fp64);
 }
 
 // ============================================================================ 3D kernels
@@ -1977,52 +2243,39 @@ StepArgs step_args(const lf_step &t) {
     return StepArgs{t.e32, t.wt32, t.ssin, t.scos, t.env, t.dr32, t.di32, t.dr, t.di};
 }
 
-// TM psi slabs compressed along x: Hzx (0) and Eyx (3); Hzy (1) and Exy (2) along y
-bool tm_x_slab(int q) { return q == 0 || q == 3; }
+// 2D psi slabs compressed along x: TM Hzx (0), Eyx (3); TE Hyx (1), Ezx (2); the others
+// are compressed along y
+bool x_slab_2d(int kind, int q) { return kind == LF_TM2D ? (q == 0 || q == 3) : (q == 1 || q == 2); }
+bool tm_x_slab(int q) { return x_slab_2d(LF_TM2D, q); }
 
-// one temporally blocked TM pass over T steps: reads arr[], writes alt[], then swaps
-int tm_tb_chunk(lf_ctx *c, const lf_step *steps, int T, int fp64) {
-    if (int r = ensure_alt(c, LF_EX)) return r;
-    if (int r = ensure_alt(c, LF_EY)) return r;
-    if (int r = ensure_alt(c, LF_HZ)) return r;
-    for (int q = 0; q < 4; ++q)
-        if (c->s2[q].psi && !c->psi_alt[q]) {
-            const size_t n = strip_elems_2d(c, c->s2[q], tm_x_slab(q));
-            LF_HIP(hipMalloc(&c->psi_alt[q], n * sizeof(float)));
-            LF_HIP(hipMemsetAsync(c->psi_alt[q], 0, n * sizeof(float), c->stream));
-        }
-    TBTMArgs f{};
-    f.g = c->g;
-    f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Hz0 = c->arr[LF_HZ];
-    f.Ex1 = c->alt[LF_EX]; f.Ey1 = c->alt[LF_EY]; f.Hz1 = c->alt[LF_HZ];
-    f.cax = c->arr[LF_CAX]; f.cay = c->arr[LF_CAY]; f.dax = c->arr[LF_DAX]; f.day = c->arr[LF_DAY];
-    f.ch = c->ch;
-    f.xe = prof_of(c, 0, false); f.xh = prof_of(c, 0, true);
-    f.ye = prof_of(c, 1, false); f.yh = prof_of(c, 1, true);
-    for (int q = 0; q < 4; ++q) {
-        f.s0[q] = c->s2[q];
-        f.s1[q] = c->s2[q];
-        f.s1[q].psi = c->psi_alt[q];
-    }
-    f.pec_xlo = c->pec[0][0]; f.pec_xhi = c->pec[0][1];
-    f.pec_ylo = c->pec[1][0]; f.pec_yhi = c->pec[1][1];
-    f.lx = c->tb_lx;
+// Wavefront plan of one temporally blocked 2D pass: up to three classes, each with the
+// x chunk length and the (x chunk, y wave) rectangles it covers.  Class 0 (MODE 0): the
+// slab-free rectangle, whose loaded planes and lanes lie in 1..N-2 and outside every
+// slab; class 1 (MODE 1): the rows below / above it (y slabs only); class 2 (MODE 3):
+// the x chunks left / right of it over all rows, or every wavefront of a grid without a
+// slab-free rectangle.
+struct TBClass {
+    int mode, lx, nseg;
+    int wx0[4], wy0[4], nwy[4], end[4];
+};
+struct TBPlan {
+    TBClass cls[3];
+    bool interior;
+};
+
+TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
     const int W = 64 - 2 * T;
     const int Nx = c->g.nx, Ny = c->g.nz;
-    const int nwy = (Ny + W - 1) / W, nwx = (Nx + f.lx - 1) / f.lx;
-    for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
-    // planes / columns with psi: strip positions below lo or at/above hi
+    const int nwy = (Ny + W - 1) / W, nwx = (Nx + lx - 1) / lx;
     int xlo = 0, xhi = Nx, ylo = 0, yhi = Ny;
     for (int q = 0; q < 4; ++q) {
         if (!c->s2[q].psi) continue;
-        if (tm_x_slab(q)) { xlo = std::max(xlo, c->s2[q].lo); xhi = std::min(xhi, c->s2[q].hi); }
+        if (x_slab_2d(c->kind, q)) { xlo = std::max(xlo, c->s2[q].lo); xhi = std::min(xhi, c->s2[q].hi); }
         else { ylo = std::max(ylo, c->s2[q].lo); yhi = std::min(yhi, c->s2[q].hi); }
     }
-    // slab-free rectangle of wavefronts: every loaded plane and lane outside all slabs
-    // and strictly inside the domain
     int ix0 = nwx, ix1 = -1, iy0 = nwy, iy1 = -1;
     for (int wx = 0; wx < nwx; ++wx) {
-        const int X0 = wx * f.lx, X1 = std::min(X0 + f.lx, Nx);
+        const int X0 = wx * lx, X1 = std::min(X0 + lx, Nx);
         if (X0 - T >= std::max(xlo, 1) && X1 + T - 1 < std::min(xhi, Nx - 2) + 1) {
             ix0 = std::min(ix0, wx); ix1 = std::max(ix1, wx);
         }
@@ -2033,85 +2286,189 @@ int tm_tb_chunk(lf_ctx *c, const lf_step *steps, int T, int fp64) {
             iy0 = std::min(iy0, wy); iy1 = std::max(iy1, wy);
         }
     }
-    const bool interior = ix1 >= ix0 && iy1 >= iy0;
-    if (T != 2 && T != 4 && T != 6 && T != 8)
-        return fail(-1, "temporal block depth must be 2, 4, 6 or 8");
-    // Three wavefront classes, each launched with only the psi registers and edge tests
-    // it needs: the slab-free rectangle (MODE 0, context stream), the rows below / above
-    // it (MODE 1, y slabs only) and the x chunks left / right of it over all rows (MODE 3,
-    // x and y slabs).  All read the step-n buffers and write disjoint cells of the
-    // step-n+T buffers, so the slab classes run on two side streams concurrently with the
-    // rectangle.  Slab launches use x chunks of lx/k planes (shorter serial chains).
-    int k = 1;
-    for (int cand : {4, 2})
-        if (f.lx % cand == 0 && f.lx / cand >= 4) { k = cand; break; }
-    const int lxs = f.lx / k;
-    const int nwxp = (Nx + lxs - 1) / lxs;
-    auto add_seg = [](TBTMArgs &q, int wx0, int nwx_s, int wy0, int nwy_s) {
+    TBPlan pl{};
+    pl.interior = ix1 >= ix0 && iy1 >= iy0;
+    auto add = [](TBClass &q, int wx0, int nwx_s, int wy0, int nwy_s) {
         if (nwx_s <= 0 || nwy_s <= 0) return;
-        const int start = q.nseg ? q.seg_end[q.nseg - 1] : 0;
-        q.seg_wx0[q.nseg] = wx0; q.seg_wy0[q.nseg] = wy0; q.seg_nwy[q.nseg] = nwy_s;
-        q.seg_end[q.nseg] = start + nwx_s * nwy_s;
+        const int start = q.nseg ? q.end[q.nseg - 1] : 0;
+        q.wx0[q.nseg] = wx0; q.wy0[q.nseg] = wy0; q.nwy[q.nseg] = nwy_s;
+        q.end[q.nseg] = start + nwx_s * nwy_s;
         ++q.nseg;
     };
-    auto launch = [&](TBTMArgs &q, int mode, hipStream_t st) {
-        if (q.nseg == 0) return;
-        const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
-#define LF_TB_CASE(TT)                                                                      \
-    case TT:                                                                              \
-        if (mode == 0) tm_tb_kernel<TT, 0><<<nb, 256, 0, st>>>(q, c->ex, fp64);            \
-        else if (mode == 1) tm_tb_kernel<TT, 1><<<nb, 256, 0, st>>>(q, c->ex, fp64);       \
-        else if (mode == 2) tm_tb_kernel<TT, 2><<<nb, 256, 0, st>>>(q, c->ex, fp64);       \
-        else tm_tb_kernel<TT, 3><<<nb, 256, 0, st>>>(q, c->ex, fp64);                       \
-        break;
-        switch (T) { LF_TB_CASE(2) LF_TB_CASE(4) LF_TB_CASE(6) LF_TB_CASE(8) }
-#undef LF_TB_CASE
+    auto div = [&](int want) {
+        for (int cand : {want, 8, 4, 2})
+            if (cand <= want && lx % cand == 0 && lx / cand >= 4) return cand;
+        return 1;
     };
-    if (!interior) {                                 // small grid: one launch, all tests on
-        TBTMArgs q = f;
-        q.lx = lxs; q.nseg = 0;
-        add_seg(q, 0, nwxp, 0, nwy);
-        launch(q, 3, c->stream);
-    } else {
-        for (int u = 0; u < 2; ++u)
-            if (!c->side[u]) LF_HIP(hipStreamCreate(&c->side[u]));
-        if (!c->ev_fork) {
-            LF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-            for (int u = 0; u < 2; ++u)
-                LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
-        }
-        LF_HIP(hipEventRecord(c->ev_fork, c->stream));
-        for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
-        const int rx0 = ix0 * k, rx1 = (ix1 + 1) * k - 1;    // rectangle in slab-launch chunks
-        const int nri = iy1 - iy0 + 1;
-        TBTMArgs qy = f, qc = f;
-        qy.lx = lxs;
-        qy.nseg = qc.nseg = 0;
-        add_seg(qy, rx0, rx1 - rx0 + 1, 0, iy0);                     // rows below
-        add_seg(qy, rx0, rx1 - rx0 + 1, iy1 + 1, nwy - 1 - iy1);     // rows above
-        // x-slab chunks (all rows): the longest chains per cell, so the finest chunks
-        int kc = 1;
-        for (int cand : {16, 8, 4, 2})
-            if (f.lx % cand == 0 && f.lx / cand >= 4) { kc = cand; break; }
-        qc.lx = f.lx / kc;
-        const int cx0 = ix0 * kc, cx1 = (ix1 + 1) * kc - 1, nwxc = (Nx + qc.lx - 1) / qc.lx;
-        add_seg(qc, 0, cx0, 0, nwy);                                 // chunks left
-        add_seg(qc, cx1 + 1, nwxc - 1 - cx1, 0, nwy);                // chunks right
-        launch(qy, 1, c->side[0]);
-        launch(qc, 3, c->side[1]);
-        f.nseg = 0;
-        add_seg(f, ix0, ix1 - ix0 + 1, iy0, nri);
-        launch(f, 0, c->stream);
-        for (int u = 0; u < 2; ++u) {
-            LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
-            LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
-        }
+    pl.cls[0].mode = 0; pl.cls[1].mode = 1; pl.cls[2].mode = 3;
+    if (!pl.interior) {
+        const int k = div(4);
+        pl.cls[2].lx = lx / k;
+        add(pl.cls[2], 0, (Nx + lx / k - 1) / (lx / k), 0, nwy);
+        return pl;
     }
-    std::swap(c->arr[LF_EX], c->alt[LF_EX]);
-    std::swap(c->arr[LF_EY], c->alt[LF_EY]);
-    std::swap(c->arr[LF_HZ], c->alt[LF_HZ]);
+    const int ni = iy1 - iy0 + 1;
+    pl.cls[0].lx = lx;
+    add(pl.cls[0], ix0, ix1 - ix0 + 1, iy0, ni);
+    const int k = div(4);                                 // y-slab rows: chunks of lx/k
+    pl.cls[1].lx = lx / k;
+    const int rx0 = ix0 * k, rx1 = (ix1 + 1) * k - 1;
+    add(pl.cls[1], rx0, rx1 - rx0 + 1, 0, iy0);
+    add(pl.cls[1], rx0, rx1 - rx0 + 1, iy1 + 1, nwy - 1 - iy1);
+    const int kc = div(16);                               // x-slab chunks: the finest
+    pl.cls[2].lx = lx / kc;
+    const int cx0 = ix0 * kc, cx1 = (ix1 + 1) * kc - 1, nwxc = (Nx + lx / kc - 1) / (lx / kc);
+    add(pl.cls[2], 0, cx0, 0, nwy);
+    add(pl.cls[2], cx1 + 1, nwxc - 1 - cx1, 0, nwy);
+    return pl;
+}
+
+template <typename A>
+void set_class(A &q, const TBClass &k) {
+    q.lx = k.lx;
+    q.nseg = k.nseg;
+    for (int u = 0; u < 4; ++u) {
+        q.seg_wx0[u] = k.wx0[u]; q.seg_wy0[u] = k.wy0[u]; q.seg_nwy[u] = k.nwy[u]; q.seg_end[u] = k.end[u];
+    }
+}
+
+int ensure_side_streams(lf_ctx *c) {
+    for (int u = 0; u < 2; ++u)
+        if (!c->side[u]) LF_HIP(hipStreamCreate(&c->side[u]));
+    if (!c->ev_fork) {
+        LF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        for (int u = 0; u < 2; ++u) LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
+    }
+    return 0;
+}
+
+template <int T, int MODE>
+void launch_tb(const TBTMArgs &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
+    tm_tb_kernel<T, MODE><<<nb, 256, 0, st>>>(q, x, fp64);
+}
+template <int T, int MODE>
+void launch_tb(const TBTEArgs &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
+    te_tb_kernel<T, MODE><<<nb, 256, 0, st>>>(q, x, fp64);
+}
+template <typename A, int T>
+void launch_tb_mode(const A &q, int mode, const Excite &x, int fp64, hipStream_t st) {
+    const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
+    if (mode == 0) launch_tb<T, 0>(q, x, fp64, st, nb);
+    else if (mode == 1) launch_tb<T, 1>(q, x, fp64, st, nb);
+    else launch_tb<T, 3>(q, x, fp64, st, nb);
+}
+
+// Launch the classes of one blocked pass.  Without a slab-free rectangle: one launch on
+// the context stream.  Otherwise the rectangle on the context stream and the two slab
+// classes on two side streams, concurrently (disjoint output cells, step-n inputs).
+template <typename A>
+int launch_tb_pass(lf_ctx *c, A &f, const TBPlan &pl, int T, int fp64) {
+    auto one = [&](const TBClass &k, hipStream_t st) {
+        if (k.nseg == 0) return;
+        A q = f;
+        set_class(q, k);
+        switch (T) {
+            case 2: launch_tb_mode<A, 2>(q, k.mode, c->ex, fp64, st); break;
+            case 4: launch_tb_mode<A, 4>(q, k.mode, c->ex, fp64, st); break;
+            case 6: launch_tb_mode<A, 6>(q, k.mode, c->ex, fp64, st); break;
+            default: launch_tb_mode<A, 8>(q, k.mode, c->ex, fp64, st); break;
+        }
+    };
+    if (!pl.interior) {
+        one(pl.cls[2], c->stream);
+        return 0;
+    }
+    if (int e = ensure_side_streams(c)) return e;
+    LF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
+    one(pl.cls[1], c->side[0]);
+    one(pl.cls[2], c->side[1]);
+    one(pl.cls[0], c->stream);
+    for (int u = 0; u < 2; ++u) {
+        LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
+        LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
+    }
+    return 0;
+}
+
+int ensure_psi_alt_2d(lf_ctx *c) {
+    for (int q = 0; q < 4; ++q)
+        if (c->s2[q].psi && !c->psi_alt[q]) {
+            const size_t n = strip_elems_2d(c, c->s2[q], x_slab_2d(c->kind, q));
+            LF_HIP(hipMalloc(&c->psi_alt[q], n * sizeof(float)));
+            LF_HIP(hipMemsetAsync(c->psi_alt[q], 0, n * sizeof(float), c->stream));
+        }
+    return 0;
+}
+
+TBTMArgs tm_tb_args(lf_ctx *c) {
+    TBTMArgs f{};
+    f.g = c->g;
+    f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Hz0 = c->arr[LF_HZ];
+    f.Ex1 = c->alt[LF_EX]; f.Ey1 = c->alt[LF_EY]; f.Hz1 = c->alt[LF_HZ];
+    f.cax = c->arr[LF_CAX]; f.cay = c->arr[LF_CAY]; f.dax = c->arr[LF_DAX]; f.day = c->arr[LF_DAY];
+    f.ch = c->ch;
+    f.xe = prof_of(c, 0, false); f.xh = prof_of(c, 0, true);
+    f.ye = prof_of(c, 1, false); f.yh = prof_of(c, 1, true);
+    for (int q = 0; q < 4; ++q) { f.s0[q] = c->s2[q]; f.s1[q] = c->s2[q]; f.s1[q].psi = c->psi_alt[q]; }
+    f.pec_xlo = c->pec[0][0]; f.pec_xhi = c->pec[0][1];
+    f.pec_ylo = c->pec[1][0]; f.pec_yhi = c->pec[1][1];
+    return f;
+}
+
+TBTEArgs te_tb_args(lf_ctx *c) {
+    TBTEArgs f{};
+    f.g = c->g;
+    f.Ez0 = c->arr[LF_EZ]; f.Hx0 = c->arr[LF_HX]; f.Hy0 = c->arr[LF_HY];
+    f.Ez1 = c->alt[LF_EZ]; f.Hx1 = c->alt[LF_HX]; f.Hy1 = c->alt[LF_HY];
+    f.ceE = c->arr[LF_CEE]; f.ceH = c->arr[LF_CEH];
+    f.ch = c->ch;
+    f.xe = prof_of(c, 0, false); f.xh = prof_of(c, 0, true);
+    f.ye = prof_of(c, 1, false); f.yh = prof_of(c, 1, true);
+    for (int q = 0; q < 4; ++q) { f.s0[q] = c->s2[q]; f.s1[q] = c->s2[q]; f.s1[q].psi = c->psi_alt[q]; }
+    f.pec_xlo = c->pec[0][0]; f.pec_xhi = c->pec[0][1];
+    f.pec_ylo = c->pec[1][0]; f.pec_yhi = c->pec[1][1];
+    return f;
+}
+
+// fields of the blocked passes (ping-pong) per kind
+void swap_tb_buffers(lf_ctx *c) {
+    if (c->kind == LF_TM2D) {
+        std::swap(c->arr[LF_EX], c->alt[LF_EX]);
+        std::swap(c->arr[LF_EY], c->alt[LF_EY]);
+        std::swap(c->arr[LF_HZ], c->alt[LF_HZ]);
+    } else {
+        std::swap(c->arr[LF_EZ], c->alt[LF_EZ]);
+        std::swap(c->arr[LF_HX], c->alt[LF_HX]);
+        std::swap(c->arr[LF_HY], c->alt[LF_HY]);
+    }
     for (int q = 0; q < 4; ++q)
         if (c->s2[q].psi) std::swap(c->s2[q].psi, c->psi_alt[q]);
+}
+
+int ensure_tb_buffers(lf_ctx *c) {
+    const int ids[2][3] = {{LF_EZ, LF_HX, LF_HY}, {LF_EX, LF_EY, LF_HZ}};
+    for (int q = 0; q < 3; ++q)
+        if (int r = ensure_alt(c, ids[c->kind == LF_TM2D][q])) return r;
+    return ensure_psi_alt_2d(c);
+}
+
+// one temporally blocked 2D pass over T steps: reads arr[], writes alt[], then swaps
+int tb_chunk_2d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
+    if (T != 2 && T != 4 && T != 6 && T != 8)
+        return fail(-1, "temporal block depth must be 2, 4, 6 or 8");
+    if (int r = ensure_tb_buffers(c)) return r;
+    const TBPlan pl = tb_plan_2d(c, T, c->tb_lx);
+    if (c->kind == LF_TM2D) {
+        TBTMArgs f = tm_tb_args(c);
+        for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
+        if (int e = launch_tb_pass(c, f, pl, T, fp64)) return e;
+    } else {
+        TBTEArgs f = te_tb_args(c);
+        for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
+        if (int e = launch_tb_pass(c, f, pl, T, fp64)) return e;
+    }
+    swap_tb_buffers(c);
     return 0;
 }
 
@@ -2502,8 +2859,22 @@ int lf_sync(lf_ctx *c) {
     return 0;
 }
 
-int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
+}  // extern "C"
+
+namespace {
+int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb);
+}
+
+extern "C" int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
     if (int e = check_ctx(c)) return e;
+    return run_impl(c, steps, nsteps, true);
+}
+
+namespace {
+
+// the step loop of lf_run; allow_tb = false runs one step per launch (the batch engine
+// advances the instances it does not block itself this way)
+int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
     if (!c->finalized) return fail(-1, "lf_run before lf_finalize");
     if (nsteps <= 0) return 0;
     if (c->ch == 0.0f) return fail(-1, "lf_run: Ch not set");
@@ -2537,17 +2908,18 @@ int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
     for (int s = 0; s < nsteps; ++s) {
         const lf_step &t = steps[s];
         StepArgs st = step_args(t);
-        const int T = c->tb;
-        if (T > 1 && c->kind == LF_TM2D && s + T <= nsteps) {
+        const int T = allow_tb ? c->tb : 1;
+        if (T > 1 && c->kind != LF_YEE3D && s + T <= nsteps) {
             bool quiet = true;                       // no monitor in the block's steps
             for (int u = 0; u < T; ++u) quiet = quiet && steps[s + u].mask == 0;
             if (quiet) {
                 if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
-                if (int e = tm_tb_chunk(c, steps + s, T, fp64)) return e;
+                if (int e = tb_chunk_2d(c, steps + s, T, fp64)) return e;
                 if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
                 tb_groups.push_back(ngroup);
                 ++ngroup;
-                tm = tm_args(c);
+                if (c->kind == LF_TM2D) tm = tm_args(c);
+                else te = te_args(c);
                 ma = mon_args(c);
                 s += T - 1;
                 continue;
@@ -2622,6 +2994,281 @@ int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
         c->last_total_ms = tot;
         c->last_steps = nsteps;
     }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+}  // extern "C"
+
+// ============================================================================ batch engine (C ABI)
+struct lf_batch {
+    std::vector<lf_ctx *> ctx;
+    hipStream_t stream = nullptr, side[2] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {};
+    int tb = 6, tb_lx = 64;
+    void *d_rec[2][3] = {};                 // [TE, TM][class] records
+    size_t rec_cap[2][3] = {};
+    int *d_start[2][3] = {};
+    size_t start_cap[2][3] = {};
+    lf_step *d_steps = nullptr;
+    size_t steps_cap = 0;
+    std::vector<lf_step> slice;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double last_ms = 0.0, last_pass_ms = 0.0;
+    int last_passes = 0;
+};
+
+namespace {
+
+int batch_check(lf_batch *b) { return b ? 0 : fail(-1, "null batch"); }
+
+template <typename Rec>
+int upload_records(lf_batch *b, int kind, int cls, const std::vector<Rec> &rec,
+                   const std::vector<int> &start) {
+    const int kk = kind == LF_TM2D;
+    const size_t rb = rec.size() * sizeof(Rec), sb = start.size() * sizeof(int);
+    if (rb > b->rec_cap[kk][cls]) {
+        if (b->d_rec[kk][cls]) LF_HIP(hipFree(b->d_rec[kk][cls]));
+        LF_HIP(hipMalloc(&b->d_rec[kk][cls], rb));
+        b->rec_cap[kk][cls] = rb;
+    }
+    if (sb > b->start_cap[kk][cls]) {
+        if (b->d_start[kk][cls]) LF_HIP(hipFree(b->d_start[kk][cls]));
+        LF_HIP(hipMalloc(&b->d_start[kk][cls], sb));
+        b->start_cap[kk][cls] = sb;
+    }
+    // pageable sources: the runtime stages them before returning, so the host vectors may
+    // be reused; the copies are ordered before the pass on the batch stream
+    LF_HIP(hipMemcpyAsync(b->d_rec[kk][cls], rec.data(), rb, hipMemcpyHostToDevice, b->stream));
+    LF_HIP(hipMemcpyAsync(b->d_start[kk][cls], start.data(), sb, hipMemcpyHostToDevice, b->stream));
+    return 0;
+}
+
+template <int T>
+void launch_batch_class(lf_batch *b, int kind, int cls, int mode, int ninst, int nwave,
+                        const lf_step *dsteps, int fp64, hipStream_t st) {
+    const int kk = kind == LF_TM2D;
+    const unsigned nb = (unsigned)(((long long)nwave + 3) / 4);
+    const int *start = b->d_start[kk][cls];
+    if (kind == LF_TM2D) {
+        const TBRecTM *rec = (const TBRecTM *)b->d_rec[kk][cls];
+        if (mode == 0) tm_tb_batch_kernel<T, 0><<<nb, 256, 0, st>>>(rec, start, ninst, dsteps, fp64);
+        else if (mode == 1) tm_tb_batch_kernel<T, 1><<<nb, 256, 0, st>>>(rec, start, ninst, dsteps, fp64);
+        else tm_tb_batch_kernel<T, 3><<<nb, 256, 0, st>>>(rec, start, ninst, dsteps, fp64);
+    } else {
+        const TBRecTE *rec = (const TBRecTE *)b->d_rec[kk][cls];
+        if (mode == 0) te_tb_batch_kernel<T, 0><<<nb, 256, 0, st>>>(rec, start, ninst, dsteps, fp64);
+        else if (mode == 1) te_tb_batch_kernel<T, 1><<<nb, 256, 0, st>>>(rec, start, ninst, dsteps, fp64);
+        else te_tb_batch_kernel<T, 3><<<nb, 256, 0, st>>>(rec, start, ninst, dsteps, fp64);
+    }
+}
+
+// one blocked pass of depth T over the listed instances (all share the step table)
+int batch_pass(lf_batch *b, const std::vector<int> &ids, const lf_step *dsteps, int T, int fp64) {
+    struct Launch { int kind, cls, mode, ninst, nwave; };
+    std::vector<Launch> launches;
+    for (int kind : {LF_TE2D, LF_TM2D}) {
+        for (int cls = 0; cls < 3; ++cls) {
+            std::vector<int> start(1, 0);
+            std::vector<TBRecTM> rtm;
+            std::vector<TBRecTE> rte;
+            int mode = cls == 0 ? 0 : (cls == 1 ? 1 : 3);
+            for (int id : ids) {
+                lf_ctx *c = b->ctx[id];
+                if (c->kind != kind) continue;
+                const TBPlan pl = tb_plan_2d(c, T, b->tb_lx);
+                const TBClass &k = pl.cls[cls];
+                if (k.nseg == 0) continue;
+                if (kind == LF_TM2D) {
+                    TBRecTM r{};
+                    r.a = tm_tb_args(c);
+                    set_class(r.a, k);
+                    r.x = c->ex;
+                    rtm.push_back(r);
+                } else {
+                    TBRecTE r{};
+                    r.a = te_tb_args(c);
+                    set_class(r.a, k);
+                    r.x = c->ex;
+                    rte.push_back(r);
+                }
+                start.push_back(start.back() + k.end[k.nseg - 1]);
+            }
+            const int n = (int)start.size() - 1;
+            if (n == 0) continue;
+            if (kind == LF_TM2D) { if (int e = upload_records(b, kind, cls, rtm, start)) return e; }
+            else { if (int e = upload_records(b, kind, cls, rte, start)) return e; }
+            launches.push_back(Launch{kind, cls, mode, n, start.back()});
+        }
+    }
+    if (launches.empty()) return 0;
+    LF_HIP(hipEventRecord(b->ev_fork, b->stream));
+    for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(b->side[u], b->ev_fork, 0));
+    for (const Launch &L : launches) {
+        hipStream_t st = L.cls == 0 ? b->stream : b->side[L.cls - 1];
+        switch (T) {
+            case 2: launch_batch_class<2>(b, L.kind, L.cls, L.mode, L.ninst, L.nwave, dsteps, fp64, st); break;
+            case 4: launch_batch_class<4>(b, L.kind, L.cls, L.mode, L.ninst, L.nwave, dsteps, fp64, st); break;
+            case 6: launch_batch_class<6>(b, L.kind, L.cls, L.mode, L.ninst, L.nwave, dsteps, fp64, st); break;
+            default: launch_batch_class<8>(b, L.kind, L.cls, L.mode, L.ninst, L.nwave, dsteps, fp64, st); break;
+        }
+    }
+    LF_HIP(hipGetLastError());
+    for (int u = 0; u < 2; ++u) {
+        LF_HIP(hipEventRecord(b->ev_join[u], b->side[u]));
+        LF_HIP(hipStreamWaitEvent(b->stream, b->ev_join[u], 0));
+    }
+    for (int id : ids) swap_tb_buffers(b->ctx[id]);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lf_batch_create(lf_batch **out) {
+    if (!out) return fail(-1, "null out");
+    lf_batch *b = new lf_batch();
+    hipError_t e = hipStreamCreate(&b->stream);
+    for (int u = 0; u < 2 && e == hipSuccess; ++u) e = hipStreamCreate(&b->side[u]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    for (int u = 0; u < 2 && e == hipSuccess; ++u) e = hipEventCreateWithFlags(&b->ev_join[u], hipEventDisableTiming);
+    if (e != hipSuccess) { delete b; return fail(-2, hipGetErrorString(e)); }
+    *out = b;
+    return 0;
+}
+
+int lf_batch_add(lf_batch *b, lf_ctx *c) {
+    if (int e = batch_check(b)) return e;
+    if (int e = check_ctx(c)) return e;
+    if (!c->finalized) return fail(-1, "lf_batch_add: context not finalized");
+    if (c->kind == LF_YEE3D) return fail(-1, "lf_batch_add: 2D contexts only");
+    if (c->mode != (b->ctx.empty() ? c->mode : b->ctx[0]->mode))
+        return fail(-1, "lf_batch_add: all instances need the same precision mode");
+    for (lf_ctx *o : b->ctx) if (o == c) return fail(-1, "lf_batch_add: context already in batch");
+    LF_HIP(hipStreamSynchronize(c->stream));
+    b->ctx.push_back(c);
+    return 0;
+}
+
+int lf_batch_set_temporal(lf_batch *b, int depth, int lx) {
+    if (int e = batch_check(b)) return e;
+    if (depth != 1 && depth != 2 && depth != 4 && depth != 6 && depth != 8)
+        return fail(-1, "lf_batch_set_temporal: depth must be 1 (off), 2, 4, 6 or 8");
+    if (lx < 1 || lx > 1 << 20) return fail(-1, "lf_batch_set_temporal: lx out of range");
+    b->tb = depth;
+    b->tb_lx = lx;
+    return 0;
+}
+
+int lf_batch_set_timing(lf_batch *b, int on) {
+    if (int e = batch_check(b)) return e;
+    b->timing = on != 0;
+    return 0;
+}
+
+/* Advance every instance over the shared step table.  masks (ninst x nsteps, may be
+ * NULL: steps[].mask for all) are per-instance monitor masks; nact (may be NULL: nsteps)
+ * the number of steps each instance takes in this call. */
+int lf_batch_run(lf_batch *b, const lf_step *steps, int nsteps, const unsigned *masks,
+                 const int *nact) {
+    if (int e = batch_check(b)) return e;
+    if (nsteps <= 0 || b->ctx.empty()) return 0;
+    const int ni = (int)b->ctx.size();
+    for (lf_ctx *c : b->ctx) {
+        if (c->ch == 0.0f) return fail(-1, "lf_batch_run: Ch not set");
+        if (c->kind == LF_TE2D && (!c->arr[LF_CEE] || !c->arr[LF_CEH]))
+            return fail(-1, "lf_batch_run: Ce_E/Ce_H not uploaded");
+        if (c->kind == LF_TM2D && (!c->arr[LF_CAX] || !c->arr[LF_CAY]))
+            return fail(-1, "lf_batch_run: Ca_ex/Ca_ey not uploaded");
+        if (int r = ensure_tb_buffers(c)) return r;
+    }
+    const int fp64 = b->ctx[0]->mode == LF_MODE_FP64ACC;
+    if ((size_t)nsteps > b->steps_cap) {
+        if (b->d_steps) LF_HIP(hipFree(b->d_steps));
+        LF_HIP(hipMalloc(&b->d_steps, nsteps * sizeof(lf_step)));
+        b->steps_cap = nsteps;
+    }
+    LF_HIP(hipMemcpyAsync(b->d_steps, steps, nsteps * sizeof(lf_step), hipMemcpyHostToDevice, b->stream));
+    // every instance works on the batch stream for the duration of the call
+    std::vector<hipStream_t> own(ni);
+    for (int i = 0; i < ni; ++i) {
+        LF_HIP(hipStreamSynchronize(b->ctx[i]->stream));
+        own[i] = b->ctx[i]->stream;
+        b->ctx[i]->stream = b->stream;
+    }
+    auto mask_of = [&](int i, int s) { return masks ? masks[(size_t)i * nsteps + s] : steps[s].mask; };
+    auto act = [&](int i) { return nact ? std::min(nact[i], nsteps) : nsteps; };
+    const bool timing = b->timing;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing) {
+        while (b->ev.size() < 2) { hipEvent_t e; LF_HIP(hipEventCreate(&e)); b->ev.push_back(e); }
+        t0 = b->ev[0]; t1 = b->ev[1];
+        LF_HIP(hipEventRecord(t0, b->stream));
+    }
+    int rc = 0, passes = 0;
+    int s = 0;
+    std::vector<int> blocked, single;
+    while (s < nsteps && rc == 0) {
+        const int T = b->tb;
+        blocked.clear(); single.clear();
+        for (int i = 0; i < ni; ++i) {
+            if (s >= act(i)) continue;
+            bool q = T > 1 && s + T <= act(i);
+            for (int u = 0; q && u < T; ++u) q = mask_of(i, s + u) == 0;
+            (q ? blocked : single).push_back(i);
+        }
+        if (blocked.empty() && single.empty()) break;
+        const int adv = blocked.empty() ? 1 : T;
+        for (int i : single) {                     // one step per launch, own masks
+            const int n = std::min(adv, act(i) - s);
+            b->slice.assign(steps + s, steps + s + n);
+            for (int u = 0; u < n; ++u) b->slice[u].mask = mask_of(i, s + u);
+            if ((rc = run_impl(b->ctx[i], b->slice.data(), n, false)) != 0) break;
+        }
+        if (rc == 0 && !blocked.empty()) {
+            rc = batch_pass(b, blocked, b->d_steps + s, T, fp64);
+            ++passes;
+        }
+        s += adv;
+    }
+    if (timing && rc == 0) {
+        LF_HIP(hipEventRecord(t1, b->stream));
+        LF_HIP(hipEventSynchronize(t1));
+        float ms = 0.f;
+        LF_HIP(hipEventElapsedTime(&ms, t0, t1));
+        b->last_ms = ms;
+        b->last_passes = passes;
+    }
+    for (int i = 0; i < ni; ++i) b->ctx[i]->stream = own[i];
+    if (rc == 0) LF_HIP(hipStreamSynchronize(b->stream));
+    return rc;
+}
+
+int lf_batch_stats(lf_batch *b, double *ms, int *passes) {
+    if (int e = batch_check(b)) return e;
+    *ms = b->last_ms;
+    *passes = b->last_passes;
+    return 0;
+}
+
+int lf_batch_destroy(lf_batch *b) {
+    if (!b) return 0;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &r : b->d_rec) for (auto p : r) if (p) (void)hipFree(p);
+    for (auto &r : b->d_start) for (auto p : r) if (p) (void)hipFree(p);
+    if (b->d_steps) (void)hipFree(b->d_steps);
+    for (auto e : b->ev) (void)hipEventDestroy(e);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    for (auto e : b->ev_join) if (e) (void)hipEventDestroy(e);
+    for (auto st : b->side) if (st) (void)hipStreamDestroy(st);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    (void)hipGetLastError();
     return 0;
 }
 

@@ -28,7 +28,9 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_set_source', 'lf_set_drive', 'lf_add_box', 'lf_add_sq', 'lf_add_dft',
            'lf_add_flux', 'lf_monitor_bytes', 'lf_monitor_read', 'lf_monitor_write',
            'lf_monitor_zero', 'lf_run', 'lf_sync', 'lf_set_timing', 'lf_kernel_ms',
-           'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats')
+           'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats',
+           'lf_batch_create', 'lf_batch_add', 'lf_batch_set_temporal', 'lf_batch_set_timing',
+           'lf_batch_run', 'lf_batch_stats', 'lf_batch_destroy')
 
 
 def library_path():
@@ -74,6 +76,13 @@ def load():
         'lf_set_temporal': [P, I, I],
         'lf_tb_stats': [P, ctypes.POINTER(ctypes.c_double), ip, ip],
         'lf_kernel_ms': [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ip],
+        'lf_batch_create': [ctypes.POINTER(P)],
+        'lf_batch_add': [P, P],
+        'lf_batch_set_temporal': [P, I, I],
+        'lf_batch_set_timing': [P, I],
+        'lf_batch_run': [P, P, I, P, P],
+        'lf_batch_stats': [P, ctypes.POINTER(ctypes.c_double), ip],
+        'lf_batch_destroy': [P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -336,3 +345,62 @@ class HipEngine:
         _check(self._lib.lf_kernel_ms(self.h, ctypes.byref(u), ctypes.byref(t), ctypes.byref(n)),
                'lf_kernel_ms')
         return u.value, t.value, n.value
+
+
+class HipBatch:
+    """Many 2D HipEngine instances advanced together through the batch engine
+    (lf_batch_*): every temporally blocked pass is one launch per wavefront class over all
+    instances that take it.  The engines must stay open while the batch exists."""
+
+    def __init__(self, engines=(), depth=6, lx=64):
+        lib = load()
+        h = ctypes.c_void_p()
+        _check(lib.lf_batch_create(ctypes.byref(h)), 'lf_batch_create')
+        self.h, self._lib, self.engines = h, lib, []
+        self.set_temporal(depth, lx)
+        for e in engines:
+            self.add(e)
+
+    def add(self, eng):
+        _check(self._lib.lf_batch_add(self.h, eng.h), 'lf_batch_add')
+        self.engines.append(eng)
+
+    def set_temporal(self, depth, lx=64):
+        _check(self._lib.lf_batch_set_temporal(self.h, int(depth), int(lx)), 'lf_batch_set_temporal')
+
+    def set_timing(self, on=True):
+        _check(self._lib.lf_batch_set_timing(self.h, int(bool(on))), 'lf_batch_set_timing')
+
+    def run(self, tab, masks=None, nact=None):
+        """Advance every instance over the shared step table `tab`; masks: (ninst, nsteps)
+        monitor masks (default tab['mask']); nact: steps per instance (default all)."""
+        tab = np.ascontiguousarray(tab, dtype=STEP_DTYPE)
+        n = len(tab)
+        mp = ap = None
+        if masks is not None:
+            m = np.ascontiguousarray(masks, dtype=np.uint32)
+            if m.shape != (len(self.engines), n):
+                raise ValueError(f'masks must be ({len(self.engines)}, {n})')
+            mp = m.ctypes.data
+        if nact is not None:
+            a = np.ascontiguousarray(nact, dtype=np.int32)
+            if a.shape != (len(self.engines),):
+                raise ValueError('nact needs one entry per instance')
+            ap = a.ctypes.data
+        _check(self._lib.lf_batch_run(self.h, tab.ctypes.data, n, mp, ap), 'lf_batch_run')
+
+    def stats(self):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _check(self._lib.lf_batch_stats(self.h, ctypes.byref(ms), ctypes.byref(n)), 'lf_batch_stats')
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self._lib.lf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

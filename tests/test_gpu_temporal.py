@@ -63,6 +63,7 @@ def test_tm_blocked_source_and_drive_equal_depth1(mode, depth, lx):
     dims = (140, 150)
     a, b = H('tm2d', dims, mode), H('tm2d', dims, mode)
     a.set_temporal(depth, lx)
+    b.set_temporal(1, 64)
     ma = _setup(a, 'tm2d', dims, np.random.default_rng(8), source=True, drive=True, pec=False)
     _setup(b, 'tm2d', dims, np.random.default_rng(8), source=True, drive=True, pec=False)
     tab = _table(50, 0, 48)
@@ -94,3 +95,39 @@ def test_temporal_setter_rejects_bad_depth():
     with pytest.raises(RuntimeError):
         e.set_temporal(3, 8)
     e.close()
+
+
+@pytest.mark.parametrize('dims,depth,lx', [((37, 70), 2, 5), ((37, 70), 4, 9), ((37, 70), 8, 64),
+                                            ((170, 240), 4, 16), ((170, 240), 6, 32),
+                                            ((170, 240), 8, 24)])
+def test_te_blocked_bit_exact_vs_oracle(dims, depth, lx):
+    H = _hip()
+    ho, oo = H('te2d', dims, 'ref32'), OracleEngine('te2d', dims, 'ref32')
+    ho.set_temporal(depth, lx)
+    mh = _setup(ho, 'te2d', dims, np.random.default_rng(31))
+    _setup(oo, 'te2d', dims, np.random.default_rng(31))
+    tab = _table(37, 3, 33)
+    ho.set_timing(True)
+    ho.run(tab)
+    oo.run(tab)
+    ms, launches, d = ho.tb_stats()
+    assert d == depth and launches == 30 // depth
+    _compare('te2d', ho, oo, mh)
+    ho.close()
+
+
+@pytest.mark.parametrize('mode', ['ref32', 'fp64acc'])
+def test_te_blocked_source_and_drive_equal_depth1(mode):
+    H = _hip()
+    dims = (150, 140)
+    a, b = H('te2d', dims, mode), H('te2d', dims, mode)
+    a.set_temporal(6, 12)
+    b.set_temporal(1, 64)
+    ma = _setup(a, 'te2d', dims, np.random.default_rng(12), source=True, drive=True, pec=False)
+    _setup(b, 'te2d', dims, np.random.default_rng(12), source=True, drive=True, pec=False)
+    tab = _table(50, 0, 48)
+    a.run(tab)
+    b.run(tab)
+    _compare('te2d', a, b, ma)
+    a.close()
+    b.close()

@@ -172,6 +172,21 @@ def bench_sweep(args):
                 full_steps=[int(x) for x in [min(r['steps'] for r in rb), max(r['steps'] for r in rb)]])
 
 
+def bench_serve(args):
+    """Seeded Poisson stream (mean 20 req/s) of CW solve requests, 80 % 2D windows
+    (4096-16384 x 1024-2048), 20 % 3D windows (256-512 x 192-256 x 192-256), admitted
+    against a 220 GB HBM budget, resident until the one-period phasor change < 1e-4."""
+    from lens_sensor_amd import serving
+    n = args.requests
+    reqs = serving.request_stream(n, rate_hz=20.0, seed=args.seed)
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    t0 = time.time()
+    recs, summ = serving.Server(log=log).run(reqs, realtime=True)
+    wall = time.time() - t0
+    cells = sum(int(np.prod(r['dims'])) * r['steps'] for r in recs)
+    return recs, summ, wall, cells
+
+
 def cpu_baseline(workload, seconds=12.0):
     """NumPy oracle (one core) on a bounded sample: the same grid and coefficients, as many
     steps as fit in ~``seconds``; Mcell-updates/s."""
@@ -244,7 +259,9 @@ def main():
     ap.add_argument('--steps', type=int, default=None,
                     help='accepted for the driver contract: the workload fixes its steps')
     ap.add_argument('--warmup', type=int, default=0)
-    ap.add_argument('--workload', default='tm2d_stitch', choices=('tm2d_stitch', 'yee3d', 'sweep'))
+    ap.add_argument('--workload', default='tm2d_stitch',
+                    choices=('tm2d_stitch', 'yee3d', 'sweep', 'serve'))
+    ap.add_argument('--requests', type=int, default=24, help='serve: requests in the stream')
     ap.add_argument('--windows', type=int, default=None, help='tm2d_stitch: windows to run')
     ap.add_argument('--steps-override', type=int, default=None,
                     help='steps per window (reduced runs for profiling only)')
@@ -262,6 +279,21 @@ def main():
         sys.exit(2)
     from lens_sensor_amd.engine import hip
     dev = hip.device_name()
+    if args.workload == 'serve':
+        recs, summ, wall, cells = bench_serve(args)
+        out = dict(metric='solves/s', value=summ['solves_per_s'], unit='solves/s', n_gpus=1,
+                   steps=int(sum(r['steps'] for r in recs)), warmup=0,
+                   ms_per_step=None, higher_is_better=True, scaling='weak', vs_baseline=None,
+                   dtype='fp32', data='synthetic',
+                   config=dict(workload='serve_poisson20_2d80_3d20', requests=summ['requests'],
+                               hbm_budget_gb=220, tol=1e-4, fields='fp32', phasors='fp64'),
+                   device=dev, latency_p50_s=summ['p50_s'], latency_p99_s=summ['p99_s'],
+                   converged=summ['converged'], wall_s=wall, mcups=cells / wall / 1e6,
+                   per_request=[dict(id=r['id'], kind=r['kind'], dims=list(r['dims']), steps=r['steps'],
+                                     latency_s=round(r['latency'], 3), converged=bool(r['converged']))
+                                for r in recs])
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == 'sweep':
         r = bench_sweep(args)
         out = dict(metric='solves/s', value=r['solves'] / r['batched_s'], unit='solves/s',

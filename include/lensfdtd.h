@@ -128,12 +128,20 @@ typedef struct lf_batch lf_batch;
 int lf_batch_create(lf_batch **out);
 /* finalized 2D context, same precision mode for all instances */
 int lf_batch_add(lf_batch *b, lf_ctx *ctx);
+/* depth 1 (off), 2, 4 or 6 (default 6); lx = x planes per wavefront chunk */
 int lf_batch_set_temporal(lf_batch *b, int depth, int lx);
 int lf_batch_set_timing(lf_batch *b, int on);
 /* steps: shared step table; masks: ninst x nsteps monitor masks or NULL (steps[].mask);
  * nact: steps per instance in this call (<= nsteps) or NULL (all nsteps) */
 int lf_batch_run(lf_batch *b, const lf_step *steps, int nsteps, const unsigned int *masks,
                  const int *nact);
+/* Resident step table for lf_batch_run_at (uploaded once; instances that started at
+ * different times read it at their own offsets). */
+int lf_batch_set_table(lf_batch *b, const lf_step *table, int ntable);
+/* instance i takes table[offs[i] + s] for s < nact[i] <= nsteps (offs / nact may be NULL:
+ * 0 / nsteps); masks: ninst x nsteps indexed by s, or NULL (table masks) */
+int lf_batch_run_at(lf_batch *b, const int *offs, int nsteps, const unsigned int *masks,
+                    const int *nact);
 /* last lf_batch_run with timing on: HIP-event time (ms) and blocked passes */
 int lf_batch_stats(lf_batch *b, double *ms, int *passes);
 int lf_batch_destroy(lf_batch *b);

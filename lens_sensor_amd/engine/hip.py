@@ -30,7 +30,8 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_monitor_zero', 'lf_run', 'lf_sync', 'lf_set_timing', 'lf_kernel_ms',
            'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats',
            'lf_batch_create', 'lf_batch_add', 'lf_batch_set_temporal', 'lf_batch_set_timing',
-           'lf_batch_run', 'lf_batch_stats', 'lf_batch_destroy')
+           'lf_batch_run', 'lf_batch_stats', 'lf_batch_destroy', 'lf_batch_set_table',
+           'lf_batch_run_at')
 
 
 def library_path():
@@ -83,6 +84,8 @@ def load():
         'lf_batch_run': [P, P, I, P, P],
         'lf_batch_stats': [P, ctypes.POINTER(ctypes.c_double), ip],
         'lf_batch_destroy': [P],
+        'lf_batch_set_table': [P, P, I],
+        'lf_batch_run_at': [P, P, I, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -388,6 +391,31 @@ class HipBatch:
                 raise ValueError('nact needs one entry per instance')
             ap = a.ctypes.data
         _check(self._lib.lf_batch_run(self.h, tab.ctypes.data, n, mp, ap), 'lf_batch_run')
+
+    def set_table(self, tab):
+        """Upload the resident step table used by run_at."""
+        self._table = np.ascontiguousarray(tab, dtype=STEP_DTYPE)
+        _check(self._lib.lf_batch_set_table(self.h, self._table.ctypes.data, len(self._table)),
+               'lf_batch_set_table')
+
+    def run_at(self, offs, nsteps, masks=None, nact=None):
+        """Instance i takes table[offs[i] + s] for s < nact[i] <= nsteps."""
+        ni = len(self.engines)
+        o = np.ascontiguousarray(offs, dtype=np.int32)
+        if o.shape != (ni,):
+            raise ValueError('offs needs one entry per instance')
+        mp = ap = None
+        if masks is not None:
+            m = np.ascontiguousarray(masks, dtype=np.uint32)
+            if m.shape != (ni, nsteps):
+                raise ValueError(f'masks must be ({ni}, {nsteps})')
+            mp = m.ctypes.data
+        if nact is not None:
+            a = np.ascontiguousarray(nact, dtype=np.int32)
+            if a.shape != (ni,):
+                raise ValueError('nact needs one entry per instance')
+            ap = a.ctypes.data
+        _check(self._lib.lf_batch_run_at(self.h, o.ctypes.data, int(nsteps), mp, ap), 'lf_batch_run_at')
 
     def stats(self):
         ms, n = ctypes.c_double(), ctypes.c_int()

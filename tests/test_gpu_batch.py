@@ -28,7 +28,7 @@ def _build(H, mode, seed0):
     return engs, mons
 
 
-@pytest.mark.parametrize('mode,depth', [('ref32', 6), ('fp64acc', 4), ('ref32', 8)])
+@pytest.mark.parametrize('mode,depth', [('ref32', 6), ('fp64acc', 4), ('ref32', 2), ('fp64acc', 6)])
 def test_batch_equals_instances_alone(mode, depth):
     H, B = _hip()
     nsteps = 61
@@ -59,3 +59,13 @@ def test_batch_equals_instances_alone(mode, depth):
     b.close()
     for e in batch_eng + alone_eng:
         e.close()
+
+
+def test_batch_rejects_depth_8():
+    H, B = _hip()
+    e = H('te2d', (37, 70), 'ref32')
+    b = B([], depth=6)
+    with pytest.raises(RuntimeError):
+        b.set_temporal(8, 64)
+    b.close()
+    e.close()

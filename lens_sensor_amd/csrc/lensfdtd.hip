@@ -14,6 +14,7 @@
 // reference to the last bit except for the float32 sin of the source phase.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,6 +24,10 @@
 #include <vector>
 
 #include "../../include/lensfdtd.h"
+#include "lensfdtd_dev.h"
+#include "yee_tb3.h"
+
+using namespace lfd;
 
 namespace {
 
@@ -44,81 +49,6 @@ constexpr int kMaxMon = 32;
 constexpr int kBoxBlocks = 1024;   // per-block FP64 partials (deterministic order)
 
 // ----------------------------------------------------------------------------- device structs
-struct Strip {            // psi slab of one stretch axis: v < lo -> v, v >= hi -> v-hi+lo
-    float *psi;
-    int lo, hi, len;      // len = lo + (n - hi); psi == nullptr -> no slab
-};
-
-struct Prof {             // 1-D CPML profile on one node family of one axis
-    const float *b, *c, *k, *km1;
-};
-
-__device__ __forceinline__ int strip_pos(const Strip &s, int v) {
-    if (s.psi == nullptr) return -1;
-    if (v < s.lo) return v;
-    if (v >= s.hi) return v - s.hi + s.lo;
-    return -1;
-}
-
-struct Grid {
-    int nx, ny, nz, P;    // logical dims, padded fast axis
-    __device__ __forceinline__ long long at(int i, int j, int k) const {
-        return ((long long)i * ny + j) * P + k;
-    }
-};
-
-struct Excite {           // source pair and stitch drive, applied inside the E kernel
-    int src_comp, src_ix, src_n;          // plane index = (j*nk + k) or j
-    const float *amp, *ph1, *ph2;
-    const double *a1c, *a1s, *a2c, *a2s;  // amp*cos/sin(phase) (FP64 phase mode)
-    int drv_n, drv_math;                  // overlap cells, LF_DRIVE_*
-    const float *wtap;
-    const float2 *drv[3];                 // per E component (or nullptr)
-    int drv_stride[3];                    // plane size of each driven component
-};
-
-struct StepArgs {
-    float e32, wt32;
-    double ssin, scos, env;
-    float dr32, di32;
-    double dr, di;
-};
-
-// ----------------------------------------------------------------------------- excite helpers
-__device__ __forceinline__ float src_term_ref32(float e32, float wt32, float ph, float amp) {
-    return (e32 * sinf(wt32 + ph)) * amp;          // vcsel.py:155 (float32)
-}
-
-__device__ __forceinline__ float apply_source(float v, int comp, int i, int plane, const Excite &x,
-                                              const StepArgs &s, int fp64) {
-    if (x.amp == nullptr || comp != x.src_comp) return v;
-    if (i == x.src_ix) {
-        if (fp64) v = v + (float)(s.ssin * x.a1c[plane] + s.scos * x.a1s[plane]);
-        else v = v + src_term_ref32(s.e32, s.wt32, x.ph1[plane], x.amp[plane]);
-    } else if (i == x.src_ix + 1) {
-        if (fp64) v = v - (float)(s.ssin * x.a2c[plane] + s.scos * x.a2s[plane]);
-        else v = v - src_term_ref32(s.e32, s.wt32, x.ph2[plane], x.amp[plane]);
-    }
-    return v;
-}
-
-__device__ __forceinline__ float apply_drive(float v, int comp, int i, int plane, const Excite &x,
-                                             const StepArgs &s) {
-    if (i >= x.drv_n || x.drv[comp] == nullptr) return v;
-    const float2 D = x.drv[comp][(long long)i * x.drv_stride[comp] + plane];
-    const float w = x.wtap[i];
-    // Re(D * e^{iwt}): NumPy forms complex products as fma(ar, br, -(ai*bi)) on FMA hosts
-    // (its AVX2/AVX-512 loops); the same single-rounding form is used here explicitly.
-    if (x.drv_math == LF_DRIVE_C128) {       // fdtd2d_core.py:1426-1427 (float64 blend)
-        const double re = fma((double)D.x, s.dr, -((double)D.y * s.di));
-        const double din = s.env * re;
-        const float keep = (1.0f - w) * v;
-        return (float)((double)w * din + (double)keep);
-    }
-    const float re = fmaf(D.x, s.dr32, -(D.y * s.di32));   // fdtd3d_core.py:1627-1630 (float32)
-    return w * (s.e32 * re) + (1.0f - w) * v;
-}
-
 // ============================================================================ 2D TE kernels
 struct TEArgs {
     Grid g;
@@ -434,18 +364,6 @@ __global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, Step
 // all four psi slabs) are separate buffers.  Per cell and step the arithmetic is that of
 // tm_h_kernel + tm_e_kernel (boundaries.py:243-258, fdtd2d_core.py:796).
 constexpr int kMaxTB = 8;
-
-// Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1, one VALU op each, no
-// LDS round trip).  Lane 63 (shl) and lane 0 (shr) have no source lane and keep their
-// own value, as __shfl_down / __shfl_up return there.
-__device__ __forceinline__ float lane_next(float v) {          // value of lane + 1
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                      0x130, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float lane_prev(float v) {          // value of lane - 1
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                      0x138, 0xf, 0xf, false));
-}
 
 struct TBTMArgs {
     Grid g;
@@ -1144,25 +1062,6 @@ __device__ __forceinline__ float psi_term_pp(const Strip &s0, const Strip &s1, i
     return p + pr.km1[n] * d;
 }
 
-// psi slot of a slab compressed along `axis`, 32-bit element offset (grids < 2^31)
-__device__ __forceinline__ unsigned psi_off32(const Strip &s, int axis, int pos, int i, int j, int k,
-                                              const Grid &g) {
-    if (axis == 0) return ((unsigned)pos * g.ny + (unsigned)j) * g.P + (unsigned)k;
-    if (axis == 1) return ((unsigned)i * s.len + (unsigned)pos) * g.P + (unsigned)k;
-    return ((unsigned)i * g.ny + (unsigned)j) * s.len + (unsigned)pos;
-}
-
-// profile triple of one node of one axis
-struct PV { float b, c, m; };
-__device__ __forceinline__ PV prof_at(const Prof &p, int n) { return PV{p.b[n], p.c[n], p.km1[n]}; }
-// psi <- b psi + c d ; returns psi + (1/kappa - 1) d   (boundaries3d.py:240-251 order)
-__device__ __forceinline__ float psi_step(float &psi, const PV &q, float d) {
-    float p = psi * q.b;
-    p = p + q.c * d;
-    psi = p;
-    return p + q.m * d;
-}
-
 // one x plane of a single-pass wavefront, as loaded from the step-n buffers: Ex and the
 // old H of plane i, the lane-63 k+1 values, coefficients, psi of every slab term, and
 // Ey / Ez of plane i+1 (the x derivatives of H(i))
@@ -1748,6 +1647,16 @@ struct lf_ctx {
     int lx = 32;                         // x rows per workgroup in the single-pass kernels
     float *psi_alt[4] = {};              // ping-pong psi of the H slabs (2D fused)
     float *psi_alt3[6] = {};             // ping-pong psi of the H slabs (3D fused)
+    float *psi_alt3e[6] = {};            // ping-pong psi of the E slabs (3D blocked)
+    // 3D blocked pass: tile plan (boxes of one wavefront class each) for (depth, lx)
+    std::vector<TB3Box> tb3_boxes;
+    TB3Box *tb3_dbox = nullptr;
+    int tb3_T = 0, tb3_lx = 0, tb3_ntile = 0;
+    int tb3_nwg = 0;                     // persistent workgroups of a pass (one per CU)
+    unsigned *tb3_ticket = nullptr;      // tile ticket counter (device) and the next pass's base
+    unsigned tb3_base = 0;
+    int tb3_key[3] = {0, 0, 0};          // source plane, drive overlap, rows of the plan
+    int tb3_rows0 = 2;                   // rows per wavefront of the slab-free class
     int ty = 4;                          // owned y rows per wavefront of the 3D single pass
     int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
     int tb_lx = 64;                      // x planes per wavefront chunk of the blocked pass
@@ -1979,6 +1888,7 @@ int lf_destroy(lf_ctx *c) {
         std::set<float *> psi(c->psi_bufs.begin(), c->psi_bufs.end());
         for (auto p : c->psi_alt) if (p) psi.insert(p);
         for (auto p : c->psi_alt3) if (p) psi.insert(p);
+        for (auto p : c->psi_alt3e) if (p) psi.insert(p);
         for (auto &s : c->s2) if (s.psi) psi.insert(s.psi);
         for (auto &s : c->sH) if (s.psi) psi.insert(s.psi);
         for (auto &s : c->sE) if (s.psi) psi.insert(s.psi);
@@ -1991,6 +1901,8 @@ int lf_destroy(lf_ctx *c) {
             if (c->prof_h[a][q]) (void)hipFree(c->prof_h[a][q]);
         }
     for (auto p : c->ex_bufs) (void)hipFree(p);
+    if (c->tb3_dbox) (void)hipFree(c->tb3_dbox);
+    if (c->tb3_ticket) (void)hipFree(c->tb3_ticket);
     for (int m = 0; m < c->nmon; ++m) {
         if (c->mon[m].buf) (void)hipFree(c->mon[m].buf);
         if (c->mon[m].type == 3)
@@ -2489,8 +2401,11 @@ int tb_chunk_2d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
 
 extern "C" int lf_set_temporal(lf_ctx *c, int depth, int lx) {
     if (int e = check_ctx(c)) return e;
-    if (depth != 1 && depth != 2 && depth != 4 && depth != 6 && depth != 8)
+    if (c->kind == LF_YEE3D) {
+        if (depth < 1 || depth > 3) return fail(-1, "lf_set_temporal: 3D depth must be 1 (off), 2 or 3");
+    } else if (depth != 1 && depth != 2 && depth != 4 && depth != 6 && depth != 8) {
         return fail(-1, "lf_set_temporal: depth must be 1 (off), 2, 4, 6 or 8");
+    }
     if (lx < 1 || lx > 1 << 20) return fail(-1, "lf_set_temporal: lx out of range");
     c->tb = depth;
     c->tb_lx = lx;
@@ -2644,6 +2559,212 @@ int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
 }
 
 }  // namespace
+
+namespace {
+
+// Tile plan of the 3D blocked pass for depth T and x chunk lx.  A tile owns 64 - 2T z
+// lanes, kTB3W * rows - 2T y rows and one x chunk, and loads a T-cell halo around them.
+// Per axis the owned positions split into at most three intervals: the middle one, whose
+// tiles load only positions in 1..N-2 outside every psi slab of that axis, and the
+// intervals below and above it (z: whole tiles of a uniform tiling, so that the tiles
+// meeting a z slab still carry interior lanes; y and x: position intervals, the x slab
+// intervals being one short chunk each).  The middle x interval further splits into runs
+// of chunks whose march meets a source or drive plane and runs that do not.  Every
+// combination is one box; its class bits say along which axes its tiles may meet a slab
+// or an edge.  Slab-free boxes without excitation run c->tb3_rows0 rows per wavefront,
+// the others one.
+int tb3_plan(lf_ctx *c, int T, int lx) {
+    // planes where the E update adds the source pair or the stitch drive
+    std::vector<std::pair<int, int>> exc;                 // [a, b) plane ranges
+    if (c->ex.amp && (c->ex.src_comp == LF_EY || c->ex.src_comp == LF_EZ))
+        exc.push_back({c->ex.src_ix, c->ex.src_ix + 2});
+    if (c->ex.wtap && (c->ex.drv[0] || c->ex.drv[1] || c->ex.drv[2])) exc.push_back({0, c->ex.drv_n});
+    const int key_src = c->ex.amp ? c->ex.src_ix : -1, key_drv = c->ex.wtap ? c->ex.drv_n : -1;
+    if (c->tb3_dbox && c->tb3_T == T && c->tb3_lx == lx && c->tb3_key[0] == key_src &&
+        c->tb3_key[1] == key_drv && c->tb3_key[2] == c->tb3_rows0)
+        return 0;
+    auto meets = [&](int x0, int x1) {                    // march of chunk [x0, x1)
+        for (auto &e : exc)
+            if (x0 - T < e.second && x1 + T > e.first) return true;
+        return false;
+    };
+    const int N3[3] = {c->g.nz, c->g.ny, c->g.nx};                   // z, y, x
+    const int axs[3][4] = {{0, 3, 1, 2}, {1, 4, 0, 5}, {2, 5, 3, 4}};   // sH, sH, sE, sE
+    struct Iv { int a, b; bool edge, exc; };
+    std::vector<Iv> iv[3];
+    const int WZ = 64 - 2 * T;
+    for (int d = 0; d < 3; ++d) {
+        const int N = N3[d];
+        int lo = 1, hi = N - 2;
+        for (int u = 0; u < 4; ++u) {
+            const Strip &sp = u < 2 ? c->sH[axs[d][u]] : c->sE[axs[d][u]];
+            if (!sp.psi) continue;
+            lo = std::max(lo, sp.lo);
+            hi = std::min(hi, sp.hi - 1);
+        }
+        int A, B;                                         // middle interval [A, B)
+        if (d == 0) {
+            const int nt = (N + WZ - 1) / WZ;
+            int ta = nt, tb = -1;
+            for (int t = 0; t < nt; ++t)
+                if (t * WZ - T >= lo && std::min((t + 1) * WZ, N) + T - 1 <= hi) {
+                    ta = std::min(ta, t);
+                    tb = std::max(tb, t);
+                }
+            A = ta * WZ;
+            B = tb >= ta ? std::min((tb + 1) * WZ, N) : A;
+        } else {
+            A = lo + T;
+            B = hi - T + 1;
+        }
+        if (B <= A) {
+            iv[d].push_back({0, N, true, false});
+            continue;
+        }
+        if (A > 0) iv[d].push_back({0, A, true, false});
+        if (d < 2) {
+            iv[d].push_back({A, B, false, false});
+        } else {                                          // x: runs of chunks by excitation
+            for (int x0 = A; x0 < B; x0 += lx) {
+                const bool e = meets(x0, std::min(x0 + lx, B));
+                if (!iv[d].empty() && !iv[d].back().edge && iv[d].back().exc == e) iv[d].back().b = std::min(x0 + lx, B);
+                else iv[d].push_back({x0, std::min(x0 + lx, B), false, e});
+            }
+        }
+        if (B < N) iv[d].push_back({B, N, true, false});
+    }
+    for (Iv &v : iv[2])                                  // x slab intervals: one chunk each
+        if (v.edge) v.exc = meets(v.a, v.b);
+    std::vector<TB3Box> boxes;
+    for (const Iv &vx : iv[2])
+        for (const Iv &vy : iv[1])
+            for (const Iv &vz : iv[0]) {
+                TB3Box b{};
+                b.mode = (vz.edge ? 1 : 0) | (vy.edge ? 2 : 0) | (vx.edge ? 4 : 0);
+                b.exc = vx.exc ? 1 : 0;
+                b.rows = (b.mode == 0 && !b.exc) ? c->tb3_rows0 : 1;
+                b.x0 = vx.a; b.x1 = vx.b;
+                const int lenx = vx.b - vx.a;
+                b.lx = (vx.edge && lenx + 2 * T <= 64) ? lenx : std::min(lx, lenx);
+                b.y0 = vy.a; b.y1 = vy.b; b.z0 = vz.a; b.z1 = vz.b;
+                const int WY = kTB3W * b.rows - 2 * T;
+                b.nty = (vy.b - vy.a + WY - 1) / WY;
+                b.ntz = (vz.b - vz.a + WZ - 1) / WZ;
+                boxes.push_back(b);
+            }
+    // boxes by decreasing tile cost, so that the persistent workgroups end on short
+    // tiles: iterations x rows, with every slab axis adding psi work and register spills
+    auto cost = [&](const TB3Box &b) {
+        const int it = std::min(b.lx, b.x1 - b.x0) + 2 * T;
+        int ax = 0;
+        for (int m = b.mode; m; m >>= 1) ax += m & 1;
+        return (double)it * (b.rows == 2 ? 1.45 : 1.0) * (1.0 + 0.45 * ax * ax) * (b.exc ? 1.3 : 1.0);
+    };
+    std::stable_sort(boxes.begin(), boxes.end(),
+                     [&](const TB3Box &p, const TB3Box &q) { return cost(p) > cost(q); });
+    int first = 0;
+    for (TB3Box &b : boxes) {
+        b.first = first;
+        const int ntx = (b.x1 - b.x0 + b.lx - 1) / b.lx;
+        first += ntx * b.nty * b.ntz;
+    }
+    if ((int)boxes.size() > kTB3MaxBox) return fail(-1, "3D blocked pass: too many boxes");
+    if (!c->tb3_dbox) LF_HIP(hipMalloc(&c->tb3_dbox, kTB3MaxBox * sizeof(TB3Box)));
+    LF_HIP(hipMemcpyAsync(c->tb3_dbox, boxes.data(), boxes.size() * sizeof(TB3Box),
+                          hipMemcpyHostToDevice, c->stream));
+    LF_HIP(hipStreamSynchronize(c->stream));
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount >= 1)
+            ncu = pr.multiProcessorCount;
+    }
+    c->tb3_nwg = std::min(ncu, first);
+    if (!c->tb3_ticket) {
+        LF_HIP(hipMalloc(&c->tb3_ticket, sizeof(unsigned)));
+        LF_HIP(hipMemsetAsync(c->tb3_ticket, 0, sizeof(unsigned), c->stream));
+        c->tb3_base = 0;
+    }
+    c->tb3_boxes = boxes;
+    c->tb3_ntile = first;
+    c->tb3_T = T;
+    c->tb3_lx = lx;
+    c->tb3_key[0] = key_src; c->tb3_key[1] = key_drv; c->tb3_key[2] = c->tb3_rows0;
+    return 0;
+}
+
+int ensure_tb3_buffers(lf_ctx *c) {
+    const int ids[6] = {LF_EX, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ};
+    for (int q = 0; q < 6; ++q)
+        if (int r = ensure_alt(c, ids[q])) return r;
+    const int hax[6] = {2, 1, 0, 2, 1, 0};            // Hx_z Hx_y Hy_x Hy_z Hz_y Hz_x
+    const int eax[6] = {1, 2, 2, 0, 0, 1};            // Ex_y Ex_z Ey_z Ey_x Ez_x Ez_y
+    for (int q = 0; q < 6; ++q) {
+        if (c->sH[q].psi && !c->psi_alt3[q]) {
+            const size_t n = strip_elems_3d(c, c->sH[q], hax[q]);
+            LF_HIP(hipMalloc(&c->psi_alt3[q], n * sizeof(float)));
+            LF_HIP(hipMemsetAsync(c->psi_alt3[q], 0, n * sizeof(float), c->stream));
+        }
+        if (c->sE[q].psi && !c->psi_alt3e[q]) {
+            const size_t n = strip_elems_3d(c, c->sE[q], eax[q]);
+            LF_HIP(hipMalloc(&c->psi_alt3e[q], n * sizeof(float)));
+            LF_HIP(hipMemsetAsync(c->psi_alt3e[q], 0, n * sizeof(float), c->stream));
+        }
+    }
+    return 0;
+}
+
+// one temporally blocked 3D pass over T steps: reads arr[] and the psi slabs, writes the
+// other buffers, then swaps
+int tb_chunk_3d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
+    if (T < 2 || T > 3) return fail(-1, "3D temporal block depth must be 2 or 3");
+    if ((long long)c->g.nx * c->g.ny * c->g.P >= (1LL << 31))
+        return fail(-1, "3D blocked pass: grid exceeds 2^31 elements");
+    if (int r = ensure_tb3_buffers(c)) return r;
+    if (int r = tb3_plan(c, T, c->tb_lx)) return r;
+    TB3Args f{};
+    f.g = c->g;
+    f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Ez0 = c->arr[LF_EZ];
+    f.Hx0 = c->arr[LF_HX]; f.Hy0 = c->arr[LF_HY]; f.Hz0 = c->arr[LF_HZ];
+    f.Ex1 = c->alt[LF_EX]; f.Ey1 = c->alt[LF_EY]; f.Ez1 = c->alt[LF_EZ];
+    f.Hx1 = c->alt[LF_HX]; f.Hy1 = c->alt[LF_HY]; f.Hz1 = c->alt[LF_HZ];
+    f.cex = c->arr[LF_CEX]; f.cey = c->arr[LF_CEY]; f.cez = c->arr[LF_CEZ];
+    f.dax = c->arr[LF_DAX]; f.day = c->arr[LF_DAY]; f.daz = c->arr[LF_DAZ];
+    f.ch = c->ch;
+    for (int ax = 0; ax < 3; ++ax) { f.pe[ax] = prof_of(c, ax, false); f.ph[ax] = prof_of(c, ax, true); }
+    for (int q = 0; q < 6; ++q) {
+        f.s0[q] = c->sH[q]; f.s1[q] = c->sH[q]; f.s1[q].psi = c->psi_alt3[q];
+        f.s0[6 + q] = c->sE[q]; f.s1[6 + q] = c->sE[q]; f.s1[6 + q].psi = c->psi_alt3e[q];
+    }
+    for (int ax = 0; ax < 3; ++ax) { f.pec[2 * ax] = c->pec[ax][0]; f.pec[2 * ax + 1] = c->pec[ax][1]; }
+    f.box = c->tb3_dbox;
+    f.nbox = (int)c->tb3_boxes.size();
+    f.ntile = c->tb3_ntile;
+    f.ticket = c->tb3_ticket;
+    f.base = c->tb3_base;
+    for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
+    const bool loss = f.dax || f.day || f.daz;
+    launch_yee_tb3(T, loss, f, c->ex, fp64, (unsigned)c->tb3_nwg, c->stream);
+    // every workgroup draws its tiles and one ticket past the end
+    c->tb3_base += (unsigned)(c->tb3_ntile + c->tb3_nwg);
+    const int ids[6] = {LF_EX, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ};
+    for (int q = 0; q < 6; ++q) std::swap(c->arr[ids[q]], c->alt[ids[q]]);
+    for (int q = 0; q < 6; ++q) {
+        if (c->sH[q].psi) std::swap(c->sH[q].psi, c->psi_alt3[q]);
+        if (c->sE[q].psi) std::swap(c->sE[q].psi, c->psi_alt3e[q]);
+    }
+    return 0;
+}
+
+}  // namespace
+
+namespace lfd {
+void launch_yee_tb3(int T, bool loss, const TB3Args &f, const Excite &x, int fp64, unsigned nb,
+                    hipStream_t st) {
+    if (T == 2) (loss ? launch_yee_tb3_2_1 : launch_yee_tb3_2_0)(f, x, fp64, nb, st);
+    else (loss ? launch_yee_tb3_3_1 : launch_yee_tb3_3_0)(f, x, fp64, nb, st);
+}
+}  // namespace lfd
 
 extern "C" int lf_set_tile(lf_ctx *c, int ty) {
     if (int e = check_ctx(c)) return e;
@@ -2922,17 +3043,22 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         const lf_step &t = steps[s];
         StepArgs st = step_args(t);
         const int T = allow_tb ? c->tb : 1;
-        if (T > 1 && c->kind != LF_YEE3D && s + T <= nsteps) {
+        if (T > 1 && s + T <= nsteps) {
             bool quiet = true;                       // no monitor in the block's steps
             for (int u = 0; u < T; ++u) quiet = quiet && steps[s + u].mask == 0;
             if (quiet) {
                 if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
-                if (int e = tb_chunk_2d(c, steps + s, T, fp64)) return e;
+                if (c->kind == LF_YEE3D) {
+                    if (int e = tb_chunk_3d(c, steps + s, T, fp64)) return e;
+                } else {
+                    if (int e = tb_chunk_2d(c, steps + s, T, fp64)) return e;
+                }
                 if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
                 tb_groups.push_back(ngroup);
                 ++ngroup;
                 if (c->kind == LF_TM2D) tm = tm_args(c);
-                else te = te_args(c);
+                else if (c->kind == LF_TE2D) te = te_args(c);
+                else yee = yee_args(c);
                 ma = mon_args(c);
                 s += T - 1;
                 continue;

@@ -1,0 +1,116 @@
+// lensfdtd_dev.h - device-side types and helpers shared by the translation units of
+// liblensfdtd.so (the step kernels in lensfdtd.hip and the 3D blocked pass in yee_tb3.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/lensfdtd.h"
+
+namespace lfd {
+
+struct Strip {            // psi slab of one stretch axis: v < lo -> v, v >= hi -> v-hi+lo
+    float *psi;
+    int lo, hi, len;      // len = lo + (n - hi); psi == nullptr -> no slab
+};
+
+struct Prof {             // 1-D CPML profile on one node family of one axis
+    const float *b, *c, *k, *km1;
+};
+
+__device__ __forceinline__ int strip_pos(const Strip &s, int v) {
+    if (s.psi == nullptr) return -1;
+    if (v < s.lo) return v;
+    if (v >= s.hi) return v - s.hi + s.lo;
+    return -1;
+}
+
+struct Grid {
+    int nx, ny, nz, P;    // logical dims, padded fast axis
+    __device__ __forceinline__ long long at(int i, int j, int k) const {
+        return ((long long)i * ny + j) * P + k;
+    }
+};
+
+struct Excite {           // source pair and stitch drive, applied inside the E kernel
+    int src_comp, src_ix, src_n;          // plane index = (j*nk + k) or j
+    const float *amp, *ph1, *ph2;
+    const double *a1c, *a1s, *a2c, *a2s;  // amp*cos/sin(phase) (FP64 phase mode)
+    int drv_n, drv_math;                  // overlap cells, LF_DRIVE_*
+    const float *wtap;
+    const float2 *drv[3];                 // per E component (or nullptr)
+    int drv_stride[3];                    // plane size of each driven component
+};
+
+struct StepArgs {
+    float e32, wt32;
+    double ssin, scos, env;
+    float dr32, di32;
+    double dr, di;
+};
+
+// ----------------------------------------------------------------------------- excite helpers
+__device__ __forceinline__ float src_term_ref32(float e32, float wt32, float ph, float amp) {
+    return (e32 * sinf(wt32 + ph)) * amp;          // vcsel.py:155 (float32)
+}
+
+__device__ __forceinline__ float apply_source(float v, int comp, int i, int plane, const Excite &x,
+                                              const StepArgs &s, int fp64) {
+    if (x.amp == nullptr || comp != x.src_comp) return v;
+    if (i == x.src_ix) {
+        if (fp64) v = v + (float)(s.ssin * x.a1c[plane] + s.scos * x.a1s[plane]);
+        else v = v + src_term_ref32(s.e32, s.wt32, x.ph1[plane], x.amp[plane]);
+    } else if (i == x.src_ix + 1) {
+        if (fp64) v = v - (float)(s.ssin * x.a2c[plane] + s.scos * x.a2s[plane]);
+        else v = v - src_term_ref32(s.e32, s.wt32, x.ph2[plane], x.amp[plane]);
+    }
+    return v;
+}
+
+__device__ __forceinline__ float apply_drive(float v, int comp, int i, int plane, const Excite &x,
+                                             const StepArgs &s) {
+    if (i >= x.drv_n || x.drv[comp] == nullptr) return v;
+    const float2 D = x.drv[comp][(long long)i * x.drv_stride[comp] + plane];
+    const float w = x.wtap[i];
+    // Re(D * e^{iwt}): NumPy forms complex products as fma(ar, br, -(ai*bi)) on FMA hosts
+    // (its AVX2/AVX-512 loops); the same single-rounding form is used here explicitly.
+    if (x.drv_math == LF_DRIVE_C128) {       // fdtd2d_core.py:1426-1427 (float64 blend)
+        const double re = fma((double)D.x, s.dr, -((double)D.y * s.di));
+        const double din = s.env * re;
+        const float keep = (1.0f - w) * v;
+        return (float)((double)w * din + (double)keep);
+    }
+    const float re = fmaf(D.x, s.dr32, -(D.y * s.di32));   // fdtd3d_core.py:1627-1630 (float32)
+    return w * (s.e32 * re) + (1.0f - w) * v;
+}
+
+// Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1, one VALU op each, no
+// LDS round trip).  Lane 63 (shl) and lane 0 (shr) have no source lane and keep their
+// own value, as __shfl_down / __shfl_up return there.
+__device__ __forceinline__ float lane_next(float v) {          // value of lane + 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {          // value of lane - 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x138, 0xf, 0xf, false));
+}
+
+// psi slot of a slab compressed along `axis`, 32-bit element offset (grids < 2^31)
+__device__ __forceinline__ unsigned psi_off32(const Strip &s, int axis, int pos, int i, int j, int k,
+                                              const Grid &g) {
+    if (axis == 0) return ((unsigned)pos * g.ny + (unsigned)j) * g.P + (unsigned)k;
+    if (axis == 1) return ((unsigned)i * s.len + (unsigned)pos) * g.P + (unsigned)k;
+    return ((unsigned)i * g.ny + (unsigned)j) * s.len + (unsigned)pos;
+}
+
+// profile triple of one node of one axis
+struct PV { float b, c, m; };
+__device__ __forceinline__ PV prof_at(const Prof &p, int n) { return PV{p.b[n], p.c[n], p.km1[n]}; }
+// psi <- b psi + c d ; returns psi + (1/kappa - 1) d   (boundaries3d.py:240-251 order)
+__device__ __forceinline__ float psi_step(float &psi, const PV &q, float d) {
+    float p = psi * q.b;
+    p = p + q.c * d;
+    psi = p;
+    return p + q.m * d;
+}
+
+}  // namespace lfd

@@ -1,5 +1,9 @@
 """Builds liblensfdtd.so (HIP, gfx950) in-tree with hipcc.
 
+Translation units: csrc/lensfdtd.hip (step kernels, monitors, batch engine, C ABI) and
+csrc/yee_tb3.hip (the temporally blocked 3D pass), the latter once per depth / loss
+variant; they compile in parallel into lens_sensor_amd/_build/ and link into one library.
+
 Flags: -ffp-contract=off keeps the reference's float32 operation order (no fused
 multiply-add), -Xclang -target-feature -Xclang -packed-fp32-ops keeps the compiler from
 emitting packed FP32 instructions.
@@ -10,19 +14,47 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-SRC = os.path.join(PKG, 'csrc', 'lensfdtd.hip')
+CSRC = os.path.join(PKG, 'csrc')
+SRC = os.path.join(CSRC, 'lensfdtd.hip')
+SRC_TB3 = os.path.join(CSRC, 'yee_tb3.hip')
 LIB = os.path.join(PKG, 'liblensfdtd.so')
+BUILD_DIR = os.path.join(PKG, '_build')
 ARCH = os.environ.get('LENSFDTD_ARCH', 'gfx950')
+FLAGS = [f'--offload-arch={ARCH}', '-O3', '-std=c++17', '-fPIC', '-ffp-contract=off',
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+TB3_VARIANTS = [(2, 0), (2, 1), (3, 0), (3, 1)]       # (depth, loss factors present)
+
+
+def _jobs():
+    jobs = [(['hipcc'] + FLAGS + ['-c', SRC, '-o', os.path.join(BUILD_DIR, 'lensfdtd.o')],
+             os.path.join(BUILD_DIR, 'lensfdtd.o'))]
+    for T, L in TB3_VARIANTS:
+        obj = os.path.join(BUILD_DIR, f'yee_tb3_{T}_{L}.o')
+        jobs.append((['hipcc'] + FLAGS + [f'-DLF_TB3_T={T}', f'-DLF_TB3_LOSS={L}', '-c', SRC_TB3,
+                                          '-o', obj], obj))
+    return jobs
 
 
 def build(verbose=False):
-    cmd = ['hipcc', f'--offload-arch={ARCH}', '-O3', '-std=c++17', '-fPIC', '-shared',
-           '-ffp-contract=off', '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-           '-o', LIB + '.tmp', SRC]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    jobs = _jobs()
+    procs = [(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True),
+              cmd) for cmd, _ in jobs]
+    failed = []
+    for p, cmd in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append((cmd, out))
+    if failed:
+        for cmd, out in failed:
+            sys.stderr.write(' '.join(cmd) + '\n' + out)
+        raise RuntimeError(f'hipcc failed on {len(failed)} translation unit(s)')
+    objs = [obj for _, obj in jobs]
+    link = ['hipcc', f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', LIB + '.tmp'] + objs
+    r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f'hipcc failed ({r.returncode})')
+        raise RuntimeError(f'hipcc link failed ({r.returncode})')
     os.replace(LIB + '.tmp', LIB)
     if verbose:
         print('built', LIB)

@@ -1655,7 +1655,7 @@ struct lf_ctx {
     int tb3_nwg = 0;                     // persistent workgroups of a pass (one per CU)
     unsigned *tb3_ticket = nullptr;      // tile ticket counter (device) and the next pass's base
     unsigned tb3_base = 0;
-    int tb3_key[3] = {0, 0, 0};          // source plane, drive overlap, rows of the plan
+    int tb3_key[4] = {0, 0, 0, 0};       // source plane, drive overlap, rows, monitors of the plan
     int tb3_rows0 = 2;                   // rows per wavefront of the slab-free class
     int ty = 4;                          // owned y rows per wavefront of the 3D single pass
     int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
@@ -1861,6 +1861,7 @@ int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out) {
     lf_ctx *c = new lf_ctx();
     c->kind = kind; c->mode = mode;
     if (kind == LF_TM2D) c->tb = 6;              // blocked passes over monitor-free steps
+    if (kind == LF_YEE3D) c->tb = 3;
     c->nx = nx; c->ny = ny; c->nz = (kind == LF_YEE3D) ? nz : 1;
     if (kind == LF_YEE3D) c->g = Grid{nx, ny, nz, round_up(nz, 64)};
     else c->g = Grid{nx, 1, ny, round_up(ny, 64)};
@@ -2562,6 +2563,16 @@ int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
 
 namespace {
 
+// Detector boxes the 3D blocked pass sums itself: the first two box monitors, when the
+// tile plan fits their per-tile slots.  Returns their monitor bits.
+unsigned tb3_fused_boxes(const lf_ctx *c, int *ids) {
+    unsigned bits = 0;
+    int n = 0;
+    for (int id = 0; id < c->nmon && n < 2; ++id)
+        if (c->mon[id].type == 0) { ids[n++] = id; bits |= 1u << id; }
+    return bits;
+}
+
 // Tile plan of the 3D blocked pass for depth T and x chunk lx.  A tile owns 64 - 2T z
 // lanes, kTB3W * rows - 2T y rows and one x chunk, and loads a T-cell halo around them.
 // Per axis the owned positions split into at most three intervals: the middle one, whose
@@ -2579,18 +2590,21 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
     if (c->ex.amp && (c->ex.src_comp == LF_EY || c->ex.src_comp == LF_EZ))
         exc.push_back({c->ex.src_ix, c->ex.src_ix + 2});
     if (c->ex.wtap && (c->ex.drv[0] || c->ex.drv[1] || c->ex.drv[2])) exc.push_back({0, c->ex.drv_n});
+    // x ranges of the detector boxes the pass sums (their owned planes need the box code)
+    std::vector<std::pair<int, int>> boxx;
+    int fid[2];
+    const unsigned fbits = tb3_fused_boxes(c, fid);
+    for (int m = 0; m < __builtin_popcount(fbits); ++m) {
+        const BoxArgs &b = c->mon[fid[m]].box;
+        for (int q = 0; q < 3; ++q) boxx.push_back({b.lo[q][0], b.hi[q][0]});
+    }
     const int key_src = c->ex.amp ? c->ex.src_ix : -1, key_drv = c->ex.wtap ? c->ex.drv_n : -1;
     if (c->tb3_dbox && c->tb3_T == T && c->tb3_lx == lx && c->tb3_key[0] == key_src &&
-        c->tb3_key[1] == key_drv && c->tb3_key[2] == c->tb3_rows0)
+        c->tb3_key[1] == key_drv && c->tb3_key[2] == c->tb3_rows0 && c->tb3_key[3] == c->nmon)
         return 0;
-    auto meets = [&](int x0, int x1) {                    // march of chunk [x0, x1)
-        for (auto &e : exc)
-            if (x0 - T < e.second && x1 + T > e.first) return true;
-        return false;
-    };
     const int N3[3] = {c->g.nz, c->g.ny, c->g.nx};                   // z, y, x
     const int axs[3][4] = {{0, 3, 1, 2}, {1, 4, 0, 5}, {2, 5, 3, 4}};   // sH, sH, sE, sE
-    struct Iv { int a, b; bool edge, exc; };
+    struct Iv { int a, b; bool edge, exc, box; };
     std::vector<Iv> iv[3];
     const int WZ = 64 - 2 * T;
     for (int d = 0; d < 3; ++d) {
@@ -2618,23 +2632,59 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
             B = hi - T + 1;
         }
         if (B <= A) {
-            iv[d].push_back({0, N, true, false});
+            iv[d].push_back({0, N, true, false, false});
             continue;
         }
-        if (A > 0) iv[d].push_back({0, A, true, false});
+        if (A > 0) iv[d].push_back({0, A, true, false, false});
         if (d < 2) {
-            iv[d].push_back({A, B, false, false});
-        } else {                                          // x: runs of chunks by excitation
-            for (int x0 = A; x0 < B; x0 += lx) {
-                const bool e = meets(x0, std::min(x0 + lx, B));
-                if (!iv[d].empty() && !iv[d].back().edge && iv[d].back().exc == e) iv[d].back().b = std::min(x0 + lx, B);
-                else iv[d].push_back({x0, std::min(x0 + lx, B), false, e});
+            iv[d].push_back({A, B, false, false, false});
+        } else {
+            // x: the planes of [A, B) by the code their tiles need: 2 excitation (planes
+            // whose chunk march can meet a source / drive plane: within T of it), 1 box
+            // sums (owned planes of a fused box), 0 neither; runs shorter than 16 planes
+            // join the heavier neighbour (a heavier body is a superset), then each run is
+            // one interval
+            std::vector<int> cls(B - A, 0);
+            for (auto &e : exc)
+                for (int q = std::max(e.first - T, A); q < std::min(e.second + T, B); ++q) cls[q - A] = 2;
+            for (auto &e : boxx)
+                for (int q = std::max(e.first, A); q < std::min(e.second, B); ++q)
+                    cls[q - A] = std::max(cls[q - A], 1);
+            for (bool changed = true; changed;) {
+                changed = false;
+                int r0 = 0;
+                while (r0 < B - A) {
+                    int r1 = r0;
+                    while (r1 < B - A && cls[r1] == cls[r0]) ++r1;
+                    if (r1 - r0 < 16 && (r0 > 0 || r1 < B - A)) {
+                        const int left = r0 > 0 ? cls[r0 - 1] : -1, right = r1 < B - A ? cls[r1] : -1;
+                        const int to = std::max(left, right);
+                        if (to > cls[r0]) {
+                            for (int q = r0; q < r1; ++q) cls[q] = to;
+                            changed = true;
+                        }
+                    }
+                    r0 = r1;
+                }
+            }
+            int r0 = 0;
+            while (r0 < B - A) {
+                int r1 = r0;
+                while (r1 < B - A && cls[r1] == cls[r0]) ++r1;
+                iv[d].push_back({A + r0, A + r1, false, cls[r0] == 2, cls[r0] >= 1});
+                r0 = r1;
             }
         }
-        if (B < N) iv[d].push_back({B, N, true, false});
+        if (B < N) iv[d].push_back({B, N, true, false, false});
     }
     for (Iv &v : iv[2])                                  // x slab intervals: one chunk each
-        if (v.edge) v.exc = meets(v.a, v.b);
+        if (v.edge) {
+            bool e = false, bx = false;
+            for (auto &q : exc) e = e || (v.a - T < q.second && v.b + T > q.first);
+            for (auto &q : boxx) bx = bx || (v.a < q.second && v.b > q.first);
+            v.exc = e;
+            v.box = bx || e;
+        }
     std::vector<TB3Box> boxes;
     for (const Iv &vx : iv[2])
         for (const Iv &vy : iv[1])
@@ -2642,6 +2692,7 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
                 TB3Box b{};
                 b.mode = (vz.edge ? 1 : 0) | (vy.edge ? 2 : 0) | (vx.edge ? 4 : 0);
                 b.exc = vx.exc ? 1 : 0;
+                b.box = vx.box ? 1 : 0;
                 b.rows = (b.mode == 0 && !b.exc) ? c->tb3_rows0 : 1;
                 b.x0 = vx.a; b.x1 = vx.b;
                 const int lenx = vx.b - vx.a;
@@ -2658,7 +2709,8 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
         const int it = std::min(b.lx, b.x1 - b.x0) + 2 * T;
         int ax = 0;
         for (int m = b.mode; m; m >>= 1) ax += m & 1;
-        return (double)it * (b.rows == 2 ? 1.45 : 1.0) * (1.0 + 0.45 * ax * ax) * (b.exc ? 1.3 : 1.0);
+        return (double)it * (b.rows == 2 ? 1.45 : 1.0) * (1.0 + 0.45 * ax * ax) *
+               (b.exc ? 1.3 : (b.box ? 1.05 : 1.0));
     };
     std::stable_sort(boxes.begin(), boxes.end(),
                      [&](const TB3Box &p, const TB3Box &q) { return cost(p) > cost(q); });
@@ -2690,6 +2742,7 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
     c->tb3_T = T;
     c->tb3_lx = lx;
     c->tb3_key[0] = key_src; c->tb3_key[1] = key_drv; c->tb3_key[2] = c->tb3_rows0;
+    c->tb3_key[3] = c->nmon;
     return 0;
 }
 
@@ -2742,9 +2795,21 @@ int tb_chunk_3d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
     f.ntile = c->tb3_ntile;
     f.ticket = c->tb3_ticket;
     f.base = c->tb3_base;
-    for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
+    for (int s = 0; s < T; ++s) { f.st[s] = step_args(steps[s]); f.mask[s] = steps[s].mask; }
+    int bid[2];
+    const unsigned bbits = c->tb3_ntile <= kTB3BoxSlots ? tb3_fused_boxes(c, bid) : 0u;
+    const int nfused = __builtin_popcount(bbits);
+    for (int m = 0; m < nfused; ++m) {
+        const BoxArgs &bx = c->mon[bid[m]].box;
+        f.boxbit[m] = (unsigned)bid[m];
+        for (int q = 0; q < 3; ++q)
+            for (int ax = 0; ax < 3; ++ax) { f.blo[m][q][ax] = bx.lo[q][ax]; f.bhi[m][q][ax] = bx.hi[q][ax]; }
+        f.bpart[m] = bx.partial + kBoxBlocks;
+        f.nboxm = m + 1;
+    }
     const bool loss = f.dax || f.day || f.daz;
     launch_yee_tb3(T, loss, f, c->ex, fp64, (unsigned)c->tb3_nwg, c->stream);
+    LF_HIP(hipGetLastError());
     // every workgroup draws its tiles and one ticket past the end
     c->tb3_base += (unsigned)(c->tb3_ntile + c->tb3_nwg);
     const int ids[6] = {LF_EX, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ};
@@ -2819,7 +2884,8 @@ int lf_add_box(lf_ctx *c, const int *r, int *id) {
     for (int q = 0; q < b.ncomp; ++q)
         for (int a = 0; a < 3; ++a)
             if (b.hi[q][a] < b.lo[q][a] || b.lo[q][a] < 0) return fail(-1, "lf_add_box: bad range");
-    m->bytes = kBoxBlocks * sizeof(double);
+    // 3D: one FP64 slot per tile of the blocked pass follows the per-block partials
+    m->bytes = (kBoxBlocks + (c->kind == LF_YEE3D ? kTB3BoxSlots : 0)) * sizeof(double);
     if (int e = alloc_zero(c, &m->buf, m->bytes)) return e;
     b.partial = (double *)m->buf;
     return 0;
@@ -2947,7 +3013,7 @@ int lf_monitor_write(lf_ctx *c, int id, const void *src, long long bytes) {
         return 0;
     }
     if (m.type == 0) {                          // box: the value goes to partial[0]
-        std::vector<double> part(kBoxBlocks, 0.0);
+        std::vector<double> part(m.bytes / sizeof(double), 0.0);
         part[0] = *(const double *)src;
         LF_HIP(hipMemcpy(m.buf, part.data(), m.bytes, hipMemcpyHostToDevice));
         return 0;
@@ -3006,6 +3072,37 @@ extern "C" int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
 
 namespace {
 
+// monitors whose bit is set in `mask`, on the fields at the end of step t
+int launch_monitors(lf_ctx *c, const MonArgs &ma, const lf_step &t, const StepArgs &st,
+                    unsigned mask) {
+    for (int id = 0; mask && id < c->nmon; ++id) {
+        if (!((mask >> id) & 1u)) continue;
+        Monitor &m = c->mon[id];
+        if (m.type == 0) {
+            box_kernel<<<kBoxBlocks, 256, 0, c->stream>>>(ma, m.box);
+        } else if (m.type == 1) {
+            const dim3 mb = mon_block(m.n1);
+            dim3 gq((m.n2 + mb.x - 1) / mb.x, (m.n1 + mb.y - 1) / mb.y, m.n0);
+            sq_kernel<<<gq, mb, 0, c->stream>>>(ma, (float *)m.buf, m.form, m.nxa, m.n1, m.n2);
+        } else if (m.type == 2) {
+            const DftArgs &d = m.dft;
+            const dim3 mb = mon_block(d.n1);
+            dim3 gq((d.n2 + mb.x - 1) / mb.x, (d.n1 + mb.y - 1) / mb.y, d.n0);
+            const float fr = d.phasor == LF_PH_Q ? t.qr32 : t.pr32;
+            const float fi = d.phasor == LF_PH_Q ? t.qi32 : t.pi32;
+            dft_kernel<<<gq, mb, 0, c->stream>>>(ma, d, t.pr, t.pi, fr, fi);
+        } else {
+            const FluxArgs &f = m.flux;
+            const int n1 = f.n1[0] > (f.npair > 1 ? f.n1[1] : 0) ? f.n1[0] : f.n1[1];
+            const int n2 = f.n2[0] > (f.npair > 1 ? f.n2[1] : 0) ? f.n2[0] : f.n2[1];
+            dim3 gq((n2 + 255) / 256, n1, f.npair);
+            flux_kernel<<<gq, 256, 0, c->stream>>>(ma, f, st, t.pr, t.pi, t.hr, t.hi, t.pr32,
+                                                   t.pi32, t.hr32, t.hi32);
+        }
+    }
+    return 0;
+}
+
 // the step loop of lf_run; allow_tb = false runs one step per launch (the batch engine
 // advances the instances it does not block itself this way)
 int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
@@ -3044,8 +3141,20 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         StepArgs st = step_args(t);
         const int T = allow_tb ? c->tb : 1;
         if (T > 1 && s + T <= nsteps) {
-            bool quiet = true;                       // no monitor in the block's steps
-            for (int u = 0; u < T; ++u) quiet = quiet && steps[s + u].mask == 0;
+            // 2D: no monitor in the block's steps.  3D: the pass sums the detector boxes
+            // itself, and the monitors of its last step run after it on the fields it
+            // leaves (they read only the fields at the end of their step)
+            int fid[2];
+            unsigned fused = 0u;
+            if (c->kind == LF_YEE3D && T <= 3) {
+                if (int e = tb3_plan(c, T, c->tb_lx)) return e;
+                if (c->tb3_ntile <= kTB3BoxSlots) fused = tb3_fused_boxes(c, fid);
+            }
+            bool quiet = true;
+            for (int u = 0; u < T; ++u) {
+                const bool last = c->kind == LF_YEE3D && u == T - 1;
+                quiet = quiet && (last || (steps[s + u].mask & ~fused) == 0);
+            }
             if (quiet) {
                 if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
                 if (c->kind == LF_YEE3D) {
@@ -3061,6 +3170,8 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
                 else yee = yee_args(c);
                 ma = mon_args(c);
                 s += T - 1;
+                const lf_step &tl = steps[s];
+                if (int e = launch_monitors(c, ma, tl, step_args(tl), tl.mask & ~fused)) return e;
                 continue;
             }
         }
@@ -3083,32 +3194,7 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         }
         if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
         ++ngroup;
-        const unsigned mask = t.mask;
-        for (int id = 0; mask && id < c->nmon; ++id) {
-            if (!((mask >> id) & 1u)) continue;
-            Monitor &m = c->mon[id];
-            if (m.type == 0) {
-                box_kernel<<<kBoxBlocks, 256, 0, c->stream>>>(ma, m.box);
-            } else if (m.type == 1) {
-                const dim3 mb = mon_block(m.n1);
-                dim3 gq((m.n2 + mb.x - 1) / mb.x, (m.n1 + mb.y - 1) / mb.y, m.n0);
-                sq_kernel<<<gq, mb, 0, c->stream>>>(ma, (float *)m.buf, m.form, m.nxa, m.n1, m.n2);
-            } else if (m.type == 2) {
-                const DftArgs &d = m.dft;
-                const dim3 mb = mon_block(d.n1);
-                dim3 gq((d.n2 + mb.x - 1) / mb.x, (d.n1 + mb.y - 1) / mb.y, d.n0);
-                const float fr = d.phasor == LF_PH_Q ? t.qr32 : t.pr32;
-                const float fi = d.phasor == LF_PH_Q ? t.qi32 : t.pi32;
-                dft_kernel<<<gq, mb, 0, c->stream>>>(ma, d, t.pr, t.pi, fr, fi);
-            } else {
-                const FluxArgs &f = m.flux;
-                const int n1 = f.n1[0] > (f.npair > 1 ? f.n1[1] : 0) ? f.n1[0] : f.n1[1];
-                const int n2 = f.n2[0] > (f.npair > 1 ? f.n2[1] : 0) ? f.n2[0] : f.n2[1];
-                dim3 gq((n2 + 255) / 256, n1, f.npair);
-                flux_kernel<<<gq, 256, 0, c->stream>>>(ma, f, st, t.pr, t.pi, t.hr, t.hi, t.pr32,
-                                                       t.pi32, t.hr32, t.hi32);
-            }
-        }
+        if (int e = launch_monitors(c, ma, t, st, t.mask)) return e;
     }
     LF_HIP(hipGetLastError());
     if (timing) {

@@ -20,11 +20,13 @@ namespace lfd {
 // recomputed halo value never sees a partly advanced neighbour.  Per cell and step the
 // arithmetic is that of yee_h_kernel + yee_e_kernel (boundaries3d.py:311-362).
 constexpr int kTB3W = 16;            // wavefronts per workgroup
-constexpr int kTB3MaxBox = 64;
+constexpr int kTB3MaxBox = 128;
+constexpr int kTB3BoxSlots = 16384;   // per-tile FP64 slots of a fused detector box
 
 struct TB3Box {                      // a box of tiles of one wavefront class
     int mode, rows;                  // class bits (1 z, 2 y, 4 x: slab or edge), rows per wavefront
     int exc;                         // the tiles' march meets a source or drive plane
+    int box;                         // the tiles' planes meet a fused detector box
     int x0, x1, lx;                  // owned x range, x chunk length
     int y0, y1, z0, z1;              // owned y / z ranges
     int nty, ntz;                    // tiles along y and z (x chunks make up the rest)
@@ -47,6 +49,14 @@ struct TB3Args {
     unsigned *ticket;                // tile tickets: workgroups draw tiles until ntile
     unsigned base;                   // ticket value of this pass's first tile
     StepArgs st[4];
+    unsigned mask[4];                // monitor mask of each step of the pass
+    // detector boxes summed inside the pass (sum of Ex^2 + Ey^2 + Ez^2 over a box, every
+    // step whose mask has the box's bit): per box and component (Ex, Ey, Ez) the index
+    // ranges, and one FP64 slot per tile
+    int nboxm;
+    unsigned boxbit[2];
+    int blo[2][3][3], bhi[2][3][3];
+    double *bpart[2];
 };
 
 // launch of one blocked pass: depth T (2 or 3), loss factors present or not

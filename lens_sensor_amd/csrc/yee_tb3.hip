@@ -33,9 +33,9 @@ __device__ __forceinline__ int tb3_hidx(int s, int w, int comp, int lane) {
     return 2 * T * kTB3W * 128 + ((s * kTB3W + w) * 2 + comp) * 64 + lane;
 }
 
-template <int T, int R, int MODE, bool LOSS, bool EXC>
+template <int T, int R, int MODE, bool LOSS, bool EXC, bool BOX>
 __device__ __forceinline__ void yee_tb_body(const TB3Args &a, const Excite &x, const TB3Box &bx,
-                                            int u, float *lds, int fp64) {
+                                            int u, float *lds, int fp64, double *bacc) {
     constexpr bool PZ = MODE & 1, PY = MODE & 2, PX = MODE & 4;
     constexpr int WY = kTB3W * R - 2 * T, WZ = 64 - 2 * T;
     const int lane = threadIdx.x & 63;
@@ -335,6 +335,23 @@ __device__ __forceinline__ void yee_tb_body(const TB3Args &a, const Excite &x, c
                         if (EXC) v = apply_drive(v, LF_EZ, q, plane, x, a.st[s]);
                         ezn = v;
                     }
+                    // fused detector boxes: owned cells of this step (fdtd3d_core.py:477-481)
+                    if (BOX && a.nboxm > 0 && q >= X0 && q < X1 && jown[r]) {
+#pragma unroll
+                        for (int m = 0; m < 2; ++m) {
+                            if (m >= a.nboxm || !((a.mask[s] >> a.boxbit[m]) & 1u)) continue;
+                            const float vv[3] = {exn, eyn, ezn};
+                            double add = 0.0;
+#pragma unroll
+                            for (int cc = 0; cc < 3; ++cc) {
+                                const bool in = kown && q >= a.blo[m][cc][0] && q < a.bhi[m][cc][0] &&
+                                                j >= a.blo[m][cc][1] && j < a.bhi[m][cc][1] &&
+                                                k >= a.blo[m][cc][2] && k < a.bhi[m][cc][2];
+                                if (in) add += (double)(vv[cc] * vv[cc]);
+                            }
+                            bacc[m * 64] += add;            // this lane's own LDS slot
+                        }
+                    }
                 }
                 cEx[s + 1][r] = exn; cEy[s + 1][r] = eyn; cEz[s + 1][r] = ezn;
                 cHx[s + 1][r] = hx; cHy[s + 1][r] = hy; cHz[s + 1][r] = hz;
@@ -396,53 +413,90 @@ __device__ __forceinline__ void yee_tb_body(const TB3Args &a, const Excite &x, c
     }
 }
 
-// Persistent workgroups (one per CU) draw tiles in order from a ticket counter; the host
-// orders the boxes by decreasing tile cost, so the pass ends on short tiles.
+// Persistent workgroups (one per CU) draw the tiles of the cost-ordered list from a
+// ticket counter, so that the pass ends on short tiles whatever each tile costs.  The
+// ticket is drawn by one thread, passed through LDS and read into a scalar register:
+// the loop exit and every branch around a barrier are uniform over the workgroup, and
+// no barrier stands under a branch, so all wavefronts meet the same barriers.
 template <int T, bool LOSS>
 __global__ void __launch_bounds__(1024) yee_tb_kernel(TB3Args a, Excite x, int fp64) {
     __shared__ float lds[3 * T * kTB3W * 128];
+    __shared__ double red[kTB3W][2];
+    __shared__ double boxacc[kTB3W][2][64];               // per-lane box sums of a tile
     __shared__ int tile_s;
     const TB3Box __attribute__((address_space(4))) *bt =
         (const TB3Box __attribute__((address_space(4))) *)a.box;
-    for (;;) {
-        if (threadIdx.x == 0) tile_s = (int)(atomicAdd(a.ticket, 1u) - a.base);
-        __syncthreads();
-        const int t = tile_s;
-        __syncthreads();                                   // every wave has read tile_s
-        if (t >= a.ntile) return;                          // the whole workgroup leaves
+    if (threadIdx.x == 0) tile_s = (int)(atomicAdd(a.ticket, 1u) - a.base);
+    __syncthreads();
+    int t = __builtin_amdgcn_readfirstlane(tile_s);
+    while (t < a.ntile) {
         int b = 0;
         while (b + 1 < a.nbox && t >= bt[b + 1].first) ++b;
         const TB3Box bx = *(const TB3Box *)&bt[b];
         const int u = t - bx.first;
-        // tiles whose march meets a source or drive plane take the bodies with excitation;
-        // the others carry no excitation loads (whose waits would also drain the plane
-        // prefetch) and the slab-free class holds two rows per wavefront
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        double *bacc = &boxacc[w][0][lane];
+        bacc[0] = 0.0;
+        bacc[64] = 0.0;
+        // tiles whose march meets a source or drive plane take the bodies with excitation
+        // (and box sums); tiles whose planes meet a fused detector box the bodies with box
+        // sums; the others carry neither (the excitation loads' waits would also drain
+        // the plane prefetch, and both cost registers).  The slab-free class holds two
+        // rows per wavefront unless it carries excitation.
+#define LF_TB3_BODY(R, M, E, B) yee_tb_body<T, R, M, LOSS, E, B>(a, x, bx, u, lds, fp64, bacc)
+#define LF_TB3_MODES(E, B)                                                                   \
+    switch (bx.mode) {                                                                       \
+        case 0:                                                                              \
+            if (bx.rows == 2) LF_TB3_BODY(2, 0, E, B);                                       \
+            else LF_TB3_BODY(1, 0, E, B);                                                    \
+            break;                                                                           \
+        case 1: LF_TB3_BODY(1, 1, E, B); break;                                              \
+        case 2: LF_TB3_BODY(1, 2, E, B); break;                                              \
+        case 3: LF_TB3_BODY(1, 3, E, B); break;                                              \
+        case 4: LF_TB3_BODY(1, 4, E, B); break;                                              \
+        case 5: LF_TB3_BODY(1, 5, E, B); break;                                              \
+        case 6: LF_TB3_BODY(1, 6, E, B); break;                                              \
+        default: LF_TB3_BODY(1, 7, E, B); break;                                             \
+    }
         if (bx.exc) {
             switch (bx.mode) {
-                case 0: yee_tb_body<T, 1, 0, LOSS, true>(a, x, bx, u, lds, fp64); break;
-                case 1: yee_tb_body<T, 1, 1, LOSS, true>(a, x, bx, u, lds, fp64); break;
-                case 2: yee_tb_body<T, 1, 2, LOSS, true>(a, x, bx, u, lds, fp64); break;
-                case 3: yee_tb_body<T, 1, 3, LOSS, true>(a, x, bx, u, lds, fp64); break;
-                case 4: yee_tb_body<T, 1, 4, LOSS, true>(a, x, bx, u, lds, fp64); break;
-                case 5: yee_tb_body<T, 1, 5, LOSS, true>(a, x, bx, u, lds, fp64); break;
-                case 6: yee_tb_body<T, 1, 6, LOSS, true>(a, x, bx, u, lds, fp64); break;
-                default: yee_tb_body<T, 1, 7, LOSS, true>(a, x, bx, u, lds, fp64); break;
+                case 0: LF_TB3_BODY(1, 0, true, true); break;
+                case 1: LF_TB3_BODY(1, 1, true, true); break;
+                case 2: LF_TB3_BODY(1, 2, true, true); break;
+                case 3: LF_TB3_BODY(1, 3, true, true); break;
+                case 4: LF_TB3_BODY(1, 4, true, true); break;
+                case 5: LF_TB3_BODY(1, 5, true, true); break;
+                case 6: LF_TB3_BODY(1, 6, true, true); break;
+                default: LF_TB3_BODY(1, 7, true, true); break;
             }
+        } else if (bx.box) {
+            LF_TB3_MODES(false, true)
         } else {
-            switch (bx.mode) {
-                case 0:
-                    if (bx.rows == 2) yee_tb_body<T, 2, 0, LOSS, false>(a, x, bx, u, lds, fp64);
-                    else yee_tb_body<T, 1, 0, LOSS, false>(a, x, bx, u, lds, fp64);
-                    break;
-                case 1: yee_tb_body<T, 1, 1, LOSS, false>(a, x, bx, u, lds, fp64); break;
-                case 2: yee_tb_body<T, 1, 2, LOSS, false>(a, x, bx, u, lds, fp64); break;
-                case 3: yee_tb_body<T, 1, 3, LOSS, false>(a, x, bx, u, lds, fp64); break;
-                case 4: yee_tb_body<T, 1, 4, LOSS, false>(a, x, bx, u, lds, fp64); break;
-                case 5: yee_tb_body<T, 1, 5, LOSS, false>(a, x, bx, u, lds, fp64); break;
-                case 6: yee_tb_body<T, 1, 6, LOSS, false>(a, x, bx, u, lds, fp64); break;
-                default: yee_tb_body<T, 1, 7, LOSS, false>(a, x, bx, u, lds, fp64); break;
-            }
+            LF_TB3_MODES(false, false)
         }
+#undef LF_TB3_MODES
+#undef LF_TB3_BODY
+        // tile sums of the fused detector boxes: fixed order, one slot per tile.  The
+        // barriers run for every tile (no barrier under a branch); the second one keeps
+        // the next tile's writes to red[] behind this tile's reads.
+        double bacc0 = bacc[0], bacc1 = bacc[64];
+        for (int o = 32; o > 0; o >>= 1) {
+            bacc0 += __shfl_xor(bacc0, o);
+            bacc1 += __shfl_xor(bacc1, o);
+        }
+        if (lane == 0) { red[w][0] = bacc0; red[w][1] = bacc1; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (a.nboxm > 0) {
+                double s0 = 0.0, s1 = 0.0;
+                for (int q = 0; q < kTB3W; ++q) { s0 += red[q][0]; s1 += red[q][1]; }
+                a.bpart[0][t] += s0;
+                if (a.nboxm > 1) a.bpart[1][t] += s1;
+            }
+            tile_s = (int)(atomicAdd(a.ticket, 1u) - a.base);   // next tile
+        }
+        __syncthreads();
+        t = __builtin_amdgcn_readfirstlane(tile_s);
     }
 }
 

@@ -106,7 +106,7 @@ def _compare(kind, ho, oo, mons, exact=True, tol=1e-6):
             assert rel <= tol, (kind, c, rel)
     for name, mid in mons.items():
         a, b = ho.read_monitor(mid), oo.read_monitor(mid)
-        if name == 'box':
+        if name.startswith('box'):
             btol = 1e-12 if exact else tol
             assert abs(a - b) <= btol * abs(b) + 1e-300, (kind, name, a, b)
         elif name == 'flux':

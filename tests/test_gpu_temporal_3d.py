@@ -83,3 +83,37 @@ def test_yee_temporal_setter_rejects_bad_depth():
     with pytest.raises(RuntimeError):
         e.set_temporal(4, 8)
     e.close()
+
+
+@pytest.mark.parametrize('mode,dims', [('ref32', (40, 70, 200)), ('fp64acc', (40, 70, 200)),
+                                       ('fp64acc', (72, 90, 150))])
+def test_yee_blocked_fused_boxes_and_pass_end_monitors(mode, dims):
+    """Detector boxes on every step are summed inside the pass; the intensity and phasor
+    monitors of a pass's last step run after it.  Fields, intensity and phasors equal
+    depth 1 bit for bit, the FP64 box sums to 1e-12 (another summation order).  The second
+    box lies in the slab-free region, away from the source and drive planes."""
+    H = _hip()
+    print(f'[3d blocked fused boxes] mode={mode} dims={dims}', flush=True)
+    a, b = H('yee3d', dims, mode), H('yee3d', dims, mode)
+    a.set_temporal(3, 16)
+    b.set_temporal(1, 64)
+    ma = _setup(a, 'yee3d', dims, np.random.default_rng(5), source=True, drive=True, pec=False)
+    _setup(b, 'yee3d', dims, np.random.default_rng(5), source=True, drive=True, pec=False)
+    box2 = ((dims[0] // 2 - 8, dims[0] // 2 + 9), (30, 47), (10, -10))
+    ma['box2'] = a.add_box(box2)
+    b.add_box(box2)
+    n = 30
+    tab = _table(n, 0, n)
+    mask = np.zeros(n, np.uint32)
+    mask[:] |= (1 << ma['box']) | (1 << ma['box2'])    # boxes every step
+    mask[2::3] |= 1 << ma['sq_s']                  # intensity on every pass's last step
+    mask[5::6] |= (1 << ma['dft_pl']) | (1 << ma['dft_q'])
+    tab['mask'] = mask
+    a.set_timing(True)
+    a.run(tab)
+    b.run(tab)
+    ms, launches, d = a.tb_stats()
+    assert launches == n // 3
+    _compare('yee3d', a, b, ma)
+    a.close()
+    b.close()

@@ -10,7 +10,8 @@ of one window (12.6 M cells); value = cell updates / second of device stepping t
 
 --workload yee3d: 3D FP32 Yee + CPML, 512 x 256 x 256 at dx = 30 nm, 12-cell CPML on
 all faces, 4-layer stack + lipid, one bead, Gaussian CW source, 8000 steps, FP64 phasor
-sums on one monitor plane.
+sums on one monitor plane; the detector boxes of every step are summed inside the
+temporally blocked passes (depth 3), the steps of the final CW window run one per launch.
 
 Prints ONE JSON line (metric, value, roofline of the dominant kernel, CPU baseline of the
 NumPy oracle on a bounded sample of the same workload).
@@ -151,7 +152,7 @@ def bench_yee3d(args):
                 upd_ms_per_step=e._upd / max(e._n, 1), cells_per_step=float(cells),
                 bytes_per_cell=BYTES_YEE3D, grid=(nx, ny, nz), steps_per_window=steps,
                 tb_ms=e._tb_ms, tb_launches=e._tb_n, tb_depth=e._tb_depth,
-                kernel='yee_h_kernel + yee_e_kernel')
+                kernel='yee_tb_kernel', pass_desc='one blocked pass, persistent workgroups')
 
 
 def bench_sweep(args):
@@ -329,7 +330,8 @@ def main():
         # tb_depth cell updates per cell per launch, HIP events on the context stream
         launch_ms = res['tb_ms'] / res['tb_launches']
         units = res['cells_per_step'] * res['tb_depth']
-        kname = f"{res['kernel']}<{res['tb_depth']}> (one blocked pass: slab + interior launches)"
+        kname = (f"{res['kernel']}<{res['tb_depth']}> "
+                 f"({res.get('pass_desc', 'one blocked pass: slab + interior launches')})")
     else:
         launch_ms = res['upd_ms_per_step']
         units = res['cells_per_step']

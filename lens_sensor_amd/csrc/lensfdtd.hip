@@ -426,7 +426,7 @@ __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j
 
 // Body of one wavefront (index wv within its instance's launch segments); st[s] are the
 // per-step scalars of step n+s.
-template <int T, int MODE>
+template <int T, int MODE, bool EXC = true>
 __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, const StepArgs *st,
                                            int wv, int fp64) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
@@ -553,8 +553,8 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                         }
                         v = v + (-cY[s + 1]) * t;
                     }
-                    v = apply_source(v, LF_EY, q, j, x, st[s], fp64);
-                    v = apply_drive(v, LF_EY, q, j, x, st[s]);
+                    if (EXC) v = apply_source(v, LF_EY, q, j, x, st[s], fp64);
+                    if (EXC) v = apply_drive(v, LF_EY, q, j, x, st[s]);
                     if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     eyn = v;
                 }
@@ -599,11 +599,13 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
 // slab wavefronts are few long serial chains that run beside the interior launch: they
 // take issue priority on their SIMD so that they finish with it.  The wave index is read
 // into a scalar register, so every x/plane test is a scalar branch.
-template <int T, int MODE>
+// EXC = false: no source or drive plane lies within the launch's marches, so the body
+// carries no excitation loads (whose waits would also drain the plane prefetch).
+template <int T, int MODE, bool EXC = true>
 __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
     if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    tm_tb_body<T, MODE>(a, x, a.st, wv, fp64);
+    tm_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
 }
 
 // ---------------------------------------------------------------------------- temporal blocking (2D TE)
@@ -663,7 +665,7 @@ __device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int j
     return v;
 }
 
-template <int T, int MODE>
+template <int T, int MODE, bool EXC = true>
 __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, const StepArgs *st,
                                            int wv, int fp64) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
@@ -771,8 +773,8 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                         }
                         v = cE[s + 1] * v + cH[s + 1] * t;
                     }
-                    v = apply_source(v, LF_EZ, q, j, x, st[s], fp64);
-                    v = apply_drive(v, LF_EZ, q, j, x, st[s]);
+                    if (EXC) v = apply_source(v, LF_EZ, q, j, x, st[s], fp64);
+                    if (EXC) v = apply_drive(v, LF_EZ, q, j, x, st[s]);
                     if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     if (!YIN && ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1))) v = 0.0f;
                     ezn = v;
@@ -809,11 +811,11 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
     }
 }
 
-template <int T, int MODE>
+template <int T, int MODE, bool EXC = true>
 __global__ void __launch_bounds__(256) te_tb_kernel(TBTEArgs a, Excite x, int fp64) {
     if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    te_tb_body<T, MODE>(a, x, a.st, wv, fp64);
+    te_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
 }
 
 // ---------------------------------------------------------------------------- batched passes
@@ -2182,6 +2184,7 @@ bool tm_x_slab(int q) { return x_slab_2d(LF_TM2D, q); }
 // slab-free rectangle.
 struct TBClass {
     int mode, lx, nseg;
+    int exc;                  // 0: no source / drive plane within the class's marches
     int wx0[4], wy0[4], nwy[4], end[4];
 };
 struct TBPlan {
@@ -2227,6 +2230,7 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
         return 1;
     };
     pl.cls[0].mode = 0; pl.cls[1].mode = 1; pl.cls[2].mode = 3;
+    pl.cls[0].exc = pl.cls[1].exc = pl.cls[2].exc = 1;
     if (!pl.interior) {
         const int k = div(4);
         pl.cls[2].lx = lx / k;
@@ -2236,6 +2240,14 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
     const int ni = iy1 - iy0 + 1;
     pl.cls[0].lx = lx;
     add(pl.cls[0], ix0, ix1 - ix0 + 1, iy0, ni);
+    {   // does a source or drive plane lie within the slab-free chunks' marches?
+        const int comp = c->kind == LF_TM2D ? LF_EY : LF_EZ;
+        const int m0 = ix0 * lx - T, m1 = std::min((ix1 + 1) * lx, Nx) + T;
+        bool e = false;
+        if (c->ex.amp && c->ex.src_comp == comp) e = e || (c->ex.src_ix < m1 && c->ex.src_ix + 2 > m0);
+        if (c->ex.wtap && c->ex.drv[comp]) e = e || (0 < m1 && c->ex.drv_n > m0);
+        pl.cls[0].exc = e ? 1 : 0;
+    }
     const int k = div(4);                                 // y-slab rows: chunks of lx/k
     pl.cls[1].lx = lx / k;
     const int rx0 = ix0 * k, rx1 = (ix1 + 1) * k - 1;
@@ -2268,18 +2280,19 @@ int ensure_side_streams(lf_ctx *c) {
     return 0;
 }
 
-template <int T, int MODE>
+template <int T, int MODE, bool EXC = true>
 void launch_tb(const TBTMArgs &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
-    tm_tb_kernel<T, MODE><<<nb, 256, 0, st>>>(q, x, fp64);
+    tm_tb_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
-template <int T, int MODE>
+template <int T, int MODE, bool EXC = true>
 void launch_tb(const TBTEArgs &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
-    te_tb_kernel<T, MODE><<<nb, 256, 0, st>>>(q, x, fp64);
+    te_tb_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
 template <typename A, int T>
-void launch_tb_mode(const A &q, int mode, const Excite &x, int fp64, hipStream_t st) {
+void launch_tb_mode(const A &q, int mode, bool exc, const Excite &x, int fp64, hipStream_t st) {
     const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
-    if (mode == 0) launch_tb<T, 0>(q, x, fp64, st, nb);
+    if (mode == 0 && !exc) launch_tb<T, 0, false>(q, x, fp64, st, nb);
+    else if (mode == 0) launch_tb<T, 0>(q, x, fp64, st, nb);
     else if (mode == 1) launch_tb<T, 1>(q, x, fp64, st, nb);
     else launch_tb<T, 3>(q, x, fp64, st, nb);
 }
@@ -2294,10 +2307,10 @@ int launch_tb_pass(lf_ctx *c, A &f, const TBPlan &pl, int T, int fp64) {
         A q = f;
         set_class(q, k);
         switch (T) {
-            case 2: launch_tb_mode<A, 2>(q, k.mode, c->ex, fp64, st); break;
-            case 4: launch_tb_mode<A, 4>(q, k.mode, c->ex, fp64, st); break;
-            case 6: launch_tb_mode<A, 6>(q, k.mode, c->ex, fp64, st); break;
-            default: launch_tb_mode<A, 8>(q, k.mode, c->ex, fp64, st); break;
+            case 2: launch_tb_mode<A, 2>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
+            case 4: launch_tb_mode<A, 4>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
+            case 6: launch_tb_mode<A, 6>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
+            default: launch_tb_mode<A, 8>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
         }
     };
     if (!pl.interior) {

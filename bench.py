@@ -53,6 +53,7 @@ def _timed_factory(records):
             self._tb_ms = 0.0
             self._tb_n = 0
             self._tb_depth = 1
+            self._tb_units = 0
             records.append(self)
 
         def run(self, tab):
@@ -65,6 +66,7 @@ def _timed_factory(records):
             self._tb_ms += ms
             self._tb_n += nl
             self._tb_depth = d
+            self._tb_units += nl * d * int(np.prod(self.dims))    # cell updates of the passes
 
     return TimedHip
 
@@ -120,7 +122,7 @@ def bench_tm2d_stitch(args):
                 bytes_per_cell=BYTES_TM2D_LOSSY, windows=len(info), grid=(nx_win, ny),
                 steps_per_window=steps, tb_ms=sum(e._tb_ms for e in records),
                 tb_launches=sum(e._tb_n for e in records), tb_depth=records[0]._tb_depth,
-                kernel='tm_tb_kernel')
+                tb_units=sum(e._tb_units for e in records), kernel='tm_tb_kernel')
 
 
 def bench_yee3d(args):
@@ -152,7 +154,8 @@ def bench_yee3d(args):
                 upd_ms_per_step=e._upd / max(e._n, 1), cells_per_step=float(cells),
                 bytes_per_cell=BYTES_YEE3D, grid=(nx, ny, nz), steps_per_window=steps,
                 tb_ms=e._tb_ms, tb_launches=e._tb_n, tb_depth=e._tb_depth,
-                kernel='yee_tb_kernel', pass_desc='one blocked pass, persistent workgroups')
+                tb_units=e._tb_units, kernel='yee_tb_kernel',
+                pass_desc='one blocked pass, persistent workgroups')
 
 
 def bench_sweep(args):
@@ -329,7 +332,7 @@ def main():
         # dominant kernel: the temporally blocked pass (its slab and interior launches),
         # tb_depth cell updates per cell per launch, HIP events on the context stream
         launch_ms = res['tb_ms'] / res['tb_launches']
-        units = res['cells_per_step'] * res['tb_depth']
+        units = res['tb_units'] / res['tb_launches']             # cell updates per launch
         kname = (f"{res['kernel']}<{res['tb_depth']}> "
                  f"({res.get('pass_desc', 'one blocked pass: slab + interior launches')})")
     else:

@@ -116,6 +116,13 @@ int lf_set_temporal(lf_ctx *ctx, int depth, int lx);
 /* Blocked passes of the last lf_run with timing on: summed HIP-event time (ms), number of
  * launches, current depth. */
 int lf_tb_stats(lf_ctx *ctx, double *ms, int *launches, int *depth);
+/* Autotuner: times every candidate temporal-block depth and x chunk (2D: depth 2/4/6/8 x
+ * chunk 32/64/128; 3D: depth 2/3 x chunk 32/48/64/96) on this context with `reps`
+ * passes of monitor-free copies of `row`, restores every field and psi slab, and keeps
+ * the fastest (results are bit-identical for every choice).  No counterpart in the
+ * reference, whose step loop has no blocking; it tunes the lf_set_temporal setting. */
+int lf_autotune(lf_ctx *ctx, const lf_step *row, int reps, int *depth, int *lx,
+                double *ms_per_step);
 
 /* ---- Batch engine: many independent 2D domains advanced together.  No reference
  * counterpart: the reference runs sweep instances and requests one after another, each

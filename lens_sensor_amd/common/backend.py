@@ -78,3 +78,26 @@ def use_engine(factory):
         yield
     finally:
         _factory_override = prev
+
+
+# Runs of at least this many steps tune the engine's temporal blocking (depth, x chunk)
+# before stepping; LENSFDTD_AUTOTUNE=0 turns it off.  The blocked passes give identical
+# results for every choice, so tuning changes only speed.
+AUTOTUNE_MIN_STEPS = 2000
+AUTOTUNE = os.environ.get('LENSFDTD_AUTOTUNE', '1') != '0'
+
+
+def tune(eng, row, nsteps, cache=None, key=None):
+    """Tune ``eng`` with the step record ``row`` when the run is long enough and the engine
+    supports it.  ``cache`` (a dict) with ``key`` reuses one tuning for engines of the same
+    shape and setup, e.g. the windows of a stitch run.  Returns (depth, lx) or None."""
+    if not AUTOTUNE or nsteps < AUTOTUNE_MIN_STEPS or not hasattr(eng, 'autotune'):
+        return None
+    if cache is not None and key in cache:
+        depth, lx = cache[key]
+        eng.set_temporal(depth, lx)
+        return depth, lx
+    depth, lx, _ms = eng.autotune(row)
+    if cache is not None:
+        cache[key] = (depth, lx)
+    return depth, lx

@@ -3241,6 +3241,113 @@ extern "C" {
 
 }  // extern "C"
 
+// ============================================================================ autotuner (C ABI)
+namespace {
+
+// device copies of every field and psi slab of a context (the state a blocked pass reads
+// and writes), for the autotuner's trial passes
+struct Snapshot {
+    std::vector<std::pair<float **, size_t>> src;   // where the live buffer pointer sits, floats
+    std::vector<float *> copy;
+};
+
+int snapshot_take(lf_ctx *c, Snapshot &sn) {
+    const int ids3[6] = {LF_EX, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ};
+    const int idsTE[3] = {LF_EZ, LF_HX, LF_HY}, idsTM[3] = {LF_EX, LF_EY, LF_HZ};
+    const int *ids = c->kind == LF_YEE3D ? ids3 : (c->kind == LF_TE2D ? idsTE : idsTM);
+    const int n = c->kind == LF_YEE3D ? 6 : 3;
+    for (int q = 0; q < n; ++q) sn.src.push_back({&c->arr[ids[q]], grid_elems(c)});
+    if (c->kind == LF_YEE3D) {
+        const int hax[6] = {2, 1, 0, 2, 1, 0}, eax[6] = {1, 2, 2, 0, 0, 1};
+        for (int q = 0; q < 6; ++q) {
+            if (c->sH[q].psi) sn.src.push_back({&c->sH[q].psi, strip_elems_3d(c, c->sH[q], hax[q])});
+            if (c->sE[q].psi) sn.src.push_back({&c->sE[q].psi, strip_elems_3d(c, c->sE[q], eax[q])});
+        }
+    } else {
+        for (int q = 0; q < 4; ++q)
+            if (c->s2[q].psi)
+                sn.src.push_back({&c->s2[q].psi, strip_elems_2d(c, c->s2[q], x_slab_2d(c->kind, q))});
+    }
+    for (auto &e : sn.src) {
+        float *d = nullptr;
+        LF_HIP(hipMalloc(&d, e.second * sizeof(float)));
+        sn.copy.push_back(d);
+        LF_HIP(hipMemcpyAsync(d, *e.first, e.second * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    }
+    return 0;
+}
+
+// copy the snapshot back into the buffers that are live now (the trial passes swap them)
+int snapshot_restore(lf_ctx *c, Snapshot &sn) {
+    for (size_t u = 0; u < sn.src.size(); ++u)
+        LF_HIP(hipMemcpyAsync(*sn.src[u].first, sn.copy[u], sn.src[u].second * sizeof(float),
+                              hipMemcpyDeviceToDevice, c->stream));
+    LF_HIP(hipStreamSynchronize(c->stream));
+    for (float *d : sn.copy) (void)hipFree(d);
+    sn.copy.clear();
+    return 0;
+}
+
+}  // namespace
+
+// Picks the temporal-block depth and x chunk with the shortest measured time per step on
+// this context (its grid, CPML slabs, source, drive and fused detector boxes): every
+// candidate runs a warm-up pass and `reps` timed passes of monitor-free copies of `row`,
+// then every field and psi slab is restored, so the simulation state is unchanged.  The
+// blocked passes give bit-identical results for every depth and chunk, so the choice
+// affects only speed.  No counterpart in the reference (its step loop has no blocking).
+extern "C" int lf_autotune(lf_ctx *c, const lf_step *row, int reps, int *depth, int *lx,
+                           double *ms_per_step) {
+    if (int e = check_ctx(c)) return e;
+    if (!c->finalized) return fail(-1, "lf_autotune before lf_finalize");
+    if (!row) return fail(-1, "lf_autotune: null step row");
+    if (reps < 1) reps = 1;
+    std::vector<int> Ts, lxs;
+    if (c->kind == LF_YEE3D) { Ts = {2, 3}; lxs = {32, 48, 64, 96}; }
+    else { Ts = {2, 4, 6, 8}; lxs = {32, 64, 128}; }
+    const int tb0 = c->tb, lx0 = c->tb_lx;
+    const bool timing0 = c->timing;
+    c->timing = false;
+    Snapshot sn;
+    if (int e = snapshot_take(c, sn)) return e;
+    hipEvent_t e0, e1;
+    LF_HIP(hipEventCreate(&e0));
+    LF_HIP(hipEventCreate(&e1));
+    int bestT = tb0, bestLx = lx0;
+    double best = 1e30;
+    int rc = 0;
+    for (int T : Ts) {
+        for (int L : lxs) {
+            std::vector<lf_step> tab((size_t)T * (reps + 1), *row);
+            for (auto &t : tab) t.mask = 0;
+            c->tb = T;
+            c->tb_lx = L;
+            if ((rc = run_impl(c, tab.data(), T, true)) != 0) break;        // warm-up pass
+            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(-2, "lf_autotune: event"); break; }
+            if ((rc = run_impl(c, tab.data() + T, T * reps, true)) != 0) break;
+            if (hipEventRecord(e1, c->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess) { rc = fail(-2, "lf_autotune: event"); break; }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = ms / (double)(T * reps);
+            if (per < best) { best = per; bestT = T; bestLx = L; }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int rr = snapshot_restore(c, sn);
+    c->timing = timing0;
+    if (rc) { c->tb = tb0; c->tb_lx = lx0; return rc; }
+    if (rr) return rr;
+    c->tb = bestT;
+    c->tb_lx = bestLx;
+    if (depth) *depth = bestT;
+    if (lx) *lx = bestLx;
+    if (ms_per_step) *ms_per_step = best;
+    return 0;
+}
+
 // ============================================================================ batch engine (C ABI)
 struct lf_batch {
     std::vector<lf_ctx *> ctx;

@@ -31,7 +31,7 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats',
            'lf_batch_create', 'lf_batch_add', 'lf_batch_set_temporal', 'lf_batch_set_timing',
            'lf_batch_run', 'lf_batch_stats', 'lf_batch_destroy', 'lf_batch_set_table',
-           'lf_batch_run_at')
+           'lf_batch_run_at', 'lf_autotune')
 
 
 def library_path():
@@ -76,6 +76,7 @@ def load():
         'lf_set_tile': [P, I],
         'lf_set_temporal': [P, I, I],
         'lf_tb_stats': [P, ctypes.POINTER(ctypes.c_double), ip, ip],
+        'lf_autotune': [P, P, I, ip, ip, ctypes.POINTER(ctypes.c_double)],
         'lf_kernel_ms': [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ip],
         'lf_batch_create': [ctypes.POINTER(P)],
         'lf_batch_add': [P, P],
@@ -333,6 +334,18 @@ class HipEngine:
 
     def set_temporal(self, depth, lx=64):
         _check(self._lib.lf_set_temporal(self.h, int(depth), int(lx)), 'lf_set_temporal')
+
+    def autotune(self, row, reps=None):
+        """Time the candidate blocking depths / x chunks on this engine with monitor-free
+        copies of step row `row` (one lf_step record); keeps the fastest and restores the
+        fields.  Returns (depth, lx, ms per step)."""
+        row = np.ascontiguousarray(np.asarray(row, dtype=STEP_DTYPE).reshape(1))
+        d, lx, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        if reps is None:
+            reps = 3 if self.kind == 'yee3d' else 8
+        _check(self._lib.lf_autotune(self.h, row.ctypes.data, int(reps), ctypes.byref(d),
+                                     ctypes.byref(lx), ctypes.byref(ms)), 'lf_autotune')
+        return d.value, lx.value, ms.value
 
     def tb_stats(self):
         ms, n, d = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()

@@ -538,6 +538,10 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
     else:
         n_run = steps_total
 
+    row = new_steps(1)
+    fill_phasors(row, omega, dt, np.array([n_avg0 * dt]), np.array([1.0]), src_omega=vs.omega)
+    backend.tune(eng, row[0], n_run)
+
     def masks(n0, n1):
         nonlocal nacc_cw, n_pl_a, n_pl_b, n_I
         ns = np.arange(n0, n1)

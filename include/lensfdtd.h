@@ -113,6 +113,13 @@ int lf_kernel_ms(lf_ctx *ctx, double *update_ms, double *total_ms, int *nsteps);
  * (1 = off, 2, 4, 6 or 8) over runs of steps whose monitor mask is 0; lx = x planes per
  * wavefront chunk.  Results are bit-identical to depth 1.  Used by 2D TM. */
 int lf_set_temporal(lf_ctx *ctx, int depth, int lx);
+/* Depth of the monitored 2D passes (no reference counterpart: an engine tuning knob):
+ * runs of `depth` steps (1 = off, 2, 3 or 4) whose monitors are volume phasors (at most
+ * two), one intensity sum and up to four flux planes advance in one pass that also sums
+ * those monitors (the reference sums them after every step, fdtd2d_core.py:832-845,
+ * flux.py:253-291).  Every accumulator receives the same additions in the same order, so
+ * results are bit-identical to depth 1.  2D TE and TM; 3D accepts only 1. */
+int lf_set_temporal_mon(lf_ctx *ctx, int depth);
 /* Blocked passes of the last lf_run with timing on: summed HIP-event time (ms), number of
  * launches, current depth. */
 int lf_tb_stats(lf_ctx *ctx, double *ms, int *launches, int *depth);

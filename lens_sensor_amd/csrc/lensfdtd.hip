@@ -26,6 +26,7 @@
 #include "../../include/lensfdtd.h"
 #include "lensfdtd_dev.h"
 #include "yee_tb3.h"
+#include "tb2d.h"
 
 using namespace lfd;
 
@@ -349,473 +350,6 @@ __global__ void __launch_bounds__(256) tm_fused_kernel(TMFArgs a, Excite x, Step
         hz_prev = hz;
         eyc = eyn;
     }
-}
-
-// ---------------------------------------------------------------------------- temporal blocking (2D TM)
-// T time steps per pass.  A wavefront holds 64 consecutive y cells
-// j = wy*(64-2T) - T + lane and marches along its x chunk [X0, X1) from X0-T to X1+T-1.
-// Iteration p loads plane p (E^n, Hz^{n-1/2}, psi, coefficients) into pipeline slot 0;
-// level s (0 <= s < T) then advances plane q = p-1-s from step n+s to n+s+1: it needs
-// Ey^{n+s}(q+1) (this iteration's output of level s-1), E^{n+s}(q) and Hz^{n+s-1/2}(q)
-// (level s-1's output of the previous iteration) and its own Hz^{n+s+1/2}(q-1).  Every
-// step shrinks the valid y range by one cell on each side (Hz needs Ex(j+1), Ex needs
-// Hz(j-1)) and the valid x range by one plane on the low side, so after T steps lanes
-// T..63-T of planes X0..X1-1 are exact and are stored.  Inputs and outputs (fields and
-// all four psi slabs) are separate buffers.  Per cell and step the arithmetic is that of
-// tm_h_kernel + tm_e_kernel (boundaries.py:243-258, fdtd2d_core.py:796).
-constexpr int kMaxTB = 8;
-
-struct TBTMArgs {
-    Grid g;
-    const float *Ex0, *Ey0, *Hz0;
-    float *Ex1, *Ey1, *Hz1;
-    const float *cax, *cay, *dax, *day;
-    float ch;
-    Prof xe, xh, ye, yh;
-    Strip s0[4], s1[4];       // Hzx Hzy Exy Eyx: read / write buffers
-    int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
-    int lx;
-    // wavefronts of this launch: up to 4 rectangles of (x chunk, y wave) indices; wave
-    // wv of segment g (seg_end[g-1] <= wv < seg_end[g]) is
-    // (seg_wx0 + (wv - start) / seg_nwy, seg_wy0 + (wv - start) % seg_nwy)
-    int nseg;
-    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4];
-    StepArgs st[kMaxTB];
-};
-
-struct TMPlane {              // one plane of one column, as loaded from the step-n buffers
-    float ex, ey, hz, ca, cy, da, dy, ph, py, qy, qx;
-};
-
-// MODE bit 1: wavefront lanes may lie in a y slab or on a y edge (psi_Hzy, psi_Exy,
-// lane tests); bit 2: planes may lie in an x slab or on an x edge (psi_Hzx, psi_Eyx,
-// plane tests).  MODE 0 is the slab-free interior: no psi registers, no edge tests.
-template <int MODE>
-__device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j, bool jin, bool jey,
-                                                 int sHzy, int sExy, bool lossy_x, bool lossy_y) {
-    constexpr bool PX = MODE & 2, PY = MODE & 1;
-    TMPlane v{};
-    const int Nx = a.g.nx, P = a.g.P;
-    if (p < Nx) {
-        const long long c = (long long)p * P + j;
-        const bool xin = !PX || p < Nx - 1;
-        if (xin && jin) {
-            v.ex = a.Ex0[c];
-            v.ca = a.cax[c];
-            if (lossy_x) v.da = a.dax[c];
-        }
-        if (jey) {
-            v.ey = a.Ey0[c];
-            v.cy = a.cay[c];
-            if (lossy_y) v.dy = a.day[c];
-        }
-        if (xin && jey) v.hz = a.Hz0[c];
-        if (PX) {
-            const int sx0 = xin ? strip_pos(a.s0[0], p) : -1;           // H nodes p <= Nx-2
-            if (sx0 >= 0 && jey) v.ph = a.s0[0].psi[(long long)sx0 * P + j];
-            const int sx3 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[3], p) : -1;
-            if (sx3 >= 0 && jey) v.qx = a.s0[3].psi[(long long)sx3 * P + j];
-        }
-        if (PY) {
-            if (sHzy >= 0 && xin) v.py = a.s0[1].psi[(long long)p * a.s0[1].len + sHzy];
-            if (sExy >= 0 && xin) v.qy = a.s0[2].psi[(long long)p * a.s0[2].len + sExy];
-        }
-    }
-    return v;
-}
-
-// Body of one wavefront (index wv within its instance's launch segments); st[s] are the
-// per-step scalars of step n+s.
-template <int T, int MODE, bool EXC = true>
-__device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, const StepArgs *st,
-                                           int wv, int fp64) {
-    constexpr bool PX = MODE & 2, PY = MODE & 1;
-    constexpr bool XIN = !PX, YIN = !PY;            // planes / lanes strictly interior
-    const int lane = threadIdx.x & 63;
-    if (wv >= a.seg_end[a.nseg - 1]) return;       // surplus waves of the last workgroup
-    int g = 0;
-    while (g < a.nseg - 1 && wv >= a.seg_end[g]) ++g;
-    const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
-    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
-    const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
-    const int X0 = wx * a.lx;
-    if (X0 >= Nx) return;                          // whole wavefront leaves together
-    const int X1 = min(X0 + a.lx, Nx);
-    const int j = wy * (64 - 2 * T) - T + lane;
-    // the host launches a MODE only on wavefronts whose loaded lanes (YIN) / planes (XIN)
-    // lie in 1..Ny-2 / 1..Nx-2 and outside every slab, so those tests are constant
-    const bool jin = YIN || (j >= 0 && j < Ny);
-    const bool jey = YIN || (jin && j < Ny - 1);   // Ey and Hz exist in this column
-    const bool own = jin && lane >= T && lane <= 63 - T;
-    // psi slots exist only where the single-step kernels update psi: Hz slabs for
-    // j <= Ny-2 (H nodes), Ex slabs for 1 <= j <= Ny-2 (E nodes)
-    const int sHzy = (PY && jey) ? strip_pos(a.s0[1], j) : -1;
-    const int sExy = (PY && jin && j >= 1 && j <= Ny - 2) ? strip_pos(a.s0[2], j) : -1;
-    float yhb = 0.f, yhc = 0.f, yhk = 1.f, yeb = 0.f, yec = 0.f, yek = 1.f;
-    if (sHzy >= 0) { yhb = a.yh.b[j]; yhc = a.yh.c[j]; yhk = a.yh.k[j]; }
-    if (sExy >= 0) { yeb = a.ye.b[j]; yec = a.ye.c[j]; yek = a.ye.k[j]; }
-    const bool lossy_x = a.dax != nullptr, lossy_y = a.day != nullptr;
-    const int ps = X0 - T > 0 ? X0 - T : 0;
-    const int pe = X1 + T;
-    // x profiles of the wavefront's planes, plane ps + lane in lane `lane`, read per level
-    // with v_readlane (no dependent scalar loads inside the level chain)
-    const bool xpre = PX && pe - ps <= 64;
-    float xhb = 0.f, xhc = 0.f, xhk = 1.f, xeb = 0.f, xec = 0.f, xek = 1.f;
-    if (xpre) {
-        const int pl = ps + lane;
-        if (pl < Nx - 1) { xhb = a.xh.b[pl]; xhc = a.xh.c[pl]; xhk = a.xh.k[pl]; }
-        if (pl < Nx) { xeb = a.xe.b[pl]; xec = a.xe.c[pl]; xek = a.xe.k[pl]; }
-    }
-    auto plane_val = [&](float reg, const float *arr, int q) {
-        return xpre ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps)) : arr[q];
-    };
-
-    // slot L holds the output of level L-1 (slot 0: loaded plane); C = this iteration,
-    // P = previous iteration
-    float exC[T + 1], eyC[T + 1], hzC[T + 1], exP[T + 1], eyP[T + 1], hzP[T + 1];
-    float hxC[T + 1], hyC[T + 1], eyyC[T + 1], exxC[T + 1];    // psi Hzx Hzy Exy Eyx
-    float hxP[T + 1], hyP[T + 1], eyyP[T + 1], exxP[T + 1];
-    float cA[T + 1], cY[T + 1], dA[T + 1], dY[T + 1];         // coefficients, by plane age
-#pragma unroll
-    for (int L = 0; L <= T; ++L) {
-        exC[L] = eyC[L] = hzC[L] = exP[L] = eyP[L] = hzP[L] = 0.f;
-        hxC[L] = hyC[L] = eyyC[L] = exxC[L] = 0.f;
-        hxP[L] = hyP[L] = eyyP[L] = exxP[L] = 0.f;
-        cA[L] = cY[L] = dA[L] = dY[L] = 0.f;
-    }
-    TMPlane nxt = tm_load_plane<MODE>(a, ps, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
-    for (int p = ps; p < pe; ++p) {
-        // ---- slot 0: plane p of step n (loaded one iteration ahead)
-        exC[0] = nxt.ex; eyC[0] = nxt.ey; hzC[0] = nxt.hz;
-        hxC[0] = nxt.ph; hyC[0] = nxt.py; eyyC[0] = nxt.qy; exxC[0] = nxt.qx;
-        cA[0] = nxt.ca; cY[0] = nxt.cy; dA[0] = nxt.da; dY[0] = nxt.dy;
-        nxt = tm_load_plane<MODE>(a, p + 1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
-        // ---- levels
-#pragma unroll
-        for (int s = 0; s < T; ++s) {
-            const int q = p - 1 - s;
-            if (q >= ps && q < Nx) {
-                const float ex_o = exP[s], ey_o = eyP[s], hz_o = hzP[s];
-                const float ex_jp = lane_next(ex_o);                // Ex(q, j+1)
-                float psx = hxP[s], psy = hyP[s];
-                float hz = 0.f;
-                if ((XIN || q < Nx - 1) && jey) {                   // Hz(q, j)
-                    const float dEyx = eyC[s] - ey_o;
-                    const float dExy = ex_jp - ex_o;
-                    float tx, ty;
-                    if (PX && strip_pos(a.s0[0], q) >= 0) {
-                        psx = plane_val(xhb, a.xh.b, q) * psx + plane_val(xhc, a.xh.c, q) * dEyx;
-                        tx = dEyx / plane_val(xhk, a.xh.k, q) + psx;
-                    } else {
-                        tx = dEyx;
-                    }
-                    if (sHzy >= 0) {
-                        psy = yhb * psy + yhc * dExy;
-                        ty = dExy / yhk + psy;
-                    } else {
-                        ty = dExy;
-                    }
-                    hz = hz_o - a.ch * (tx - ty);
-                }
-                const float hz_jm = lane_prev(hz);                  // Hz(q, j-1)
-                float qy = eyyP[s];
-                float exn = 0.f;
-                if ((XIN || q < Nx - 1) && jin) {                   // Ex(q, j)
-                    float v = ex_o;
-                    if (lossy_x) v = v * dA[s + 1];
-                    if (YIN || (j >= 1 && j <= Ny - 2)) {
-                        const float d = hz - hz_jm;
-                        float t;
-                        if (sExy >= 0) {
-                            qy = yeb * qy + yec * d;
-                            t = d / yek + qy;
-                        } else {
-                            t = d;
-                        }
-                        v = v + cA[s + 1] * t;
-                    }
-                    if (!YIN && ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1))) v = 0.0f;
-                    exn = v;
-                }
-                float qx = exxP[s];
-                float eyn = 0.f;
-                if (jey) {                                          // Ey(q, j)
-                    float v = ey_o;
-                    if (lossy_y) v = v * dY[s + 1];
-                    if (XIN || (q >= 1 && q <= Nx - 2)) {
-                        const float d = hz - hzP[s + 1];            // Hz(q) - Hz(q-1)
-                        float t;
-                        if (PX && strip_pos(a.s0[3], q) >= 0) {
-                            qx = plane_val(xeb, a.xe.b, q) * qx + plane_val(xec, a.xe.c, q) * d;
-                            t = d / plane_val(xek, a.xe.k, q) + qx;
-                        } else {
-                            t = d;
-                        }
-                        v = v + (-cY[s + 1]) * t;
-                    }
-                    if (EXC) v = apply_source(v, LF_EY, q, j, x, st[s], fp64);
-                    if (EXC) v = apply_drive(v, LF_EY, q, j, x, st[s]);
-                    if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
-                    eyn = v;
-                }
-                exC[s + 1] = exn; eyC[s + 1] = eyn; hzC[s + 1] = hz;
-                hxC[s + 1] = psx; hyC[s + 1] = psy; eyyC[s + 1] = qy; exxC[s + 1] = qx;
-            }
-            exP[s] = exC[s]; eyP[s] = eyC[s]; hzP[s] = hzC[s];
-            hxP[s] = hxC[s]; hyP[s] = hyC[s]; eyyP[s] = eyyC[s]; exxP[s] = exxC[s];
-        }
-        // ---- store plane p-T of step n+T
-        {
-            const int q = p - T;
-            if (q >= X0 && q < X1 && own) {
-                const long long c = (long long)q * P + j;
-                if (XIN || q < Nx - 1) {
-                    a.Ex1[c] = exC[T];
-                    if (jey) a.Hz1[c] = hzC[T];
-                }
-                if (jey) a.Ey1[c] = eyC[T];
-                if (PX) {
-                    const int sx0 = q < Nx - 1 ? strip_pos(a.s1[0], q) : -1;
-                    if (sx0 >= 0 && jey) a.s1[0].psi[(long long)sx0 * P + j] = hxC[T];
-                    const int sx3 = (q >= 1 && q <= Nx - 2) ? strip_pos(a.s1[3], q) : -1;
-                    if (sx3 >= 0 && jey) a.s1[3].psi[(long long)sx3 * P + j] = exxC[T];
-                }
-                if (PY) {
-                    if (sHzy >= 0 && (XIN || q < Nx - 1))
-                        a.s1[1].psi[(long long)q * a.s1[1].len + sHzy] = hyC[T];
-                    if (sExy >= 0 && (XIN || q < Nx - 1))
-                        a.s1[2].psi[(long long)q * a.s1[2].len + sExy] = eyyC[T];
-                }
-            }
-            exP[T] = exC[T]; eyP[T] = eyC[T]; hzP[T] = hzC[T];
-        }
-#pragma unroll
-        for (int L = T; L >= 1; --L) {
-            cA[L] = cA[L - 1]; cY[L] = cY[L - 1]; dA[L] = dA[L - 1]; dY[L] = dY[L - 1];
-        }
-    }
-}
-
-// slab wavefronts are few long serial chains that run beside the interior launch: they
-// take issue priority on their SIMD so that they finish with it.  The wave index is read
-// into a scalar register, so every x/plane test is a scalar branch.
-// EXC = false: no source or drive plane lies within the launch's marches, so the body
-// carries no excitation loads (whose waits would also drain the plane prefetch).
-template <int T, int MODE, bool EXC = true>
-__global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
-    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
-    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    tm_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
-}
-
-// ---------------------------------------------------------------------------- temporal blocking (2D TE)
-// Same pipeline as the TM pass with the roles of E and H exchanged: level s advances
-// plane q = p-1-s from step n+s to n+s+1: Hx^{n+s+1/2}(q) needs Ez(q, j+1) (lane above),
-// Hy^{n+s+1/2}(q) needs Ez(q+1) (this iteration's output of level s-1), and
-// Ez^{n+s+1}(q) needs Hy(q-1) (own previous output) and Hx(q, j-1) (lane below).  Per
-// cell and step the arithmetic is that of te_h_kernel + te_e_kernel
-// (boundaries.py:189-215).  psi slabs: Hxy (y), Hyx (x), Ezx (x), Ezy (y).
-struct TBTEArgs {
-    Grid g;
-    const float *Ez0, *Hx0, *Hy0;
-    float *Ez1, *Hx1, *Hy1;
-    const float *ceE, *ceH;
-    float ch;
-    Prof xe, xh, ye, yh;
-    Strip s0[4], s1[4];       // Hxy Hyx Ezx Ezy: read / write buffers
-    int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
-    int lx;
-    int nseg;
-    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4];
-    StepArgs st[kMaxTB];
-};
-
-struct TEPlane {
-    float ez, hx, hy, ce, cH, pxy, pyx, pzx, pzy;
-};
-
-template <int MODE>
-__device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int j, bool jin, bool jhx,
-                                                 int sHxy, int sEzy) {
-    constexpr bool PX = MODE & 2, PY = MODE & 1;
-    TEPlane v{};
-    const int Nx = a.g.nx, P = a.g.P;
-    if (p < Nx) {
-        const long long c = (long long)p * P + j;
-        const bool xin = !PX || p < Nx - 1;            // Hy exists on this plane
-        if (jin) {
-            v.ez = a.Ez0[c];
-            v.ce = a.ceE[c];
-            v.cH = a.ceH[c];
-            if (xin) v.hy = a.Hy0[c];
-        }
-        if (jhx) v.hx = a.Hx0[c];
-        if (PX) {
-            const int sx1 = xin ? strip_pos(a.s0[1], p) : -1;           // H nodes p <= Nx-2
-            if (sx1 >= 0 && jin) v.pyx = a.s0[1].psi[(long long)sx1 * P + j];
-            const int sx2 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[2], p) : -1;
-            if (sx2 >= 0 && jin) v.pzx = a.s0[2].psi[(long long)sx2 * P + j];
-        }
-        if (PY) {
-            if (sHxy >= 0) v.pxy = a.s0[0].psi[(long long)p * a.s0[0].len + sHxy];
-            if (sEzy >= 0 && (!PX || (p >= 1 && p <= Nx - 2)))
-                v.pzy = a.s0[3].psi[(long long)p * a.s0[3].len + sEzy];
-        }
-    }
-    return v;
-}
-
-template <int T, int MODE, bool EXC = true>
-__device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, const StepArgs *st,
-                                           int wv, int fp64) {
-    constexpr bool PX = MODE & 2, PY = MODE & 1;
-    constexpr bool XIN = !PX, YIN = !PY;
-    const int lane = threadIdx.x & 63;
-    if (wv >= a.seg_end[a.nseg - 1]) return;
-    int g = 0;
-    while (g < a.nseg - 1 && wv >= a.seg_end[g]) ++g;
-    const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
-    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
-    const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
-    const int X0 = wx * a.lx;
-    if (X0 >= Nx) return;
-    const int X1 = min(X0 + a.lx, Nx);
-    const int j = wy * (64 - 2 * T) - T + lane;
-    const bool jin = YIN || (j >= 0 && j < Ny);
-    const bool jhx = YIN || (jin && j < Ny - 1);   // Hx exists in this column
-    const bool jcurl = YIN || (j >= 1 && j <= Ny - 2);
-    const bool own = jin && lane >= T && lane <= 63 - T;
-    const int sHxy = (PY && jhx) ? strip_pos(a.s0[0], j) : -1;
-    const int sEzy = (PY && jin && j >= 1 && j <= Ny - 2) ? strip_pos(a.s0[3], j) : -1;
-    float yhb = 0.f, yhc = 0.f, yhk = 1.f, yeb = 0.f, yec = 0.f, yek = 1.f;
-    if (sHxy >= 0) { yhb = a.yh.b[j]; yhc = a.yh.c[j]; yhk = a.yh.k[j]; }
-    if (sEzy >= 0) { yeb = a.ye.b[j]; yec = a.ye.c[j]; yek = a.ye.k[j]; }
-    const int ps = X0 - T > 0 ? X0 - T : 0;
-    const int pe = X1 + T;
-    const bool xpre = PX && pe - ps <= 64;
-    float xhb = 0.f, xhc = 0.f, xhk = 1.f, xeb = 0.f, xec = 0.f, xek = 1.f;
-    if (xpre) {
-        const int pl = ps + lane;
-        if (pl < Nx - 1) { xhb = a.xh.b[pl]; xhc = a.xh.c[pl]; xhk = a.xh.k[pl]; }
-        if (pl < Nx) { xeb = a.xe.b[pl]; xec = a.xe.c[pl]; xek = a.xe.k[pl]; }
-    }
-    auto plane_val = [&](float reg, const float *arr, int q) {
-        return xpre ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps)) : arr[q];
-    };
-
-    float ezC[T + 1], hxC[T + 1], hyC[T + 1], ezP[T + 1], hxP[T + 1], hyP[T + 1];
-    float axyC[T + 1], ayxC[T + 1], azxC[T + 1], azyC[T + 1];   // psi Hxy Hyx Ezx Ezy
-    float axyP[T + 1], ayxP[T + 1], azxP[T + 1], azyP[T + 1];
-    float cE[T + 1], cH[T + 1];
-#pragma unroll
-    for (int L = 0; L <= T; ++L) {
-        ezC[L] = hxC[L] = hyC[L] = ezP[L] = hxP[L] = hyP[L] = 0.f;
-        axyC[L] = ayxC[L] = azxC[L] = azyC[L] = 0.f;
-        axyP[L] = ayxP[L] = azxP[L] = azyP[L] = 0.f;
-        cE[L] = cH[L] = 0.f;
-    }
-    TEPlane nxt = te_load_plane<MODE>(a, ps, j, jin, jhx, sHxy, sEzy);
-    for (int p = ps; p < pe; ++p) {
-        ezC[0] = nxt.ez; hxC[0] = nxt.hx; hyC[0] = nxt.hy;
-        axyC[0] = nxt.pxy; ayxC[0] = nxt.pyx; azxC[0] = nxt.pzx; azyC[0] = nxt.pzy;
-        cE[0] = nxt.ce; cH[0] = nxt.cH;
-        nxt = te_load_plane<MODE>(a, p + 1, j, jin, jhx, sHxy, sEzy);
-#pragma unroll
-        for (int s = 0; s < T; ++s) {
-            const int q = p - 1 - s;
-            if (q >= ps && q < Nx) {
-                const float ez_o = ezP[s];
-                const float ez_jp = lane_next(ez_o);                // Ez(q, j+1)
-                float qxy = axyP[s], qyx = ayxP[s];
-                float hx = 0.f, hy = 0.f;
-                if (jhx) {                                          // Hx(q, j)
-                    const float d = ez_jp - ez_o;
-                    float t;
-                    if (sHxy >= 0) {
-                        qxy = yhb * qxy + yhc * d;
-                        t = d / yhk + qxy;
-                    } else {
-                        t = d;
-                    }
-                    hx = hxP[s] - a.ch * t;
-                }
-                if ((XIN || q < Nx - 1) && jin) {                   // Hy(q, j)
-                    const float d = ezC[s] - ez_o;
-                    float t;
-                    if (PX && strip_pos(a.s0[1], q) >= 0) {
-                        qyx = plane_val(xhb, a.xh.b, q) * qyx + plane_val(xhc, a.xh.c, q) * d;
-                        t = d / plane_val(xhk, a.xh.k, q) + qyx;
-                    } else {
-                        t = d;
-                    }
-                    hy = hyP[s] + a.ch * t;
-                }
-                const float hx_jm = lane_prev(hx);                  // Hx(q, j-1)
-                float qzx = azxP[s], qzy = azyP[s];
-                float ezn = 0.f;
-                if (jin) {                                          // Ez(q, j)
-                    float v = ez_o;
-                    if ((XIN || (q >= 1 && q <= Nx - 2)) && jcurl) {
-                        const float dHy = hy - hyP[s + 1];          // Hy(q) - Hy(q-1)
-                        const float dHx = hx - hx_jm;
-                        float t;
-                        if (PX && strip_pos(a.s0[2], q) >= 0) {
-                            qzx = plane_val(xeb, a.xe.b, q) * qzx + plane_val(xec, a.xe.c, q) * dHy;
-                            t = dHy / plane_val(xek, a.xe.k, q) + qzx;
-                        } else {
-                            t = dHy;
-                        }
-                        if (sEzy >= 0) {
-                            qzy = yeb * qzy + yec * dHx;
-                            t = (t - dHx / yek) - qzy;
-                        } else {
-                            t = t - dHx;
-                        }
-                        v = cE[s + 1] * v + cH[s + 1] * t;
-                    }
-                    if (EXC) v = apply_source(v, LF_EZ, q, j, x, st[s], fp64);
-                    if (EXC) v = apply_drive(v, LF_EZ, q, j, x, st[s]);
-                    if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
-                    if (!YIN && ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1))) v = 0.0f;
-                    ezn = v;
-                }
-                ezC[s + 1] = ezn; hxC[s + 1] = hx; hyC[s + 1] = hy;
-                axyC[s + 1] = qxy; ayxC[s + 1] = qyx; azxC[s + 1] = qzx; azyC[s + 1] = qzy;
-            }
-            ezP[s] = ezC[s]; hxP[s] = hxC[s]; hyP[s] = hyC[s];
-            axyP[s] = axyC[s]; ayxP[s] = ayxC[s]; azxP[s] = azxC[s]; azyP[s] = azyC[s];
-        }
-        {
-            const int q = p - T;
-            if (q >= X0 && q < X1 && own) {
-                const long long c = (long long)q * P + j;
-                a.Ez1[c] = ezC[T];
-                if (jhx) a.Hx1[c] = hxC[T];
-                if (XIN || q < Nx - 1) a.Hy1[c] = hyC[T];
-                if (PX) {
-                    const int sx1 = q < Nx - 1 ? strip_pos(a.s1[1], q) : -1;
-                    if (sx1 >= 0) a.s1[1].psi[(long long)sx1 * P + j] = ayxC[T];
-                    const int sx2 = (q >= 1 && q <= Nx - 2) ? strip_pos(a.s1[2], q) : -1;
-                    if (sx2 >= 0) a.s1[2].psi[(long long)sx2 * P + j] = azxC[T];
-                }
-                if (PY) {
-                    if (sHxy >= 0) a.s1[0].psi[(long long)q * a.s1[0].len + sHxy] = axyC[T];
-                    if (sEzy >= 0 && (XIN || (q >= 1 && q <= Nx - 2)))
-                        a.s1[3].psi[(long long)q * a.s1[3].len + sEzy] = azyC[T];
-                }
-            }
-            ezP[T] = ezC[T]; hxP[T] = hxC[T]; hyP[T] = hyC[T];
-        }
-#pragma unroll
-        for (int L = T; L >= 1; --L) { cE[L] = cE[L - 1]; cH[L] = cH[L - 1]; }
-    }
-}
-
-template <int T, int MODE, bool EXC = true>
-__global__ void __launch_bounds__(256) te_tb_kernel(TBTEArgs a, Excite x, int fp64) {
-    if (MODE != 0) __builtin_amdgcn_s_setprio(2);
-    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    te_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
 }
 
 // ---------------------------------------------------------------------------- batched passes
@@ -1641,8 +1175,8 @@ struct lf_ctx {
     int nx = 0, ny = 0, nz = 0;          // logical dims (2D: nz = 1)
     Grid g{};                            // padded grid; 2D stored as (nx, 1, ny)
     hipStream_t stream = nullptr;
-    hipStream_t side[2] = {};            // side streams (default priority) for slab launches
-    hipEvent_t ev_fork = nullptr, ev_join[2] = {};
+    hipStream_t side[3] = {};            // side streams (default priority) for slab launches
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {};
     float *arr[LF_NARRAYS] = {};
     float *alt[LF_NARRAYS] = {};         // second buffer of ping-pong fields (fused path)
     int variant = 1;                     // 0: single-pass kernels where available, 1: H then E
@@ -1662,6 +1196,8 @@ struct lf_ctx {
     int ty = 4;                          // owned y rows per wavefront of the 3D single pass
     int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
     int tb_lx = 64;                      // x planes per wavefront chunk of the blocked pass
+    int tb_mon = 1;                      // 2D: depth of the passes that sum CW-window monitors
+    float *tbm_stage = nullptr;          // flux-plane staging of those passes
     double last_tb_ms = 0;               // blocked launches of the last lf_run: time, count
     int last_tb_launches = 0;
     float ch = 0.f;
@@ -1862,8 +1398,10 @@ int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out) {
     if (nx < 3 || ny < 3 || (kind == LF_YEE3D && nz < 3)) return fail(-1, "grid too small");
     lf_ctx *c = new lf_ctx();
     c->kind = kind; c->mode = mode;
-    if (kind == LF_TM2D) c->tb = 6;              // blocked passes over monitor-free steps
+    if (kind != LF_YEE3D) c->tb = 6;             // blocked passes over monitor-free steps
     if (kind == LF_YEE3D) c->tb = 3;
+    if (kind == LF_TM2D) c->tb_mon = 3;          // monitored passes (CW-window steps)
+    if (kind == LF_TE2D) c->tb_mon = 4;
     c->nx = nx; c->ny = ny; c->nz = (kind == LF_YEE3D) ? nz : 1;
     if (kind == LF_YEE3D) c->g = Grid{nx, ny, nz, round_up(nz, 64)};
     else c->g = Grid{nx, 1, ny, round_up(ny, 64)};
@@ -1906,6 +1444,7 @@ int lf_destroy(lf_ctx *c) {
     for (auto p : c->ex_bufs) (void)hipFree(p);
     if (c->tb3_dbox) (void)hipFree(c->tb3_dbox);
     if (c->tb3_ticket) (void)hipFree(c->tb3_ticket);
+    if (c->tbm_stage) (void)hipFree(c->tbm_stage);
     for (int m = 0; m < c->nmon; ++m) {
         if (c->mon[m].buf) (void)hipFree(c->mon[m].buf);
         if (c->mon[m].type == 3)
@@ -2181,14 +1720,17 @@ bool tm_x_slab(int q) { return x_slab_2d(LF_TM2D, q); }
 // slab-free rectangle, whose loaded planes and lanes lie in 1..N-2 and outside every
 // slab; class 1 (MODE 1): the rows below / above it (y slabs only); class 2 (MODE 3):
 // the x chunks left / right of it over all rows, or every wavefront of a grid without a
-// slab-free rectangle.
+// slab-free rectangle; class 3 (MODE 0 with excitation code): the chunks of the
+// slab-free rectangle whose marches meet a source or drive plane (the stitch drive
+// covers the whole window overlap, a quarter of a window), so that class 0 runs the
+// body without excitation loads.
 struct TBClass {
     int mode, lx, nseg;
     int exc;                  // 0: no source / drive plane within the class's marches
     int wx0[4], wy0[4], nwy[4], end[4];
 };
 struct TBPlan {
-    TBClass cls[3];
+    TBClass cls[4];
     bool interior;
 };
 
@@ -2229,8 +1771,8 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
             if (cand <= want && lx % cand == 0 && lx / cand >= 4) return cand;
         return 1;
     };
-    pl.cls[0].mode = 0; pl.cls[1].mode = 1; pl.cls[2].mode = 3;
-    pl.cls[0].exc = pl.cls[1].exc = pl.cls[2].exc = 1;
+    pl.cls[0].mode = 0; pl.cls[1].mode = 1; pl.cls[2].mode = 3; pl.cls[3].mode = 0;
+    pl.cls[0].exc = pl.cls[1].exc = pl.cls[2].exc = pl.cls[3].exc = 1;
     if (!pl.interior) {
         const int k = div(4);
         pl.cls[2].lx = lx / k;
@@ -2239,14 +1781,35 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
     }
     const int ni = iy1 - iy0 + 1;
     pl.cls[0].lx = lx;
-    add(pl.cls[0], ix0, ix1 - ix0 + 1, iy0, ni);
-    {   // does a source or drive plane lie within the slab-free chunks' marches?
+    pl.cls[3].lx = lx;
+    const int ke = div(4);                               // class 3: chunks of lx/ke
+    {   // slab-free chunks whose marches meet a source or drive plane go to class 3
         const int comp = c->kind == LF_TM2D ? LF_EY : LF_EZ;
-        const int m0 = ix0 * lx - T, m1 = std::min((ix1 + 1) * lx, Nx) + T;
-        bool e = false;
-        if (c->ex.amp && c->ex.src_comp == comp) e = e || (c->ex.src_ix < m1 && c->ex.src_ix + 2 > m0);
-        if (c->ex.wtap && c->ex.drv[comp]) e = e || (0 < m1 && c->ex.drv_n > m0);
-        pl.cls[0].exc = e ? 1 : 0;
+        auto meets = [&](int wx) {
+            const int m0 = wx * lx - T, m1 = std::min((wx + 1) * lx, Nx) + T;
+            bool e = false;
+            if (c->ex.amp && c->ex.src_comp == comp) e = e || (c->ex.src_ix < m1 && c->ex.src_ix + 2 > m0);
+            if (c->ex.wtap && c->ex.drv[comp]) e = e || (0 < m1 && c->ex.drv_n > m0);
+            return e;
+        };
+        int runs[2] = {0, 0};                    // runs of chunks per class
+        for (int wx = ix0; wx <= ix1; ++wx)
+            if (wx == ix0 || meets(wx) != meets(wx - 1)) ++runs[meets(wx) ? 1 : 0];
+        if (runs[0] <= 4 && runs[1] <= 4) {
+            // class 3 marches finer chunks: its waves carry the excitation loads and
+            // arithmetic, and few long marches would leave most SIMDs with one wave
+            pl.cls[3].lx = lx / ke;
+            for (int wx = ix0; wx <= ix1;) {
+                int w1 = wx;
+                while (w1 + 1 <= ix1 && meets(w1 + 1) == meets(wx)) ++w1;
+                if (meets(wx)) add(pl.cls[3], wx * ke, (w1 - wx + 1) * ke, iy0, ni);
+                else add(pl.cls[0], wx, w1 - wx + 1, iy0, ni);
+                wx = w1 + 1;
+            }
+            pl.cls[0].exc = 0;
+        } else {                                 // scattered planes: one class with excitation code
+            add(pl.cls[0], ix0, ix1 - ix0 + 1, iy0, ni);
+        }
     }
     const int k = div(4);                                 // y-slab rows: chunks of lx/k
     pl.cls[1].lx = lx / k;
@@ -2271,11 +1834,35 @@ void set_class(A &q, const TBClass &k) {
 }
 
 int ensure_side_streams(lf_ctx *c) {
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < 3; ++u)
         if (!c->side[u]) LF_HIP(hipStreamCreate(&c->side[u]));
     if (!c->ev_fork) {
         LF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-        for (int u = 0; u < 2; ++u) LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
+        for (int u = 0; u < 3; ++u) LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
+    }
+    return 0;
+}
+
+// Fork the classes of one 2D pass over the context stream (class 0) and the side streams
+// (classes 1, 2, 3), concurrently: disjoint output cells, step-n inputs.  `one` launches
+// a class on a stream.
+template <typename F>
+int fork_tb_classes(lf_ctx *c, const TBPlan &pl, F one) {
+    if (!pl.interior) {
+        one(pl.cls[2], c->stream);
+        return 0;
+    }
+    if (int e = ensure_side_streams(c)) return e;
+    const int nside = pl.cls[3].nseg > 0 ? 3 : 2;
+    LF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    for (int u = 0; u < nside; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
+    one(pl.cls[1], c->side[0]);
+    one(pl.cls[2], c->side[1]);
+    if (nside == 3) one(pl.cls[3], c->side[2]);
+    one(pl.cls[0], c->stream);
+    for (int u = 0; u < nside; ++u) {
+        LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
+        LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
     }
     return 0;
 }
@@ -2313,21 +1900,7 @@ int launch_tb_pass(lf_ctx *c, A &f, const TBPlan &pl, int T, int fp64) {
             default: launch_tb_mode<A, 8>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
         }
     };
-    if (!pl.interior) {
-        one(pl.cls[2], c->stream);
-        return 0;
-    }
-    if (int e = ensure_side_streams(c)) return e;
-    LF_HIP(hipEventRecord(c->ev_fork, c->stream));
-    for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
-    one(pl.cls[1], c->side[0]);
-    one(pl.cls[2], c->side[1]);
-    one(pl.cls[0], c->stream);
-    for (int u = 0; u < 2; ++u) {
-        LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
-        LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
-    }
-    return 0;
+    return fork_tb_classes(c, pl, one);
 }
 
 int ensure_psi_alt_2d(lf_ctx *c) {
@@ -2411,7 +1984,195 @@ int tb_chunk_2d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
     return 0;
 }
 
+// ---- monitored 2D passes: the CW-window monitors summed inside the pass
+struct TBMonPlan {
+    int mon = 0;                      // kernel variant: 9 (one phasor) or 14 (two + intensity)
+    TBMon m{};
+    int nflux = 0;
+    int fid[kTBFlux] = {};            // flux monitor ids in staging order
+};
+
+// Can every monitor set on steps[0..T) be summed inside a 2D pass?  Phasors of 2D
+// sources (TM: Ex, Ey, node Ey, node Ex; TE: Ez), one intensity sum of a 2D form and up
+// to kTBFlux flux planes qualify; detector boxes do not.
+bool tb_mon_plan(const lf_ctx *c, const lf_step *steps, int T, TBMonPlan &pl) {
+    unsigned any = 0u;
+    for (int u = 0; u < T; ++u) any |= steps[u].mask;
+    if (any == 0u) return false;
+    int did[2] = {-1, -1}, nd = 0, sid = -1;
+    pl = TBMonPlan{};
+    for (int id = 0; id < c->nmon; ++id) {
+        if (!((any >> id) & 1u)) continue;
+        const Monitor &mo = c->mon[id];
+        if (mo.type == 2) {
+            const int src = mo.dft.src;
+            const bool ok = c->kind == LF_TM2D
+                                ? (src == LF_SRC_EX || src == LF_SRC_EY || src == LF_SRC_EY_NODE ||
+                                   src == LF_SRC_EX_NODE)
+                                : src == LF_SRC_EZ;
+            if (!ok || mo.dft.kplane >= 0 || mo.dft.n1 != 1 || nd == 2) return false;
+            did[nd++] = id;
+        } else if (mo.type == 1) {
+            const bool ok = c->kind == LF_TM2D ? (mo.form == LF_SQ_TM || mo.form == LF_SQ_TM_EX2)
+                                               : mo.form == LF_SQ_TE;
+            if (!ok || sid >= 0) return false;
+            sid = id;
+        } else if (mo.type == 3) {
+            if (pl.nflux == kTBFlux || mo.flux.npair != 1 || mo.flux.n1[0] != 1) return false;
+            pl.fid[pl.nflux++] = id;
+        } else {
+            return false;
+        }
+    }
+    pl.mon = (nd <= 1 && sid < 0) ? 9 : 14;
+    TBMon &m = pl.m;
+    for (int k = 0; k < 2; ++k) {
+        if (k >= nd) { m.dn2[k] = 0; continue; }
+        const DftArgs &d = c->mon[did[k]].dft;
+        m.dsrc[k] = d.src; m.dacc[k] = d.acc;
+        m.dx0[k] = d.x0; m.dx1[k] = d.x0 + d.n0; m.dn2[k] = d.n2;
+        m.dbuf[k] = d.buf;
+    }
+    if (sid >= 0) {
+        const Monitor &mo = c->mon[sid];
+        m.sform = mo.form; m.sn2 = mo.n2; m.sbuf = (float *)mo.buf;
+    }
+    m.nflux = pl.nflux;
+    for (int f = 0; f < pl.nflux; ++f) {
+        const FluxArgs &fa = c->mon[pl.fid[f]].flux;
+        m.fil[f] = fa.il; m.fn2[f] = fa.n2[0];
+        m.fst[f] = c->tbm_stage ? c->tbm_stage + (size_t)f * kMaxTB * 2 * c->g.P : nullptr;
+    }
+    for (int u = 0; u < T; ++u) {
+        const lf_step &t = steps[u];
+        unsigned b = 0u;
+        for (int k = 0; k < nd; ++k) b |= ((t.mask >> did[k]) & 1u) << k;
+        if (sid >= 0) b |= ((t.mask >> sid) & 1u) << 2;
+        for (int f = 0; f < pl.nflux; ++f) b |= ((t.mask >> pl.fid[f]) & 1u) << (3 + f);
+        m.lm[u] = b;
+        m.pr[u] = t.pr; m.pi[u] = t.pi;
+        for (int k = 0; k < 2; ++k) {
+            const bool q = k < nd && c->mon[did[k]].dft.phasor == LF_PH_Q;
+            m.fr[k][u] = q ? t.qr32 : t.pr32;
+            m.fi[k][u] = q ? t.qi32 : t.pi32;
+        }
+    }
+    return true;
+}
+
+// Flux planes staged by a monitored pass, added in step order with the arithmetic of
+// flux_kernel (2D: one pair per plane, one row of cells).
+struct TBFluxPost {
+    int nflux, T, wide;
+    int n2[kTBFlux];
+    unsigned steps[kTBFlux];          // bit s: the plane accumulates on level s
+    const float *st[kTBFlux];
+    void *aE[kTBFlux], *aH[kTBFlux];
+    double pr[kMaxTB], pi[kMaxTB], hr[kMaxTB], hi[kMaxTB];
+    float pr32[kMaxTB], pi32[kMaxTB], hr32[kMaxTB], hi32[kMaxTB];
+};
+
+__global__ void __launch_bounds__(256) tb_flux_kernel(TBFluxPost f) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    if (k >= f.nflux || j >= f.n2[k]) return;
+    const int n2 = f.n2[k];
+    if (f.wide) {
+        double2 *aE = (double2 *)f.aE[k];
+        double2 *aH = (double2 *)f.aH[k];
+        double2 u = aE[j], w = aH[j];
+        for (int s = 0; s < f.T; ++s) {
+            if (!((f.steps[k] >> s) & 1u)) continue;
+            const float e = f.st[k][(2 * s) * n2 + j];
+            const float h = f.st[k][(2 * s + 1) * n2 + j];
+            u.x = u.x + (double)e * f.pr[s]; u.y = u.y + (double)e * f.pi[s];
+            w.x = w.x + (double)h * f.hr[s]; w.y = w.y + (double)h * f.hi[s];
+        }
+        aE[j] = u; aH[j] = w;
+    } else {
+        float2 *aE = (float2 *)f.aE[k];
+        float2 *aH = (float2 *)f.aH[k];
+        float2 u = aE[j], w = aH[j];
+        for (int s = 0; s < f.T; ++s) {
+            if (!((f.steps[k] >> s) & 1u)) continue;
+            const float e = f.st[k][(2 * s) * n2 + j];
+            const float h = f.st[k][(2 * s + 1) * n2 + j];
+            u.x = u.x + e * f.pr32[s]; u.y = u.y + e * f.pi32[s];
+            w.x = w.x + h * f.hr32[s]; w.y = w.y + h * f.hi32[s];
+        }
+        aE[j] = u; aH[j] = w;
+    }
+}
+
+// one monitored 2D pass over T steps (T in 2..4): reads arr[], writes alt[], sums the
+// planned monitors, then swaps
+int tb_chunk_2d_mon(lf_ctx *c, const lf_step *steps, int T, TBMonPlan &mp, int fp64) {
+    if (T < 2 || T > 4) return fail(-1, "monitored pass depth must be 2, 3 or 4");
+    if (int r = ensure_tb_buffers(c)) return r;
+    if (mp.nflux > 0 && !c->tbm_stage) {
+        const size_t n = (size_t)kTBFlux * kMaxTB * 2 * c->g.P;
+        LF_HIP(hipMalloc(&c->tbm_stage, n * sizeof(float)));
+        LF_HIP(hipMemsetAsync(c->tbm_stage, 0, n * sizeof(float), c->stream));
+        for (int f = 0; f < mp.nflux; ++f) mp.m.fst[f] = c->tbm_stage + (size_t)f * kMaxTB * 2 * c->g.P;
+    }
+    const TBPlan pl = tb_plan_2d(c, T, c->tb_lx);
+    auto pass = [&](auto &f, auto launch) -> int {
+        for (int s = 0; s < T; ++s) f.st[s] = step_args(steps[s]);
+        auto one = [&](const TBClass &k, hipStream_t st) {
+            if (k.nseg == 0) return;
+            auto q = f;
+            set_class(q, k);
+            const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
+            launch(T, k.mode, k.exc != 0, mp.mon, q, c->ex, mp.m, fp64, st, nb);
+        };
+        return fork_tb_classes(c, pl, one);
+    };
+    if (c->kind == LF_TM2D) {
+        TBTMArgs f = tm_tb_args(c);
+        if (int e = pass(f, launch_tm_tb_mon)) return e;
+    } else {
+        TBTEArgs f = te_tb_args(c);
+        if (int e = pass(f, launch_te_tb_mon)) return e;
+    }
+    LF_HIP(hipGetLastError());
+    if (mp.nflux > 0) {
+        TBFluxPost fp{};
+        fp.nflux = mp.nflux; fp.T = T; fp.wide = c->mon[mp.fid[0]].flux.wide;
+        int n2max = 0;
+        for (int f = 0; f < mp.nflux; ++f) {
+            const FluxArgs &fa = c->mon[mp.fid[f]].flux;
+            fp.n2[f] = fa.n2[0];
+            n2max = std::max(n2max, fa.n2[0]);
+            fp.st[f] = mp.m.fst[f];
+            fp.aE[f] = fa.aE[0]; fp.aH[f] = fa.aH[0];
+            unsigned b = 0u;
+            for (int s = 0; s < T; ++s) b |= ((mp.m.lm[s] >> (3 + f)) & 1u) << s;
+            fp.steps[f] = b;
+        }
+        for (int s = 0; s < T; ++s) {
+            const lf_step &t = steps[s];
+            fp.pr[s] = t.pr; fp.pi[s] = t.pi; fp.hr[s] = t.hr; fp.hi[s] = t.hi;
+            fp.pr32[s] = t.pr32; fp.pi32[s] = t.pi32; fp.hr32[s] = t.hr32; fp.hi32[s] = t.hi32;
+        }
+        tb_flux_kernel<<<dim3((n2max + 255) / 256, mp.nflux, 1), 256, 0, c->stream>>>(fp);
+        LF_HIP(hipGetLastError());
+    }
+    swap_tb_buffers(c);
+    return 0;
+}
+
 }  // namespace
+
+extern "C" int lf_set_temporal_mon(lf_ctx *c, int depth) {
+    if (int e = check_ctx(c)) return e;
+    if (c->kind == LF_YEE3D) {
+        if (depth != 1) return fail(-1, "lf_set_temporal_mon: monitored passes are 2D only");
+    } else if (depth < 1 || depth > 4) {
+        return fail(-1, "lf_set_temporal_mon: depth must be 1 (off), 2, 3 or 4");
+    }
+    c->tb_mon = depth;
+    return 0;
+}
 
 extern "C" int lf_set_temporal(lf_ctx *c, int depth, int lx) {
     if (int e = check_ctx(c)) return e;
@@ -2511,11 +2272,7 @@ int yee_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
             if (u0 >= lo[d] && u1 <= hi[d]) { ia3[d] = std::min(ia3[d], w); ib3[d] = std::max(ib3[d], w); }
         }
     }
-    if (!c->side[0]) {
-        for (int u = 0; u < 2; ++u) LF_HIP(hipStreamCreate(&c->side[u]));
-        LF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-        for (int u = 0; u < 2; ++u) LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
-    }
+    if (int e = ensure_side_streams(c)) return e;
     LF_HIP(hipEventRecord(c->ev_fork, c->stream));
     for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
     for (int mode = 7; mode >= 0; --mode) {
@@ -3188,6 +2945,20 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
                 continue;
             }
         }
+        // 2D steps with monitors: a monitored pass when all of them can be summed inside it
+        const int Tm = allow_tb ? c->tb_mon : 1;
+        TBMonPlan mp;
+        if (c->kind != LF_YEE3D && Tm > 1 && s + Tm <= nsteps && tb_mon_plan(c, steps + s, Tm, mp)) {
+            if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
+            if (int e = tb_chunk_2d_mon(c, steps + s, Tm, mp, fp64)) return e;
+            if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
+            ++ngroup;
+            if (c->kind == LF_TM2D) tm = tm_args(c);
+            else te = te_args(c);
+            ma = mon_args(c);
+            s += Tm - 1;
+            continue;
+        }
         if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
         if (c->kind == LF_TE2D) {
             te_h_kernel<<<grd2, blk, 0, c->stream>>>(te);
@@ -3395,6 +3166,9 @@ void launch_batch_class(int kind, int mode, const void *rec, const int *start, i
     }
 }
 
+// MODE of a 2D pass class (classes 0 and 3 are both slab-free)
+int pl_mode(int cls) { return cls == 1 ? 1 : (cls == 2 ? 3 : 0); }
+
 // one blocked pass of depth T over the listed instances; dsteps[k]: device step table
 // position of ids[k] for this pass
 int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const lf_step *> &dsteps,
@@ -3404,11 +3178,11 @@ int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const
     std::vector<char> pack;                        // records and prefix tables of the pass
     auto align = [&](size_t a) { while (pack.size() % a) pack.push_back(0); };
     for (int kind : {LF_TE2D, LF_TM2D}) {
-        for (int cls = 0; cls < 3; ++cls) {
+        for (int cls = 0; cls < 4; ++cls) {
             std::vector<int> start(1, 0);
             std::vector<TBRecTM> rtm;
             std::vector<TBRecTE> rte;
-            const int mode = cls == 0 ? 0 : (cls == 1 ? 1 : 3);
+            const int mode = pl_mode(cls);
             for (size_t n = 0; n < ids.size(); ++n) {
                 lf_ctx *c = b->ctx[ids[n]];
                 if (c->kind != kind) continue;
@@ -3467,7 +3241,7 @@ int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const
     LF_HIP(hipEventRecord(b->ev_fork, b->stream));
     for (int u = 0; u < 2; ++u) LF_HIP(hipStreamWaitEvent(b->side[u], b->ev_fork, 0));
     for (const Launch &L : launches) {
-        hipStream_t st = L.cls == 0 ? b->stream : b->side[L.cls - 1];
+        hipStream_t st = (L.cls == 0 || L.cls == 3) ? b->stream : b->side[L.cls - 1];
         const void *rec = sg.d + L.rec_off;
         const int *start = (const int *)(sg.d + L.start_off);
         switch (T) {

@@ -65,14 +65,13 @@ __device__ __forceinline__ float apply_source(float v, int comp, int i, int plan
     return v;
 }
 
-__device__ __forceinline__ float apply_drive(float v, int comp, int i, int plane, const Excite &x,
-                                             const StepArgs &s) {
-    if (i >= x.drv_n || x.drv[comp] == nullptr) return v;
-    const float2 D = x.drv[comp][(long long)i * x.drv_stride[comp] + plane];
-    const float w = x.wtap[i];
+// The stitch drive blend of one cell: D = the handed-over phasor of the cell, w = the
+// taper weight of its plane.
+__device__ __forceinline__ float drive_mix(float v, float2 D, float w, int drv_math,
+                                           const StepArgs &s) {
     // Re(D * e^{iwt}): NumPy forms complex products as fma(ar, br, -(ai*bi)) on FMA hosts
     // (its AVX2/AVX-512 loops); the same single-rounding form is used here explicitly.
-    if (x.drv_math == LF_DRIVE_C128) {       // fdtd2d_core.py:1426-1427 (float64 blend)
+    if (drv_math == LF_DRIVE_C128) {         // fdtd2d_core.py:1426-1427 (float64 blend)
         const double re = fma((double)D.x, s.dr, -((double)D.y * s.di));
         const double din = s.env * re;
         const float keep = (1.0f - w) * v;
@@ -80,6 +79,13 @@ __device__ __forceinline__ float apply_drive(float v, int comp, int i, int plane
     }
     const float re = fmaf(D.x, s.dr32, -(D.y * s.di32));   // fdtd3d_core.py:1627-1630 (float32)
     return w * (s.e32 * re) + (1.0f - w) * v;
+}
+
+__device__ __forceinline__ float apply_drive(float v, int comp, int i, int plane, const Excite &x,
+                                             const StepArgs &s) {
+    if (i >= x.drv_n || x.drv[comp] == nullptr) return v;
+    const float2 D = x.drv[comp][(long long)i * x.drv_stride[comp] + plane];
+    return drive_mix(v, D, x.wtap[i], x.drv_math, s);
 }
 
 // Whole-wavefront shifts by one lane (DPP wave_shl:1 / wave_shr:1, one VALU op each, no

@@ -1,8 +1,9 @@
 """Builds liblensfdtd.so (HIP, gfx950) in-tree with hipcc.
 
-Translation units: csrc/lensfdtd.hip (step kernels, monitors, batch engine, C ABI) and
-csrc/yee_tb3.hip (the temporally blocked 3D pass), the latter once per depth / loss
-variant; they compile in parallel into lens_sensor_amd/_build/ and link into one library.
+Translation units: csrc/lensfdtd.hip (step kernels, monitors, batch engine, C ABI),
+csrc/tb2d_mon.hip (monitored 2D blocked passes, once per polarisation) and
+csrc/yee_tb3.hip (the temporally blocked 3D pass, once per depth / loss variant); they
+compile in parallel into lens_sensor_amd/_build/ and link into one library.
 
 Flags: -ffp-contract=off keeps the reference's float32 operation order (no fused
 multiply-add), -Xclang -target-feature -Xclang -packed-fp32-ops keeps the compiler from
@@ -17,6 +18,7 @@ PKG = os.path.dirname(HERE)
 CSRC = os.path.join(PKG, 'csrc')
 SRC = os.path.join(CSRC, 'lensfdtd.hip')
 SRC_TB3 = os.path.join(CSRC, 'yee_tb3.hip')
+SRC_TB2M = os.path.join(CSRC, 'tb2d_mon.hip')
 LIB = os.path.join(PKG, 'liblensfdtd.so')
 BUILD_DIR = os.path.join(PKG, '_build')
 ARCH = os.environ.get('LENSFDTD_ARCH', 'gfx950')
@@ -28,6 +30,9 @@ TB3_VARIANTS = [(2, 0), (2, 1), (3, 0), (3, 1)]       # (depth, loss factors pre
 def _jobs():
     jobs = [(['hipcc'] + FLAGS + ['-c', SRC, '-o', os.path.join(BUILD_DIR, 'lensfdtd.o')],
              os.path.join(BUILD_DIR, 'lensfdtd.o'))]
+    for tm in (0, 1):
+        obj = os.path.join(BUILD_DIR, f'tb2d_mon_{tm}.o')
+        jobs.append((['hipcc'] + FLAGS + [f'-DLF_TB2M_TM={tm}', '-c', SRC_TB2M, '-o', obj], obj))
     for T, L in TB3_VARIANTS:
         obj = os.path.join(BUILD_DIR, f'yee_tb3_{T}_{L}.o')
         jobs.append((['hipcc'] + FLAGS + [f'-DLF_TB3_T={T}', f'-DLF_TB3_LOSS={L}', '-c', SRC_TB3,

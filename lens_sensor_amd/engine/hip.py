@@ -31,7 +31,7 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats',
            'lf_batch_create', 'lf_batch_add', 'lf_batch_set_temporal', 'lf_batch_set_timing',
            'lf_batch_run', 'lf_batch_stats', 'lf_batch_destroy', 'lf_batch_set_table',
-           'lf_batch_run_at', 'lf_autotune')
+           'lf_batch_run_at', 'lf_autotune', 'lf_set_temporal_mon')
 
 
 def library_path():
@@ -75,6 +75,7 @@ def load():
         'lf_set_variant': [P, I, I],
         'lf_set_tile': [P, I],
         'lf_set_temporal': [P, I, I],
+        'lf_set_temporal_mon': [P, I],
         'lf_tb_stats': [P, ctypes.POINTER(ctypes.c_double), ip, ip],
         'lf_autotune': [P, P, I, ip, ip, ctypes.POINTER(ctypes.c_double)],
         'lf_kernel_ms': [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ip],
@@ -334,6 +335,10 @@ class HipEngine:
 
     def set_temporal(self, depth, lx=64):
         _check(self._lib.lf_set_temporal(self.h, int(depth), int(lx)), 'lf_set_temporal')
+
+    def set_temporal_mon(self, depth):
+        """Depth of the 2D passes that also sum the CW-window monitors (1 = off)."""
+        _check(self._lib.lf_set_temporal_mon(self.h, int(depth)), 'lf_set_temporal_mon')
 
     def autotune(self, row, reps=None):
         """Time the candidate blocking depths / x chunks on this engine with monitor-free

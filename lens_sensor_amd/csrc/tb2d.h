@@ -250,7 +250,9 @@ __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *
         }
     }
     if (bits >> 3) {
-        for (int f = 0; f < m->nflux; ++f) {
+#pragma unroll
+        for (int f = 0; f < kTBFlux; ++f) {
+            if (f >= m->nflux) break;
             if (((bits >> (3 + f)) & 1u) && q == m->fil[f] && own && j < m->fn2[f]) {
                 float *o = m->fst[f] + (long long)(2 * s) * m->fn2[f] + j;
                 o[0] = eyn;                                         // Ey(il, j)
@@ -277,7 +279,9 @@ __device__ __forceinline__ void te_mon_level(TBMonRegs<T, MON> &r, const TBMon *
         if ((bits >> 2) & 1u) r.s[0][s + 1] = r.s[0][s + 1] + ezn * ezn;
     }
     if (bits >> 3) {
-        for (int f = 0; f < m->nflux; ++f) {
+#pragma unroll
+        for (int f = 0; f < kTBFlux; ++f) {
+            if (f >= m->nflux) break;
             if (((bits >> (3 + f)) & 1u) && q == m->fil[f] && own && j < m->fn2[f]) {
                 float *o = m->fst[f] + (long long)(2 * s) * m->fn2[f] + j;
                 o[0] = ezn;                                         // Ez(il, j)

@@ -109,7 +109,7 @@ def _compare(kind, ho, oo, mons, exact=True, tol=1e-6):
         if name.startswith('box'):
             btol = 1e-12 if exact else tol
             assert abs(a - b) <= btol * abs(b) + 1e-300, (kind, name, a, b)
-        elif name == 'flux':
+        elif name.startswith('flux'):
             for (ae, ah), (be, bh) in zip(a, b):
                 if exact:
                     np.testing.assert_array_equal(ae, be)

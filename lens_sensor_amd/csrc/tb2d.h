@@ -51,32 +51,77 @@ __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j
     constexpr bool PX = MODE & 2, PY = MODE & 1;
     TMPlane v{};
     const int Nx = a.g.nx, P = a.g.P;
-    if (p < Nx) {
-        const long long c = (long long)p * P + j;
-        const bool xin = !PX || p < Nx - 1;
-        if (xin && jin) {
-            v.ex = a.Ex0[c];
-            v.ca = a.cax[c];
-            if (lossy_x) v.da = a.dax[c];
-        }
-        if (jey) {
-            v.ey = a.Ey0[c];
-            v.cy = a.cay[c];
-            if (lossy_y) v.dy = a.day[c];
-        }
-        if (xin && jey) v.hz = a.Hz0[c];
-        if (PX) {
-            const int sx0 = xin ? strip_pos(a.s0[0], p) : -1;           // H nodes p <= Nx-2
-            if (sx0 >= 0 && jey) v.ph = a.s0[0].psi[(long long)sx0 * P + j];
-            const int sx3 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[3], p) : -1;
-            if (sx3 >= 0 && jey) v.qx = a.s0[3].psi[(long long)sx3 * P + j];
-        }
-        if (PY) {
-            if (sHzy >= 0 && xin) v.py = a.s0[1].psi[(long long)p * a.s0[1].len + sHzy];
-            if (sExy >= 0 && xin) v.qy = a.s0[2].psi[(long long)p * a.s0[2].len + sExy];
-        }
+    if constexpr (MODE == 0) {
+        // slab-free chunks: every loaded lane and plane lies inside the grid, so the
+        // loads are unconditional (planes past the grid, loaded ahead and never used,
+        // read the last plane).  No merge with zero: a loaded register then flows
+        // straight into the iteration that uses it, with no copy that would wait for it.
+        const long long c = (long long)(p < Nx ? p : Nx - 1) * P + j;
+        v.ex = a.Ex0[c];
+        v.ca = a.cax[c];
+        v.da = (lossy_x ? a.dax : a.cax)[c];          // read only when lossy
+        v.ey = a.Ey0[c];
+        v.cy = a.cay[c];
+        v.dy = (lossy_y ? a.day : a.cay)[c];
+        v.hz = a.Hz0[c];
+        return v;
+    }
+    // slab classes: the loads are unconditional too, from clamped addresses (a dummy
+    // array stands in for a missing psi slab); tm_mask_plane, applied where the plane
+    // enters the pipeline, zeroes what the grid and the slabs do not hold
+    const int pc = p < Nx ? p : Nx - 1;
+    const int jc = j < 0 ? 0 : (j < P ? j : P - 1);
+    const long long c = (long long)pc * P + jc;
+    v.ex = a.Ex0[c];
+    v.ca = a.cax[c];
+    v.da = (lossy_x ? a.dax : a.cax)[c];
+    v.ey = a.Ey0[c];
+    v.cy = a.cay[c];
+    v.dy = (lossy_y ? a.day : a.cay)[c];
+    v.hz = a.Hz0[c];
+    if (PX) {
+        const int sx0 = strip_pos(a.s0[0], pc), sx3 = strip_pos(a.s0[3], pc);
+        const float *p0 = a.s0[0].psi ? a.s0[0].psi : a.Ex0, *p3 = a.s0[3].psi ? a.s0[3].psi : a.Ex0;
+        v.ph = p0[(long long)(sx0 > 0 ? sx0 : 0) * P + jc];
+        v.qx = p3[(long long)(sx3 > 0 ? sx3 : 0) * P + jc];
+    }
+    if (PY) {
+        const float *p1 = a.s0[1].psi ? a.s0[1].psi : a.Ex0, *p2 = a.s0[2].psi ? a.s0[2].psi : a.Ex0;
+        const long long l1 = a.s0[1].psi ? a.s0[1].len : 0, l2 = a.s0[2].psi ? a.s0[2].len : 0;
+        v.py = p1[pc * l1 + (sHzy > 0 ? sHzy : 0)];
+        v.qy = p2[pc * l2 + (sExy > 0 ? sExy : 0)];
     }
     return v;
+}
+
+// Zeroes, for plane p of a slab class, every value the grid and the slabs do not hold
+// (the values the conditional loads of the one-step kernels never read).
+template <int MODE>
+__device__ __forceinline__ void tm_mask_plane(TMPlane &v, const TBTMArgs &a, int p, bool jin, bool jey,
+                                              int sHzy, int sExy) {
+    constexpr bool PX = MODE & 2, PY = MODE & 1;
+    if constexpr (MODE == 0) return;
+    const int Nx = a.g.nx;
+    const bool pin = p < Nx;
+    const bool xin = pin && (!PX || p < Nx - 1);
+    const bool mx = xin && jin, my = pin && jey;
+    v.ex = mx ? v.ex : 0.f; v.ca = mx ? v.ca : 0.f; v.da = mx ? v.da : 0.f;
+    v.ey = my ? v.ey : 0.f; v.cy = my ? v.cy : 0.f; v.dy = my ? v.dy : 0.f;
+    v.hz = (xin && jey) ? v.hz : 0.f;
+    if (PX) {
+        const int sx0 = xin ? strip_pos(a.s0[0], p) : -1;               // H nodes p <= Nx-2
+        const int sx3 = (pin && p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[3], p) : -1;
+        v.ph = (sx0 >= 0 && jey) ? v.ph : 0.f;
+        v.qx = (sx3 >= 0 && jey) ? v.qx : 0.f;
+    } else {
+        v.ph = v.qx = 0.f;
+    }
+    if (PY) {
+        v.py = (sHzy >= 0 && xin) ? v.py : 0.f;
+        v.qy = (sExy >= 0 && xin) ? v.qy : 0.f;
+    } else {
+        v.py = v.qy = 0.f;
+    }
 }
 
 // ---------------------------------------------------------------------------- monitors inside a 2D pass
@@ -372,9 +417,17 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
         mr.init(m);
         mr.load(m, ps, j, own, X0, X1);
     }
-    TMPlane nxt = tm_load_plane<MODE>(a, ps, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
-    for (int p = ps; p < pe; ++p) {
-        // ---- slot 0: plane p of step n (loaded one iteration ahead)
+    // planes are prefetched two iterations ahead: a wave's march is a serial chain, and
+    // one plane of loads in flight left the waves waiting on memory latency
+    TMPlane bufA = tm_load_plane<MODE>(a, ps, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    TMPlane bufB = tm_load_plane<MODE>(a, ps + 1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    // iteration p takes plane p from the buffer loaded two iterations earlier and
+    // reloads it with plane p + 2; the loop runs two iterations per trip so that the
+    // two buffers alternate without register copies (a copy of a load in flight
+    // would wait for it at once)
+    auto iter = [&](const int p, TMPlane &nxt) {
+        // ---- slot 0: plane p of step n (loaded two iterations ahead)
+        tm_mask_plane<MODE>(nxt, a, p, jin, jey, sHzy, sExy);
         exC[0] = nxt.ex; eyC[0] = nxt.ey; hzC[0] = nxt.hz;
         hxC[0] = nxt.ph; hyC[0] = nxt.py; eyyC[0] = nxt.qy; exxC[0] = nxt.qx;
         cA[0] = nxt.ca; cY[0] = nxt.cy; dA[0] = nxt.da; dY[0] = nxt.dy;
@@ -386,7 +439,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = tm_load_plane<MODE>(a, p + 1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+        nxt = tm_load_plane<MODE>(a, p + 2, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
         // ---- levels
 #pragma unroll
         for (int s = 0; s < T; ++s) {
@@ -497,6 +550,10 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
             if (EXC) { dR[L] = dR[L - 1]; dI[L] = dI[L - 1]; dW[L] = dW[L - 1]; }
         }
         if constexpr (MON != 0) mr.shift();
+    };
+    for (int p = ps; p < pe; p += 2) {
+        iter(p, bufA);
+        if (p + 1 < pe) iter(p + 1, bufB);
     }
 }
 
@@ -544,29 +601,65 @@ __device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int j
     constexpr bool PX = MODE & 2, PY = MODE & 1;
     TEPlane v{};
     const int Nx = a.g.nx, P = a.g.P;
-    if (p < Nx) {
-        const long long c = (long long)p * P + j;
-        const bool xin = !PX || p < Nx - 1;            // Hy exists on this plane
-        if (jin) {
-            v.ez = a.Ez0[c];
-            v.ce = a.ceE[c];
-            v.cH = a.ceH[c];
-            if (xin) v.hy = a.Hy0[c];
-        }
-        if (jhx) v.hx = a.Hx0[c];
-        if (PX) {
-            const int sx1 = xin ? strip_pos(a.s0[1], p) : -1;           // H nodes p <= Nx-2
-            if (sx1 >= 0 && jin) v.pyx = a.s0[1].psi[(long long)sx1 * P + j];
-            const int sx2 = (p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[2], p) : -1;
-            if (sx2 >= 0 && jin) v.pzx = a.s0[2].psi[(long long)sx2 * P + j];
-        }
-        if (PY) {
-            if (sHxy >= 0) v.pxy = a.s0[0].psi[(long long)p * a.s0[0].len + sHxy];
-            if (sEzy >= 0 && (!PX || (p >= 1 && p <= Nx - 2)))
-                v.pzy = a.s0[3].psi[(long long)p * a.s0[3].len + sEzy];
-        }
+    if constexpr (MODE == 0) {                         // unconditional, as for TM
+        const long long c = (long long)(p < Nx ? p : Nx - 1) * P + j;
+        v.ez = a.Ez0[c];
+        v.ce = a.ceE[c];
+        v.cH = a.ceH[c];
+        v.hy = a.Hy0[c];
+        v.hx = a.Hx0[c];
+        return v;
+    }
+    // slab classes: unconditional loads from clamped addresses, masked by te_mask_plane
+    const int pc = p < Nx ? p : Nx - 1;
+    const int jc = j < 0 ? 0 : (j < P ? j : P - 1);
+    const long long c = (long long)pc * P + jc;
+    v.ez = a.Ez0[c];
+    v.ce = a.ceE[c];
+    v.cH = a.ceH[c];
+    v.hy = a.Hy0[c];
+    v.hx = a.Hx0[c];
+    if (PX) {
+        const int sx1 = strip_pos(a.s0[1], pc), sx2 = strip_pos(a.s0[2], pc);
+        const float *p1 = a.s0[1].psi ? a.s0[1].psi : a.Ez0, *p2 = a.s0[2].psi ? a.s0[2].psi : a.Ez0;
+        v.pyx = p1[(long long)(sx1 > 0 ? sx1 : 0) * P + jc];
+        v.pzx = p2[(long long)(sx2 > 0 ? sx2 : 0) * P + jc];
+    }
+    if (PY) {
+        const float *p0 = a.s0[0].psi ? a.s0[0].psi : a.Ez0, *p3 = a.s0[3].psi ? a.s0[3].psi : a.Ez0;
+        const long long l0 = a.s0[0].psi ? a.s0[0].len : 0, l3 = a.s0[3].psi ? a.s0[3].len : 0;
+        v.pxy = p0[pc * l0 + (sHxy > 0 ? sHxy : 0)];
+        v.pzy = p3[pc * l3 + (sEzy > 0 ? sEzy : 0)];
     }
     return v;
+}
+
+template <int MODE>
+__device__ __forceinline__ void te_mask_plane(TEPlane &v, const TBTEArgs &a, int p, bool jin, bool jhx,
+                                              int sHxy, int sEzy) {
+    constexpr bool PX = MODE & 2, PY = MODE & 1;
+    if constexpr (MODE == 0) return;
+    const int Nx = a.g.nx;
+    const bool pin = p < Nx;
+    const bool xin = pin && (!PX || p < Nx - 1);                   // Hy exists on this plane
+    const bool mj = pin && jin;
+    v.ez = mj ? v.ez : 0.f; v.ce = mj ? v.ce : 0.f; v.cH = mj ? v.cH : 0.f;
+    v.hy = (xin && jin) ? v.hy : 0.f;
+    v.hx = (pin && jhx) ? v.hx : 0.f;
+    if (PX) {
+        const int sx1 = xin ? strip_pos(a.s0[1], p) : -1;               // H nodes p <= Nx-2
+        const int sx2 = (pin && p >= 1 && p <= Nx - 2) ? strip_pos(a.s0[2], p) : -1;
+        v.pyx = (sx1 >= 0 && jin) ? v.pyx : 0.f;
+        v.pzx = (sx2 >= 0 && jin) ? v.pzx : 0.f;
+    } else {
+        v.pyx = v.pzx = 0.f;
+    }
+    if (PY) {
+        v.pxy = (pin && sHxy >= 0) ? v.pxy : 0.f;
+        v.pzy = (pin && sEzy >= 0 && (!PX || (p >= 1 && p <= Nx - 2))) ? v.pzy : 0.f;
+    } else {
+        v.pxy = v.pzy = 0.f;
+    }
 }
 
 template <int T, int MODE, bool EXC = true, int MON = 0>
@@ -639,8 +732,14 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         mr.init(m);
         mr.load(m, ps, j, own, X0, X1);
     }
-    TEPlane nxt = te_load_plane<MODE>(a, ps, j, jin, jhx, sHxy, sEzy);
-    for (int p = ps; p < pe; ++p) {
+    TEPlane bufA = te_load_plane<MODE>(a, ps, j, jin, jhx, sHxy, sEzy);      // two ahead, as TM
+    TEPlane bufB = te_load_plane<MODE>(a, ps + 1, j, jin, jhx, sHxy, sEzy);
+    // iteration p takes plane p from the buffer loaded two iterations earlier and
+    // reloads it with plane p + 2; the loop runs two iterations per trip so that the
+    // two buffers alternate without register copies (a copy of a load in flight
+    // would wait for it at once)
+    auto iter = [&](const int p, TEPlane &nxt) {
+        te_mask_plane<MODE>(nxt, a, p, jin, jhx, sHxy, sEzy);
         ezC[0] = nxt.ez; hxC[0] = nxt.hx; hyC[0] = nxt.hy;
         axyC[0] = nxt.pxy; ayxC[0] = nxt.pyx; azxC[0] = nxt.pzx; azyC[0] = nxt.pzy;
         cE[0] = nxt.ce; cH[0] = nxt.cH;
@@ -652,7 +751,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = te_load_plane<MODE>(a, p + 1, j, jin, jhx, sHxy, sEzy);
+        nxt = te_load_plane<MODE>(a, p + 2, j, jin, jhx, sHxy, sEzy);
 #pragma unroll
         for (int s = 0; s < T; ++s) {
             const int q = p - 1 - s;
@@ -750,6 +849,10 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
             if (EXC) { dR[L] = dR[L - 1]; dI[L] = dI[L - 1]; dW[L] = dW[L - 1]; }
         }
         if constexpr (MON != 0) mr.shift();
+    };
+    for (int p = ps; p < pe; p += 2) {
+        iter(p, bufA);
+        if (p + 1 < pe) iter(p + 1, bufB);
     }
 }
 

@@ -1715,22 +1715,33 @@ StepArgs step_args(const lf_step &t) {
 bool x_slab_2d(int kind, int q) { return kind == LF_TM2D ? (q == 0 || q == 3) : (q == 1 || q == 2); }
 bool tm_x_slab(int q) { return x_slab_2d(LF_TM2D, q); }
 
-// Wavefront plan of one temporally blocked 2D pass: up to three classes, each with the
-// x chunk length and the (x chunk, y wave) rectangles it covers.  Class 0 (MODE 0): the
-// slab-free rectangle, whose loaded planes and lanes lie in 1..N-2 and outside every
-// slab; class 1 (MODE 1): the rows below / above it (y slabs only); class 2 (MODE 3):
-// the x chunks left / right of it over all rows, or every wavefront of a grid without a
-// slab-free rectangle; class 3 (MODE 0 with excitation code): the chunks of the
-// slab-free rectangle whose marches meet a source or drive plane (the stitch drive
-// covers the whole window overlap, a quarter of a window), so that class 0 runs the
-// body without excitation loads.
+// Wavefront plan of one temporally blocked 2D pass: up to five classes, each a list of
+// at most four rectangles of (x chunk, y wave) indices with their own x chunk length.
+// The slab-free rectangle holds the wavefronts whose loaded planes and lanes lie in
+// 1..N-2 and outside every slab; the y-slab rows lie below / above it, the x-slab chunks
+// left / right of it over all rows.  A chunk whose march comes within T planes of a
+// source or drive plane needs the body with excitation code (the stitch drive covers the
+// whole 2048-plane window overlap, a quarter of a window); the others take the body
+// without it, which holds fewer registers and no excitation loads.
+//   class 0: MODE 0, slab-free chunks without excitation (chunks of lx)
+//   class 1: MODE 1, y-slab rows (chunks of lx/4)
+//   class 2: MODE 3, x-slab chunks (chunks of lx/16: the finest), or every wavefront
+//            of a grid without a slab-free rectangle (with excitation)
+//   class 3: MODE 0, slab-free chunks with excitation (chunks of lx/4: their waves
+//            carry the excitation work, and few long marches would leave SIMDs idle)
+// The slab classes choose the body per rectangle when only some of their rectangles
+// meet an excitation plane (their bodies with and without excitation code hold the same
+// number of waves per SIMD, or nearly).  When the excitation planes scatter so that a
+// class would need more than four rectangles, the slab classes take the excitation code
+// over all their chunks.
 struct TBClass {
-    int mode, lx, nseg;
-    int exc;                  // 0: no source / drive plane within the class's marches
-    int wx0[4], wy0[4], nwy[4], end[4];
+    int mode, nseg;
+    int exc;                  // 0: no rectangle meets an excitation plane, 1: all, 2: some
+    int wx0[4], wy0[4], nwy[4], end[4], lx[4], sexc[4];
 };
+constexpr int kTBClasses = 4;
 struct TBPlan {
-    TBClass cls[4];
+    TBClass cls[kTBClasses];
     bool interior;
 };
 
@@ -1757,80 +1768,93 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
             iy0 = std::min(iy0, wy); iy1 = std::max(iy1, wy);
         }
     }
-    TBPlan pl{};
-    pl.interior = ix1 >= ix0 && iy1 >= iy0;
-    auto add = [](TBClass &q, int wx0, int nwx_s, int wy0, int nwy_s) {
-        if (nwx_s <= 0 || nwy_s <= 0) return;
-        const int start = q.nseg ? q.end[q.nseg - 1] : 0;
-        q.wx0[q.nseg] = wx0; q.wy0[q.nseg] = wy0; q.nwy[q.nseg] = nwy_s;
-        q.end[q.nseg] = start + nwx_s * nwy_s;
-        ++q.nseg;
-    };
     auto div = [&](int want) {
         for (int cand : {want, 8, 4, 2})
             if (cand <= want && lx % cand == 0 && lx / cand >= 4) return cand;
         return 1;
     };
-    pl.cls[0].mode = 0; pl.cls[1].mode = 1; pl.cls[2].mode = 3; pl.cls[3].mode = 0;
-    pl.cls[0].exc = pl.cls[1].exc = pl.cls[2].exc = pl.cls[3].exc = 1;
-    if (!pl.interior) {
-        const int k = div(4);
-        pl.cls[2].lx = lx / k;
-        add(pl.cls[2], 0, (Nx + lx / k - 1) / (lx / k), 0, nwy);
-        return pl;
-    }
-    const int ni = iy1 - iy0 + 1;
-    pl.cls[0].lx = lx;
-    pl.cls[3].lx = lx;
-    const int ke = div(4);                               // class 3: chunks of lx/ke
-    {   // slab-free chunks whose marches meet a source or drive plane go to class 3
-        const int comp = c->kind == LF_TM2D ? LF_EY : LF_EZ;
-        auto meets = [&](int wx) {
-            const int m0 = wx * lx - T, m1 = std::min((wx + 1) * lx, Nx) + T;
-            bool e = false;
-            if (c->ex.amp && c->ex.src_comp == comp) e = e || (c->ex.src_ix < m1 && c->ex.src_ix + 2 > m0);
-            if (c->ex.wtap && c->ex.drv[comp]) e = e || (0 < m1 && c->ex.drv_n > m0);
-            return e;
+    // does the march of the chunk [X0, X1) meet a source or drive plane?
+    const int comp = c->kind == LF_TM2D ? LF_EY : LF_EZ;
+    auto meets = [&](int X0, int X1) {
+        const int m0 = X0 - T, m1 = std::min(X1, Nx) + T;
+        bool e = false;
+        if (c->ex.amp && c->ex.src_comp == comp) e = e || (c->ex.src_ix < m1 && c->ex.src_ix + 2 > m0);
+        if (c->ex.wtap && c->ex.drv[comp]) e = e || (0 < m1 && c->ex.drv_n > m0);
+        return e;
+    };
+    auto build = [&](bool split_slabs, TBPlan &pl) -> bool {
+        pl = TBPlan{};
+        pl.interior = ix1 >= ix0 && iy1 >= iy0;
+        const int modes[kTBClasses] = {0, 1, 3, 0};
+        for (int u = 0; u < kTBClasses; ++u) pl.cls[u].mode = modes[u];
+        bool ok = true;
+        auto add = [&](TBClass &q, int wx0, int nwx_s, int wy0, int nwy_s, int lxs, bool e) {
+            if (nwx_s <= 0 || nwy_s <= 0) return;
+            if (q.nseg == 4) { ok = false; return; }
+            const int start = q.nseg ? q.end[q.nseg - 1] : 0;
+            q.wx0[q.nseg] = wx0; q.wy0[q.nseg] = wy0; q.nwy[q.nseg] = nwy_s; q.lx[q.nseg] = lxs;
+            q.sexc[q.nseg] = e ? 1 : 0;
+            q.end[q.nseg] = start + nwx_s * nwy_s;
+            ++q.nseg;
         };
-        int runs[2] = {0, 0};                    // runs of chunks per class
-        for (int wx = ix0; wx <= ix1; ++wx)
-            if (wx == ix0 || meets(wx) != meets(wx - 1)) ++runs[meets(wx) ? 1 : 0];
-        if (runs[0] <= 4 && runs[1] <= 4) {
-            // class 3 marches finer chunks: its waves carry the excitation loads and
-            // arithmetic, and few long marches would leave most SIMDs with one wave
-            pl.cls[3].lx = lx / ke;
-            for (int wx = ix0; wx <= ix1;) {
-                int w1 = wx;
-                while (w1 + 1 <= ix1 && meets(w1 + 1) == meets(wx)) ++w1;
-                if (meets(wx)) add(pl.cls[3], wx * ke, (w1 - wx + 1) * ke, iy0, ni);
-                else add(pl.cls[0], wx, w1 - wx + 1, iy0, ni);
-                wx = w1 + 1;
+        // runs of chunks [c0, c1] of length L, one rectangle per run of equal excitation
+        auto runs = [&](int c0, int c1, int L, int wy0, int nwy_s, int cls, bool split) {
+            for (int w = c0; w <= c1;) {
+                const bool e = !split || meets(w * L, (w + 1) * L);
+                int w1 = w;
+                while (w1 + 1 <= c1 && (!split || meets((w1 + 1) * L, (w1 + 2) * L)) == e) ++w1;
+                add(pl.cls[cls], w, w1 - w + 1, wy0, nwy_s, L, e);
+                w = w1 + 1;
             }
-            pl.cls[0].exc = 0;
-        } else {                                 // scattered planes: one class with excitation code
-            add(pl.cls[0], ix0, ix1 - ix0 + 1, iy0, ni);
+        };
+        auto finish = [&]() {
+            for (TBClass &q : pl.cls) {
+                int n = 0;
+                for (int g = 0; g < q.nseg; ++g) n += q.sexc[g];
+                q.exc = n == 0 ? 0 : (n == q.nseg ? 1 : 2);
+            }
+            return ok;
+        };
+        if (!pl.interior) {
+            const int L = lx / div(4);
+            add(pl.cls[2], 0, (Nx + L - 1) / L, 0, nwy, L, true);
+            return finish();
         }
-    }
-    const int k = div(4);                                 // y-slab rows: chunks of lx/k
-    pl.cls[1].lx = lx / k;
-    const int rx0 = ix0 * k, rx1 = (ix1 + 1) * k - 1;
-    add(pl.cls[1], rx0, rx1 - rx0 + 1, 0, iy0);
-    add(pl.cls[1], rx0, rx1 - rx0 + 1, iy1 + 1, nwy - 1 - iy1);
-    const int kc = div(16);                               // x-slab chunks: the finest
-    pl.cls[2].lx = lx / kc;
-    const int cx0 = ix0 * kc, cx1 = (ix1 + 1) * kc - 1, nwxc = (Nx + lx / kc - 1) / (lx / kc);
-    add(pl.cls[2], 0, cx0, 0, nwy);
-    add(pl.cls[2], cx1 + 1, nwxc - 1 - cx1, 0, nwy);
+        const int ni = iy1 - iy0 + 1;
+        const int ke = div(4);
+        for (int wx = ix0; wx <= ix1;) {                 // slab-free chunks: classes 0 / 3
+            const bool e = meets(wx * lx, (wx + 1) * lx);
+            int w1 = wx;
+            while (w1 + 1 <= ix1 && meets((w1 + 1) * lx, (w1 + 2) * lx) == e) ++w1;
+            if (e) add(pl.cls[3], wx * ke, (w1 - wx + 1) * ke, iy0, ni, lx / ke, true);
+            else add(pl.cls[0], wx, w1 - wx + 1, iy0, ni, lx, false);
+            wx = w1 + 1;
+        }
+        const int k = div(4);                            // y-slab rows: class 1
+        const int rx0 = ix0 * k, rx1 = (ix1 + 1) * k - 1;
+        runs(rx0, rx1, lx / k, 0, iy0, 1, split_slabs);
+        runs(rx0, rx1, lx / k, iy1 + 1, nwy - 1 - iy1, 1, split_slabs);
+        const int kc = div(16);                          // x-slab chunks: class 2
+        const int L = lx / kc;
+        const int cx0 = ix0 * kc, cx1 = (ix1 + 1) * kc - 1, nwxc = (Nx + L - 1) / L;
+        runs(0, cx0 - 1, L, 0, nwy, 2, split_slabs);
+        runs(cx1 + 1, nwxc - 1, L, 0, nwy, 2, split_slabs);
+        return finish();
+    };
+    TBPlan pl;
+    if (!build(true, pl)) build(false, pl);
     return pl;
 }
 
 template <typename A>
 void set_class(A &q, const TBClass &k) {
-    q.lx = k.lx;
     q.nseg = k.nseg;
     for (int u = 0; u < 4; ++u) {
         q.seg_wx0[u] = k.wx0[u]; q.seg_wy0[u] = k.wy0[u]; q.seg_nwy[u] = k.nwy[u]; q.seg_end[u] = k.end[u];
+        q.seg_lx[u] = k.lx[u];
     }
+    q.seg_exc = 0u;
+    for (int u = 0; u < k.nseg; ++u) q.seg_exc |= (unsigned)(k.sexc[u] != 0) << u;
 }
 
 int ensure_side_streams(lf_ctx *c) {
@@ -1853,35 +1877,26 @@ int fork_tb_classes(lf_ctx *c, const TBPlan &pl, F one) {
         return 0;
     }
     if (int e = ensure_side_streams(c)) return e;
-    const int nside = pl.cls[3].nseg > 0 ? 3 : 2;
     LF_HIP(hipEventRecord(c->ev_fork, c->stream));
-    for (int u = 0; u < nside; ++u) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
-    one(pl.cls[1], c->side[0]);
-    one(pl.cls[2], c->side[1]);
-    if (nside == 3) one(pl.cls[3], c->side[2]);
+    for (int u = 0; u < 3; ++u)
+        if (pl.cls[u + 1].nseg > 0) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
+    for (int u = 0; u < 3; ++u) one(pl.cls[u + 1], c->side[u]);
     one(pl.cls[0], c->stream);
-    for (int u = 0; u < nside; ++u) {
+    for (int u = 0; u < 3; ++u) {
+        if (pl.cls[u + 1].nseg == 0) continue;
         LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
         LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
     }
     return 0;
 }
 
-template <int T, int MODE, bool EXC = true>
-void launch_tb(const TBTMArgs &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
-    tm_tb_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
-}
-template <int T, int MODE, bool EXC = true>
-void launch_tb(const TBTEArgs &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
-    te_tb_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
-}
-template <typename A, int T>
-void launch_tb_mode(const A &q, int mode, bool exc, const Excite &x, int fp64, hipStream_t st) {
+void launch_tb_mode(const TBTMArgs &q, int T, int mode, int exc, const Excite &x, int fp64, hipStream_t st) {
     const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
-    if (mode == 0 && !exc) launch_tb<T, 0, false>(q, x, fp64, st, nb);
-    else if (mode == 0) launch_tb<T, 0>(q, x, fp64, st, nb);
-    else if (mode == 1) launch_tb<T, 1>(q, x, fp64, st, nb);
-    else launch_tb<T, 3>(q, x, fp64, st, nb);
+    launch_tm_tb(T, mode, exc, q, x, fp64, st, nb);
+}
+void launch_tb_mode(const TBTEArgs &q, int T, int mode, int exc, const Excite &x, int fp64, hipStream_t st) {
+    const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
+    launch_te_tb(T, mode, exc, q, x, fp64, st, nb);
 }
 
 // Launch the classes of one blocked pass.  Without a slab-free rectangle: one launch on
@@ -1893,12 +1908,7 @@ int launch_tb_pass(lf_ctx *c, A &f, const TBPlan &pl, int T, int fp64) {
         if (k.nseg == 0) return;
         A q = f;
         set_class(q, k);
-        switch (T) {
-            case 2: launch_tb_mode<A, 2>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
-            case 4: launch_tb_mode<A, 4>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
-            case 6: launch_tb_mode<A, 6>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
-            default: launch_tb_mode<A, 8>(q, k.mode, k.exc != 0, c->ex, fp64, st); break;
-        }
+        launch_tb_mode(q, T, k.mode, k.exc, c->ex, fp64, st);
     };
     return fork_tb_classes(c, pl, one);
 }
@@ -3178,7 +3188,7 @@ int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const
     std::vector<char> pack;                        // records and prefix tables of the pass
     auto align = [&](size_t a) { while (pack.size() % a) pack.push_back(0); };
     for (int kind : {LF_TE2D, LF_TM2D}) {
-        for (int cls = 0; cls < 4; ++cls) {
+        for (int cls = 0; cls < kTBClasses; ++cls) {
             std::vector<int> start(1, 0);
             std::vector<TBRecTM> rtm;
             std::vector<TBRecTE> rte;

@@ -29,12 +29,13 @@ struct TBTMArgs {
     Prof xe, xh, ye, yh;
     Strip s0[4], s1[4];       // Hzx Hzy Exy Eyx: read / write buffers
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
-    int lx;
     // wavefronts of this launch: up to 4 rectangles of (x chunk, y wave) indices; wave
     // wv of segment g (seg_end[g-1] <= wv < seg_end[g]) is
-    // (seg_wx0 + (wv - start) / seg_nwy, seg_wy0 + (wv - start) % seg_nwy)
+    // (seg_wx0 + (wv - start) / seg_nwy, seg_wy0 + (wv - start) % seg_nwy), its x chunk
+    // [wx * seg_lx, (wx + 1) * seg_lx)
     int nseg;
-    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4];
+    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
+    unsigned seg_exc;         // bit g: segment g needs the body with excitation code
     StepArgs st[kMaxTB];
 };
 
@@ -350,9 +351,9 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
     const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
     const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
-    const int X0 = wx * a.lx;
+    const int X0 = wx * a.seg_lx[g];
     if (X0 >= Nx) return;                          // whole wavefront leaves together
-    const int X1 = min(X0 + a.lx, Nx);
+    const int X1 = min(X0 + a.seg_lx[g], Nx);
     const int j = wy * (64 - 2 * T) - T + lane;
     // the host launches a MODE only on wavefronts whose loaded lanes (YIN) / planes (XIN)
     // lie in 1..Ny-2 / 1..Nx-2 and outside every slab, so those tests are constant
@@ -569,6 +570,26 @@ __global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp
     tm_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
 }
 
+// segment of wavefront wv (uniform), -1 past the last
+__device__ __forceinline__ int tb_segment(const int *seg_end, int nseg, int wv) {
+    if (wv >= seg_end[nseg - 1]) return -1;
+    int g = 0;
+    while (g < nseg - 1 && wv >= seg_end[g]) ++g;
+    return g;
+}
+
+// A slab class whose rectangles differ in excitation: each wavefront takes the body its
+// rectangle needs (the two bodies hold nearly the same number of waves per SIMD).
+template <int T, int MODE>
+__global__ void __launch_bounds__(256) tm_tb_mixed_kernel(TBTMArgs a, Excite x, int fp64) {
+    __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = tb_segment(a.seg_end, a.nseg, wv);
+    if (g < 0) return;
+    if ((a.seg_exc >> g) & 1u) tm_tb_body<T, MODE, true>(a, x, a.st, wv, fp64);
+    else tm_tb_body<T, MODE, false>(a, x, a.st, wv, fp64);
+}
+
 // ---------------------------------------------------------------------------- temporal blocking (2D TE)
 // Same pipeline as the TM pass with the roles of E and H exchanged: level s advances
 // plane q = p-1-s from step n+s to n+s+1: Hx^{n+s+1/2}(q) needs Ez(q, j+1) (lane above),
@@ -585,9 +606,9 @@ struct TBTEArgs {
     Prof xe, xh, ye, yh;
     Strip s0[4], s1[4];       // Hxy Hyx Ezx Ezy: read / write buffers
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
-    int lx;
-    int nseg;
-    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4];
+    int nseg;                 // wavefront rectangles, as in TBTMArgs
+    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
+    unsigned seg_exc;
     StepArgs st[kMaxTB];
 };
 
@@ -674,9 +695,9 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
     const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
     const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
     const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
-    const int X0 = wx * a.lx;
+    const int X0 = wx * a.seg_lx[g];
     if (X0 >= Nx) return;
-    const int X1 = min(X0 + a.lx, Nx);
+    const int X1 = min(X0 + a.seg_lx[g], Nx);
     const int j = wy * (64 - 2 * T) - T + lane;
     const bool jin = YIN || (j >= 0 && j < Ny);
     const bool jhx = YIN || (jin && j < Ny - 1);   // Hx exists in this column
@@ -863,6 +884,16 @@ __global__ void __launch_bounds__(256) te_tb_kernel(TBTEArgs a, Excite x, int fp
     te_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
 }
 
+template <int T, int MODE>
+__global__ void __launch_bounds__(256) te_tb_mixed_kernel(TBTEArgs a, Excite x, int fp64) {
+    __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = tb_segment(a.seg_end, a.nseg, wv);
+    if (g < 0) return;
+    if ((a.seg_exc >> g) & 1u) te_tb_body<T, MODE, true>(a, x, a.st, wv, fp64);
+    else te_tb_body<T, MODE, false>(a, x, a.st, wv, fp64);
+}
+
 // Monitored passes: the plain pass plus the monitor work of TBMon (instantiated in
 // tb2d_mon.hip for depths 2..4 and MON in {9, 14}).
 template <int T, int MODE, bool EXC, int MON>
@@ -878,6 +909,13 @@ __global__ void __launch_bounds__(256) te_tb_mon_kernel(TBTEArgs a, Excite x, TB
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     te_tb_body<T, MODE, EXC, MON>(a, x, a.st, wv, fp64, &m);
 }
+
+// launchers of the plain passes (tb2d_pass.hip): depth T in {2, 4, 6, 8}, mode 0 / 1 /
+// 3; exc 0 without excitation code, 1 with it, 2 per rectangle (modes 1 and 3)
+void launch_tm_tb(int T, int mode, int exc, const TBTMArgs &q, const Excite &x, int fp64,
+                  hipStream_t st, unsigned nb);
+void launch_te_tb(int T, int mode, int exc, const TBTEArgs &q, const Excite &x, int fp64,
+                  hipStream_t st, unsigned nb);
 
 // launchers (tb2d_mon.hip): depth T in {2, 3, 4}, mode 0 / 1 / 3, mon 9 (one phasor) or
 // 14 (two phasors and an intensity sum); flux staging in both

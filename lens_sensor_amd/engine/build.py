@@ -1,7 +1,8 @@
 """Builds liblensfdtd.so (HIP, gfx950) in-tree with hipcc.
 
 Translation units: csrc/lensfdtd.hip (step kernels, monitors, batch engine, C ABI),
-csrc/tb2d_mon.hip (monitored 2D blocked passes, once per polarisation) and
+csrc/tb2d_pass.hip and csrc/tb2d_mon.hip (plain and monitored 2D blocked passes, once per
+polarisation each) and
 csrc/yee_tb3.hip (the temporally blocked 3D pass, once per depth / loss variant); they
 compile in parallel into lens_sensor_amd/_build/ and link into one library.
 
@@ -19,6 +20,7 @@ CSRC = os.path.join(PKG, 'csrc')
 SRC = os.path.join(CSRC, 'lensfdtd.hip')
 SRC_TB3 = os.path.join(CSRC, 'yee_tb3.hip')
 SRC_TB2M = os.path.join(CSRC, 'tb2d_mon.hip')
+SRC_TB2P = os.path.join(CSRC, 'tb2d_pass.hip')
 LIB = os.path.join(PKG, 'liblensfdtd.so')
 BUILD_DIR = os.path.join(PKG, '_build')
 ARCH = os.environ.get('LENSFDTD_ARCH', 'gfx950')
@@ -33,6 +35,8 @@ def _jobs():
     for tm in (0, 1):
         obj = os.path.join(BUILD_DIR, f'tb2d_mon_{tm}.o')
         jobs.append((['hipcc'] + FLAGS + [f'-DLF_TB2M_TM={tm}', '-c', SRC_TB2M, '-o', obj], obj))
+        obj = os.path.join(BUILD_DIR, f'tb2d_pass_{tm}.o')
+        jobs.append((['hipcc'] + FLAGS + [f'-DLF_TB2P_TM={tm}', '-c', SRC_TB2P, '-o', obj], obj))
     for T, L in TB3_VARIANTS:
         obj = os.path.join(BUILD_DIR, f'yee_tb3_{T}_{L}.o')
         jobs.append((['hipcc'] + FLAGS + [f'-DLF_TB3_T={T}', f'-DLF_TB3_LOSS={L}', '-c', SRC_TB3,

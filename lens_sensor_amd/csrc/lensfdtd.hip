@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <set>
+#include <tuple>
 #include <utility>
 #include <string>
 #include <vector>
@@ -1715,18 +1716,20 @@ StepArgs step_args(const lf_step &t) {
 bool x_slab_2d(int kind, int q) { return kind == LF_TM2D ? (q == 0 || q == 3) : (q == 1 || q == 2); }
 bool tm_x_slab(int q) { return x_slab_2d(LF_TM2D, q); }
 
-// Wavefront plan of one temporally blocked 2D pass: up to five classes, each a list of
-// at most four rectangles of (x chunk, y wave) indices with their own x chunk length.
-// The slab-free rectangle holds the wavefronts whose loaded planes and lanes lie in
-// 1..N-2 and outside every slab; the y-slab rows lie below / above it, the x-slab chunks
-// left / right of it over all rows.  A chunk whose march comes within T planes of a
-// source or drive plane needs the body with excitation code (the stitch drive covers the
-// whole 2048-plane window overlap, a quarter of a window); the others take the body
-// without it, which holds fewer registers and no excitation loads.
+// Wavefront plan of one temporally blocked 2D pass: up to four classes, each a list of
+// at most four rectangles, a rectangle being an x range cut into chunks of its own
+// length times a range of y waves.  The slab-free x range [A, B) holds the chunks whose
+// loaded planes lie in 1..Nx-2 and outside the x slabs, the slab-free rows the waves
+// whose lanes lie in 1..Ny-2 and outside the y slabs; the chunk grid starts at A, so the
+// x-slab chunks cover only the planes within T of a slab or an edge.  A chunk whose march
+// comes within T planes of a source or drive plane needs the body with excitation code
+// (the stitch drive covers the whole 2048-plane window overlap, a quarter of a window);
+// the others take the body without it, which holds fewer registers and no excitation
+// loads.
 //   class 0: MODE 0, slab-free chunks without excitation (chunks of lx)
-//   class 1: MODE 1, y-slab rows (chunks of lx/4)
-//   class 2: MODE 3, x-slab chunks (chunks of lx/16: the finest), or every wavefront
-//            of a grid without a slab-free rectangle (with excitation)
+//   class 1: MODE 1, y-slab rows over [A, B) (chunks of lx/4)
+//   class 2: MODE 3, x-slab chunks over all rows (chunks of lx/16: the finest), or
+//            every wavefront of a grid without a slab-free rectangle (with excitation)
 //   class 3: MODE 0, slab-free chunks with excitation (chunks of lx/4: their waves
 //            carry the excitation work, and few long marches would leave SIMDs idle)
 // The slab classes choose the body per rectangle when only some of their rectangles
@@ -1737,7 +1740,8 @@ bool tm_x_slab(int q) { return x_slab_2d(LF_TM2D, q); }
 struct TBClass {
     int mode, nseg;
     int exc;                  // 0: no rectangle meets an excitation plane, 1: all, 2: some
-    int wx0[4], wy0[4], nwy[4], end[4], lx[4], sexc[4];
+    int x0[4], x1[4], lx[4];  // per rectangle: x range, chunk length
+    int wy0[4], nwy[4], end[4], sexc[4];
 };
 constexpr int kTBClasses = 4;
 struct TBPlan {
@@ -1748,64 +1752,66 @@ struct TBPlan {
 TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
     const int W = 64 - 2 * T;
     const int Nx = c->g.nx, Ny = c->g.nz;
-    const int nwy = (Ny + W - 1) / W, nwx = (Nx + lx - 1) / lx;
+    const int nwy = (Ny + W - 1) / W;
     int xlo = 0, xhi = Nx, ylo = 0, yhi = Ny;
     for (int q = 0; q < 4; ++q) {
         if (!c->s2[q].psi) continue;
         if (x_slab_2d(c->kind, q)) { xlo = std::max(xlo, c->s2[q].lo); xhi = std::min(xhi, c->s2[q].hi); }
         else { ylo = std::max(ylo, c->s2[q].lo); yhi = std::min(yhi, c->s2[q].hi); }
     }
-    int ix0 = nwx, ix1 = -1, iy0 = nwy, iy1 = -1;
-    for (int wx = 0; wx < nwx; ++wx) {
-        const int X0 = wx * lx, X1 = std::min(X0 + lx, Nx);
-        if (X0 - T >= std::max(xlo, 1) && X1 + T - 1 < std::min(xhi, Nx - 2) + 1) {
-            ix0 = std::min(ix0, wx); ix1 = std::max(ix1, wx);
-        }
-    }
+    int iy0 = nwy, iy1 = -1;
     for (int wy = 0; wy < nwy; ++wy) {
         const int j0 = wy * W - T;
         if (j0 >= std::max(ylo, 1) && j0 + 63 < std::min(yhi, Ny - 2) + 1) {
             iy0 = std::min(iy0, wy); iy1 = std::max(iy1, wy);
         }
     }
+    // slab-free x range: a chunk [X0, X1) inside [A, B) loads planes X0-T .. X1+T-1, all
+    // in 1..Nx-2 and outside the x slabs; the chunk grid starts at A, so the x-slab
+    // chunks cover only the planes within T of a slab or edge
+    const int A = std::max(xlo, 1) + T, B = std::min(xhi, Nx - 2) - T + 1;
     auto div = [&](int want) {
         for (int cand : {want, 8, 4, 2})
             if (cand <= want && lx % cand == 0 && lx / cand >= 4) return cand;
         return 1;
     };
-    // does the march of the chunk [X0, X1) meet a source or drive plane?
+    // planes whose chunks' marches can meet a source or drive plane: T around each
     const int comp = c->kind == LF_TM2D ? LF_EY : LF_EZ;
-    auto meets = [&](int X0, int X1) {
-        const int m0 = X0 - T, m1 = std::min(X1, Nx) + T;
-        bool e = false;
-        if (c->ex.amp && c->ex.src_comp == comp) e = e || (c->ex.src_ix < m1 && c->ex.src_ix + 2 > m0);
-        if (c->ex.wtap && c->ex.drv[comp]) e = e || (0 < m1 && c->ex.drv_n > m0);
-        return e;
+    std::vector<std::pair<int, int>> zones;
+    if (c->ex.amp && c->ex.src_comp == comp) zones.push_back({c->ex.src_ix - T, c->ex.src_ix + 2 + T});
+    if (c->ex.wtap && c->ex.drv[comp]) zones.push_back({-T, c->ex.drv_n + T});
+    // [u0, u1) in runs of equal excitation (zone edges split it)
+    auto runs_of = [&](int u0, int u1) {
+        std::vector<int> cut = {u0, u1};
+        for (auto &z : zones)
+            for (int e : {z.first, z.second})
+                if (e > u0 && e < u1) cut.push_back(e);
+        std::sort(cut.begin(), cut.end());
+        cut.erase(std::unique(cut.begin(), cut.end()), cut.end());
+        std::vector<std::tuple<int, int, bool>> out;
+        for (size_t i = 0; i + 1 < cut.size(); ++i) {
+            const int a = cut[i], b = cut[i + 1];
+            bool e = false;
+            for (auto &z : zones) e = e || (a < z.second && b > z.first);
+            if (!out.empty() && std::get<2>(out.back()) == e) std::get<1>(out.back()) = b;
+            else out.push_back({a, b, e});
+        }
+        return out;
     };
     auto build = [&](bool split_slabs, TBPlan &pl) -> bool {
         pl = TBPlan{};
-        pl.interior = ix1 >= ix0 && iy1 >= iy0;
+        pl.interior = B > A && iy1 >= iy0;
         const int modes[kTBClasses] = {0, 1, 3, 0};
         for (int u = 0; u < kTBClasses; ++u) pl.cls[u].mode = modes[u];
         bool ok = true;
-        auto add = [&](TBClass &q, int wx0, int nwx_s, int wy0, int nwy_s, int lxs, bool e) {
-            if (nwx_s <= 0 || nwy_s <= 0) return;
+        auto add = [&](TBClass &q, int x0, int x1, int L, int wy0, int nwy_s, bool e) {
+            if (x1 <= x0 || nwy_s <= 0) return;
             if (q.nseg == 4) { ok = false; return; }
             const int start = q.nseg ? q.end[q.nseg - 1] : 0;
-            q.wx0[q.nseg] = wx0; q.wy0[q.nseg] = wy0; q.nwy[q.nseg] = nwy_s; q.lx[q.nseg] = lxs;
-            q.sexc[q.nseg] = e ? 1 : 0;
-            q.end[q.nseg] = start + nwx_s * nwy_s;
-            ++q.nseg;
-        };
-        // runs of chunks [c0, c1] of length L, one rectangle per run of equal excitation
-        auto runs = [&](int c0, int c1, int L, int wy0, int nwy_s, int cls, bool split) {
-            for (int w = c0; w <= c1;) {
-                const bool e = !split || meets(w * L, (w + 1) * L);
-                int w1 = w;
-                while (w1 + 1 <= c1 && (!split || meets((w1 + 1) * L, (w1 + 2) * L)) == e) ++w1;
-                add(pl.cls[cls], w, w1 - w + 1, wy0, nwy_s, L, e);
-                w = w1 + 1;
-            }
+            const int g = q.nseg++;
+            q.x0[g] = x0; q.x1[g] = x1; q.lx[g] = L;
+            q.wy0[g] = wy0; q.nwy[g] = nwy_s; q.sexc[g] = e ? 1 : 0;
+            q.end[g] = start + (x1 - x0 + L - 1) / L * nwy_s;
         };
         auto finish = [&]() {
             for (TBClass &q : pl.cls) {
@@ -1815,30 +1821,25 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
             }
             return ok;
         };
+        // a slab class over [u0, u1) and rows [wy0, wy0 + nwy_s): one rectangle per run
+        auto slab = [&](int cls, int u0, int u1, int L, int wy0, int nwy_s) {
+            if (!split_slabs) { add(pl.cls[cls], u0, u1, L, wy0, nwy_s, true); return; }
+            for (auto &r : runs_of(u0, u1))
+                add(pl.cls[cls], std::get<0>(r), std::get<1>(r), L, wy0, nwy_s, std::get<2>(r));
+        };
         if (!pl.interior) {
-            const int L = lx / div(4);
-            add(pl.cls[2], 0, (Nx + L - 1) / L, 0, nwy, L, true);
+            add(pl.cls[2], 0, Nx, lx / div(4), 0, nwy, true);
             return finish();
         }
         const int ni = iy1 - iy0 + 1;
-        const int ke = div(4);
-        for (int wx = ix0; wx <= ix1;) {                 // slab-free chunks: classes 0 / 3
-            const bool e = meets(wx * lx, (wx + 1) * lx);
-            int w1 = wx;
-            while (w1 + 1 <= ix1 && meets((w1 + 1) * lx, (w1 + 2) * lx) == e) ++w1;
-            if (e) add(pl.cls[3], wx * ke, (w1 - wx + 1) * ke, iy0, ni, lx / ke, true);
-            else add(pl.cls[0], wx, w1 - wx + 1, iy0, ni, lx, false);
-            wx = w1 + 1;
+        for (auto &r : runs_of(A, B)) {                  // slab-free chunks: classes 0 / 3
+            if (std::get<2>(r)) add(pl.cls[3], std::get<0>(r), std::get<1>(r), lx / div(4), iy0, ni, true);
+            else add(pl.cls[0], std::get<0>(r), std::get<1>(r), lx, iy0, ni, false);
         }
-        const int k = div(4);                            // y-slab rows: class 1
-        const int rx0 = ix0 * k, rx1 = (ix1 + 1) * k - 1;
-        runs(rx0, rx1, lx / k, 0, iy0, 1, split_slabs);
-        runs(rx0, rx1, lx / k, iy1 + 1, nwy - 1 - iy1, 1, split_slabs);
-        const int kc = div(16);                          // x-slab chunks: class 2
-        const int L = lx / kc;
-        const int cx0 = ix0 * kc, cx1 = (ix1 + 1) * kc - 1, nwxc = (Nx + L - 1) / L;
-        runs(0, cx0 - 1, L, 0, nwy, 2, split_slabs);
-        runs(cx1 + 1, nwxc - 1, L, 0, nwy, 2, split_slabs);
+        slab(1, A, B, lx / div(4), 0, iy0);              // y-slab rows: class 1
+        slab(1, A, B, lx / div(4), iy1 + 1, nwy - 1 - iy1);
+        slab(2, 0, A, lx / div(16), 0, nwy);             // x-slab chunks: class 2
+        slab(2, B, Nx, lx / div(16), 0, nwy);
         return finish();
     };
     TBPlan pl;
@@ -1849,12 +1850,12 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
 template <typename A>
 void set_class(A &q, const TBClass &k) {
     q.nseg = k.nseg;
-    for (int u = 0; u < 4; ++u) {
-        q.seg_wx0[u] = k.wx0[u]; q.seg_wy0[u] = k.wy0[u]; q.seg_nwy[u] = k.nwy[u]; q.seg_end[u] = k.end[u];
-        q.seg_lx[u] = k.lx[u];
-    }
     q.seg_exc = 0u;
-    for (int u = 0; u < k.nseg; ++u) q.seg_exc |= (unsigned)(k.sexc[u] != 0) << u;
+    for (int u = 0; u < 4; ++u) {
+        q.seg_x0[u] = k.x0[u]; q.seg_x1[u] = k.x1[u]; q.seg_lx[u] = k.lx[u];
+        q.seg_wy0[u] = k.wy0[u]; q.seg_nwy[u] = k.nwy[u]; q.seg_end[u] = k.end[u];
+        if (u < k.nseg) q.seg_exc |= (unsigned)(k.sexc[u] != 0) << u;
+    }
 }
 
 int ensure_side_streams(lf_ctx *c) {

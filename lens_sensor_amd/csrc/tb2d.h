@@ -30,11 +30,11 @@ struct TBTMArgs {
     Strip s0[4], s1[4];       // Hzx Hzy Exy Eyx: read / write buffers
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
     // wavefronts of this launch: up to 4 rectangles of (x chunk, y wave) indices; wave
-    // wv of segment g (seg_end[g-1] <= wv < seg_end[g]) is
-    // (seg_wx0 + (wv - start) / seg_nwy, seg_wy0 + (wv - start) % seg_nwy), its x chunk
-    // [wx * seg_lx, (wx + 1) * seg_lx)
+    // wv of segment g (seg_end[g-1] <= wv < seg_end[g], loc = wv - seg_end[g-1]) owns
+    // the x chunk [X0, min(X0 + seg_lx, seg_x1)) with X0 = seg_x0 + (loc / seg_nwy) *
+    // seg_lx, and the y wave seg_wy0 + loc % seg_nwy
     int nseg;
-    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
+    int seg_x0[4], seg_x1[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
     unsigned seg_exc;         // bit g: segment g needs the body with excitation code
     StepArgs st[kMaxTB];
 };
@@ -349,11 +349,11 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     int g = 0;
     while (g < a.nseg - 1 && wv >= a.seg_end[g]) ++g;
     const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
-    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
+    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g];
     const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
-    const int X0 = wx * a.seg_lx[g];
-    if (X0 >= Nx) return;                          // whole wavefront leaves together
-    const int X1 = min(X0 + a.seg_lx[g], Nx);
+    const int X0 = a.seg_x0[g] + (loc / a.seg_nwy[g]) * a.seg_lx[g];
+    if (X0 >= a.seg_x1[g]) return;                 // whole wavefront leaves together
+    const int X1 = min(X0 + a.seg_lx[g], a.seg_x1[g]);
     const int j = wy * (64 - 2 * T) - T + lane;
     // the host launches a MODE only on wavefronts whose loaded lanes (YIN) / planes (XIN)
     // lie in 1..Ny-2 / 1..Nx-2 and outside every slab, so those tests are constant
@@ -607,7 +607,7 @@ struct TBTEArgs {
     Strip s0[4], s1[4];       // Hxy Hyx Ezx Ezy: read / write buffers
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
     int nseg;                 // wavefront rectangles, as in TBTMArgs
-    int seg_wx0[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
+    int seg_x0[4], seg_x1[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
     unsigned seg_exc;
     StepArgs st[kMaxTB];
 };
@@ -693,11 +693,11 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
     int g = 0;
     while (g < a.nseg - 1 && wv >= a.seg_end[g]) ++g;
     const int loc = wv - (g > 0 ? a.seg_end[g - 1] : 0);
-    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g], wx = a.seg_wx0[g] + loc / a.seg_nwy[g];
+    const int wy = a.seg_wy0[g] + loc % a.seg_nwy[g];
     const int Nx = a.g.nx, Ny = a.g.nz, P = a.g.P;
-    const int X0 = wx * a.seg_lx[g];
-    if (X0 >= Nx) return;
-    const int X1 = min(X0 + a.seg_lx[g], Nx);
+    const int X0 = a.seg_x0[g] + (loc / a.seg_nwy[g]) * a.seg_lx[g];
+    if (X0 >= a.seg_x1[g]) return;
+    const int X1 = min(X0 + a.seg_lx[g], a.seg_x1[g]);
     const int j = wy * (64 - 2 * T) - T + lane;
     const bool jin = YIN || (j >= 0 && j < Ny);
     const bool jhx = YIN || (jin && j < Ny - 1);   // Hx exists in this column

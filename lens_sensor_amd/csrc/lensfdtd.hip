@@ -1822,10 +1822,13 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
             return ok;
         };
         // a slab class over [u0, u1) and rows [wy0, wy0 + nwy_s): one rectangle per run
-        auto slab = [&](int cls, int u0, int u1, int L, int wy0, int nwy_s) {
+        // (rectangles with excitation take chunks of Lx: their waves run the heavier body)
+        auto slab = [&](int cls, int u0, int u1, int L, int Lx, int wy0, int nwy_s) {
             if (!split_slabs) { add(pl.cls[cls], u0, u1, L, wy0, nwy_s, true); return; }
-            for (auto &r : runs_of(u0, u1))
-                add(pl.cls[cls], std::get<0>(r), std::get<1>(r), L, wy0, nwy_s, std::get<2>(r));
+            for (auto &r : runs_of(u0, u1)) {
+                const bool e = std::get<2>(r);
+                add(pl.cls[cls], std::get<0>(r), std::get<1>(r), e ? Lx : L, wy0, nwy_s, e);
+            }
         };
         if (!pl.interior) {
             add(pl.cls[2], 0, Nx, lx / div(4), 0, nwy, true);
@@ -1836,10 +1839,10 @@ TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) {
             if (std::get<2>(r)) add(pl.cls[3], std::get<0>(r), std::get<1>(r), lx / div(4), iy0, ni, true);
             else add(pl.cls[0], std::get<0>(r), std::get<1>(r), lx, iy0, ni, false);
         }
-        slab(1, A, B, lx / div(4), 0, iy0);              // y-slab rows: class 1
-        slab(1, A, B, lx / div(4), iy1 + 1, nwy - 1 - iy1);
-        slab(2, 0, A, lx / div(16), 0, nwy);             // x-slab chunks: class 2
-        slab(2, B, Nx, lx / div(16), 0, nwy);
+        slab(1, A, B, lx / div(4), lx / div(16), 0, iy0);   // y-slab rows: class 1
+        slab(1, A, B, lx / div(4), lx / div(16), iy1 + 1, nwy - 1 - iy1);
+        slab(2, 0, A, lx / div(16), lx / div(16), 0, nwy);  // x-slab chunks: class 2
+        slab(2, B, Nx, lx / div(16), lx / div(16), 0, nwy);
         return finish();
     };
     TBPlan pl;

@@ -296,9 +296,10 @@ __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *
         }
     }
     if (bits >> 3) {
-#pragma unroll
-        for (int f = 0; f < kTBFlux; ++f) {
-            if (f >= m->nflux) break;
+        // a loop, not unrolled: four unrolled copies per level hold their plane indices
+        // and pointers in scalar registers and push the body over 150 VGPRs (the
+        // monitor block is read through a pointer; the loop copies nothing to scratch)
+        for (int f = 0; f < m->nflux; ++f) {
             if (((bits >> (3 + f)) & 1u) && q == m->fil[f] && own && j < m->fn2[f]) {
                 float *o = m->fst[f] + (long long)(2 * s) * m->fn2[f] + j;
                 o[0] = eyn;                                         // Ey(il, j)
@@ -325,9 +326,7 @@ __device__ __forceinline__ void te_mon_level(TBMonRegs<T, MON> &r, const TBMon *
         if ((bits >> 2) & 1u) r.s[0][s + 1] = r.s[0][s + 1] + ezn * ezn;
     }
     if (bits >> 3) {
-#pragma unroll
-        for (int f = 0; f < kTBFlux; ++f) {
-            if (f >= m->nflux) break;
+        for (int f = 0; f < m->nflux; ++f) {
             if (((bits >> (3 + f)) & 1u) && q == m->fil[f] && own && j < m->fn2[f]) {
                 float *o = m->fst[f] + (long long)(2 * s) * m->fn2[f] + j;
                 o[0] = ezn;                                         // Ez(il, j)
@@ -420,8 +419,12 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     }
     // planes are prefetched two iterations ahead: a wave's march is a serial chain, and
     // one plane of loads in flight left the waves waiting on memory latency
+    // (monitored passes carry their accumulators in registers as well: they keep one
+    // plane in flight, in one buffer)
+    constexpr int PFD = MON == 0 ? 2 : 1;
     TMPlane bufA = tm_load_plane<MODE>(a, ps, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
-    TMPlane bufB = tm_load_plane<MODE>(a, ps + 1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    TMPlane bufB{};
+    if constexpr (PFD == 2) bufB = tm_load_plane<MODE>(a, ps + 1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
     // iteration p takes plane p from the buffer loaded two iterations earlier and
     // reloads it with plane p + 2; the loop runs two iterations per trip so that the
     // two buffers alternate without register copies (a copy of a load in flight
@@ -440,7 +443,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = tm_load_plane<MODE>(a, p + 2, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+        nxt = tm_load_plane<MODE>(a, p + PFD, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
         // ---- levels
 #pragma unroll
         for (int s = 0; s < T; ++s) {
@@ -552,9 +555,13 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
         }
         if constexpr (MON != 0) mr.shift();
     };
-    for (int p = ps; p < pe; p += 2) {
-        iter(p, bufA);
-        if (p + 1 < pe) iter(p + 1, bufB);
+    if constexpr (PFD == 2) {
+        for (int p = ps; p < pe; p += 2) {
+            iter(p, bufA);
+            if (p + 1 < pe) iter(p + 1, bufB);
+        }
+    } else {
+        for (int p = ps; p < pe; ++p) iter(p, bufA);
     }
 }
 
@@ -753,8 +760,10 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         mr.init(m);
         mr.load(m, ps, j, own, X0, X1);
     }
-    TEPlane bufA = te_load_plane<MODE>(a, ps, j, jin, jhx, sHxy, sEzy);      // two ahead, as TM
-    TEPlane bufB = te_load_plane<MODE>(a, ps + 1, j, jin, jhx, sHxy, sEzy);
+    constexpr int PFD = MON == 0 ? 2 : 1;                                   // as for TM
+    TEPlane bufA = te_load_plane<MODE>(a, ps, j, jin, jhx, sHxy, sEzy);
+    TEPlane bufB{};
+    if constexpr (PFD == 2) bufB = te_load_plane<MODE>(a, ps + 1, j, jin, jhx, sHxy, sEzy);
     // iteration p takes plane p from the buffer loaded two iterations earlier and
     // reloads it with plane p + 2; the loop runs two iterations per trip so that the
     // two buffers alternate without register copies (a copy of a load in flight
@@ -772,7 +781,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = te_load_plane<MODE>(a, p + 2, j, jin, jhx, sHxy, sEzy);
+        nxt = te_load_plane<MODE>(a, p + PFD, j, jin, jhx, sHxy, sEzy);
 #pragma unroll
         for (int s = 0; s < T; ++s) {
             const int q = p - 1 - s;
@@ -871,9 +880,13 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         }
         if constexpr (MON != 0) mr.shift();
     };
-    for (int p = ps; p < pe; p += 2) {
-        iter(p, bufA);
-        if (p + 1 < pe) iter(p + 1, bufB);
+    if constexpr (PFD == 2) {
+        for (int p = ps; p < pe; p += 2) {
+            iter(p, bufA);
+            if (p + 1 < pe) iter(p + 1, bufB);
+        }
+    } else {
+        for (int p = ps; p < pe; ++p) iter(p, bufA);
     }
 }
 

@@ -281,6 +281,13 @@ def main():
         print('bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not '
               'part of it', file=sys.stderr)
         sys.exit(2)
+    if args.workload in ('sweep', 'serve'):
+        # the 64-instance batched sweep faulted the device (illegal address, reported at a
+        # batch-pass staging sync) with the current blocked-pass kernels; its cause is not
+        # found yet, so the batched workloads are not run until it is
+        print(f'bench.py: --workload {args.workload} is disabled: its batched passes faulted the '
+              'device at full size and the cause is still open', file=sys.stderr)
+        sys.exit(2)
     from lens_sensor_amd.engine import hip
     dev = hip.device_name()
     if args.workload == 'serve':

@@ -1,0 +1,78 @@
+// CPU check of the wavefront plan of the temporally blocked 2D passes (tb2d_plan.h).
+// Reads cases "nx ny xlo xhi ylo yhi src_ix drv_n" from stdin and, for depths 2..8 and
+// several x chunks, checks that
+//   - every owned cell of the grid belongs to exactly one wavefront of one class,
+//   - every chunk lies in the grid and every rectangle's wave count matches its chunks,
+//   - the slab-free bodies (MODE 0) load only planes and lanes inside 1..N-2 and outside
+//     the slabs, the y-slab body (MODE 1) only such planes,
+//   - a rectangle without excitation code has no source or drive plane within its marches.
+// Prints one line per problem (at most a few per plan) and the totals.
+#include <cstdio>
+#include <vector>
+
+#include "../lens_sensor_amd/csrc/tb2d_plan.h"
+
+using namespace lfd;
+
+static int check(const TBPlanIn &in, int T, int lx) {
+    const TBPlan pl = tb_plan_2d(in, T, lx);
+    const int Nx = in.nx, Ny = in.ny, W = 64 - 2 * T;
+    std::vector<unsigned char> cov((size_t)Nx * Ny, 0);
+    int bad = 0;
+    auto err = [&](const char *m, int k, int g, int a, int b) {
+        if (bad++ < 3)
+            std::printf("BAD %s: %dx%d slabs x[%d,%d) y[%d,%d) src %d drv %d T %d lx %d class %d rect %d (%d, %d)\n",
+                        m, Nx, Ny, in.xlo, in.xhi, in.ylo, in.yhi, in.src_ix, in.drv_n, T, lx, k, g, a, b);
+    };
+    for (int k = 0; k < kTBClasses; ++k) {
+        const TBClass &q = pl.cls[k];
+        if (q.nseg < 0 || q.nseg > 4) { err("rectangles", k, -1, q.nseg, 0); continue; }
+        int prev = 0;
+        for (int g = 0; g < q.nseg; ++g) {
+            const int nw = q.end[g] - prev;
+            prev = q.end[g];
+            if (q.x0[g] < 0 || q.x1[g] > Nx || q.x1[g] <= q.x0[g] || q.lx[g] < 1) { err("range", k, g, q.x0[g], q.x1[g]); continue; }
+            if (nw != (q.x1[g] - q.x0[g] + q.lx[g] - 1) / q.lx[g] * q.nwy[g]) err("count", k, g, nw, q.nwy[g]);
+            const bool exc = q.exc == 1 || (q.exc == 2 && q.sexc[g]);
+            for (int loc = 0; loc < nw; ++loc) {
+                const int wy = q.wy0[g] + loc % q.nwy[g];
+                const int X0 = q.x0[g] + (loc / q.nwy[g]) * q.lx[g];
+                const int X1 = std::min(X0 + q.lx[g], q.x1[g]);
+                const int j0 = wy * W - T;
+                if ((q.mode == 0 || q.mode == 1) &&
+                    (X0 - T < std::max(in.xlo, 1) || X1 + T - 1 > std::min(in.xhi, Nx - 2)))
+                    err("x interior", k, g, X0, X1);
+                if (q.mode == 0 && (j0 < std::max(in.ylo, 1) || j0 + 63 > std::min(in.yhi, Ny - 2)))
+                    err("y interior", k, g, X0, wy);
+                if (!exc) {
+                    const int m0 = X0 - T, m1 = X1 + T;
+                    if (in.src_ix >= 0 && in.src_ix < m1 && in.src_ix + 2 > m0) err("source", k, g, X0, X1);
+                    if (in.drv_n > 0 && m1 > 0 && in.drv_n > m0) err("drive", k, g, X0, X1);
+                }
+                for (int x = X0; x < X1; ++x)
+                    for (int l = T; l <= 63 - T; ++l) {
+                        const int j = j0 + l;
+                        if (j >= 0 && j < Ny) cov[(size_t)x * Ny + j]++;
+                    }
+            }
+        }
+    }
+    for (size_t i = 0; i < cov.size(); ++i)
+        if (cov[i] != 1) { err(cov[i] ? "covered twice" : "not covered", -1, -1, (int)(i / Ny), (int)(i % Ny)); break; }
+    return bad;
+}
+
+int main() {
+    TBPlanIn in{};
+    int total = 0, n = 0;
+    while (std::scanf("%d %d %d %d %d %d %d %d", &in.nx, &in.ny, &in.xlo, &in.xhi, &in.ylo, &in.yhi,
+                      &in.src_ix, &in.drv_n) == 8) {
+        for (int T : {2, 3, 4, 6, 8})
+            for (int lx : {3, 5, 16, 32, 64, 128}) {
+                total += check(in, T, lx);
+                ++n;
+            }
+    }
+    std::printf("%d plans, %d problems\n", n, total);
+    return total != 0;
+}

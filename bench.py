@@ -37,7 +37,7 @@ BYTES_YEE3D = 6 * 8 + 3 * 4           # 6 fields r/w + ce_x, ce_y, ce_z = 60 B
 # HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), scaled by the same counters on a plain 1 GiB copy
 # (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth).
-TRAFFIC_BYTES = {('tm2d_stitch', 8): 727.0e6}
+TRAFFIC_BYTES = {('tm2d_stitch', 8): 724.2e6}
 
 
 def _timed_factory(records):

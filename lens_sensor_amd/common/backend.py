@@ -90,14 +90,15 @@ AUTOTUNE = os.environ.get('LENSFDTD_AUTOTUNE', '1') != '0'
 def tune(eng, row, nsteps, cache=None, key=None):
     """Tune ``eng`` with the step record ``row`` when the run is long enough and the engine
     supports it.  ``cache`` (a dict) with ``key`` reuses one tuning for engines of the same
-    shape and setup, e.g. the windows of a stitch run.  Returns (depth, lx) or None."""
+    shape and setup, e.g. the windows of a stitch run.  Returns (depth, lx, lead) or None."""
     if not AUTOTUNE or nsteps < AUTOTUNE_MIN_STEPS or not hasattr(eng, 'autotune'):
         return None
     if cache is not None and key in cache:
-        depth, lx = cache[key]
+        depth, lx, lead = cache[key]
         eng.set_temporal(depth, lx)
-        return depth, lx
-    depth, lx, _ms = eng.autotune(row)
+        eng.set_tb_lead(lead)
+        return depth, lx, lead
+    depth, lx, lead, _ms = eng.autotune(row)
     if cache is not None:
-        cache[key] = (depth, lx)
-    return depth, lx
+        cache[key] = (depth, lx, lead)
+    return depth, lx, lead

@@ -1179,6 +1179,12 @@ struct lf_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side[3] = {};            // side streams (default priority) for slab launches
     hipEvent_t ev_fork = nullptr, ev_join[3] = {};
+    hipEvent_t ev_lead = nullptr;        // context stream, after the lead class of a 2D pass
+    // 2D passes in a row: the side streams of the next pass wait on the previous pass's
+    // events directly (one hop) instead of on a join-then-fork through the context stream.
+    // Valid only while nothing but passes has been queued since the last pass.
+    bool tb_chain = false;
+    unsigned tb_chain_sides = 0;         // side streams that recorded ev_join in that pass
     float *arr[LF_NARRAYS] = {};
     float *alt[LF_NARRAYS] = {};         // second buffer of ping-pong fields (fused path)
     int variant = 1;                     // 0: single-pass kernels where available, 1: H then E
@@ -1199,6 +1205,7 @@ struct lf_ctx {
     int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
     int tb_lx = 64;                      // x planes per wavefront chunk of the blocked pass
     int tb_mon = 1;                      // 2D: depth of the passes that sum CW-window monitors
+    int tb_lead = 0;                     // 2D: pass class launched first (0 slab-free .. 3)
     float *tbm_stage = nullptr;          // flux-plane staging of those passes
     double last_tb_ms = 0;               // blocked launches of the last lf_run: time, count
     int last_tb_launches = 0;
@@ -1458,6 +1465,7 @@ int lf_destroy(lf_ctx *c) {
     for (auto e : c->ev) (void)hipEventDestroy(e);
     for (auto st : c->side) if (st) (void)hipStreamSynchronize(st);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_lead) (void)hipEventDestroy(c->ev_lead);
     for (auto e : c->ev_join) if (e) (void)hipEventDestroy(e);
     for (auto st : c->side) if (st) (void)hipStreamDestroy(st);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1659,6 +1667,7 @@ int ensure_alt(lf_ctx *c, int id) {
     if (c->alt[id]) return 0;
     LF_HIP(hipMalloc(&c->alt[id], grid_elems(c) * sizeof(float)));
     LF_HIP(hipMemsetAsync(c->alt[id], 0, grid_elems(c) * sizeof(float), c->stream));
+    c->tb_chain = false;
     return 0;
 }
 
@@ -1676,6 +1685,7 @@ int tm_fused_step(lf_ctx *c, const StepArgs &st, int fp64) {
             const size_t n = strip_elems_2d(c, c->s2[q], q == 0);
             LF_HIP(hipMalloc(&c->psi_alt[q], n * sizeof(float)));
             LF_HIP(hipMemsetAsync(c->psi_alt[q], 0, n * sizeof(float), c->stream));
+            c->tb_chain = false;
         }
     TMFArgs f{};
     f.g = c->g;
@@ -1749,31 +1759,65 @@ int ensure_side_streams(lf_ctx *c) {
         if (!c->side[u]) LF_HIP(hipStreamCreate(&c->side[u]));
     if (!c->ev_fork) {
         LF_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        LF_HIP(hipEventCreateWithFlags(&c->ev_lead, hipEventDisableTiming));
         for (int u = 0; u < 3; ++u) LF_HIP(hipEventCreateWithFlags(&c->ev_join[u], hipEventDisableTiming));
     }
     return 0;
 }
 
-// Fork the classes of one 2D pass over the context stream (class 0) and the side streams
-// (classes 1, 2, 3), concurrently: disjoint output cells, step-n inputs.  `one` launches
-// a class on a stream.
+// Fork the classes of one 2D pass over the context stream and the three side streams,
+// concurrently: disjoint output cells, step-n inputs.  `one` launches a class on a stream.
+// Class `lead` goes on the context stream, ahead of the fork event the side streams wait
+// for, so its waves are dispatched first.  Which order finishes soonest depends on the
+// window: with an x-slab class of few long 256-VGPR waves it pays to start that class
+// first (started behind the slab-free waves it found no room on the SIMDs until they
+// drained), with a stitch drive over a quarter of the grid the drive classes are the
+// long pole instead; lf_autotune measures the choice.
+//
+// Every class of a pass reads what every class of the previous pass wrote, so a pass waits
+// for all of the previous one.  After a plain fork the side streams wait on the context
+// stream, which waited on the side streams of the previous pass: two cross-queue hops
+// per pass (about 10 us each).  When the previous queued work was a pass (tb_chain), each
+// side stream instead waits on the previous pass's lead-class event and on the other side
+// streams' events, one hop; its own stream orders its own previous class.
 template <typename F>
-int fork_tb_classes(lf_ctx *c, const TBPlan &pl, F one) {
+int fork_tb_classes(lf_ctx *c, const TBPlan &pl, int lead, F one) {
     if (!pl.interior) {
         one(pl.cls[2], c->stream);
+        c->tb_chain = false;
         return 0;
     }
     if (int e = ensure_side_streams(c)) return e;
-    LF_HIP(hipEventRecord(c->ev_fork, c->stream));
-    for (int u = 0; u < 3; ++u)
-        if (pl.cls[u + 1].nseg > 0) LF_HIP(hipStreamWaitEvent(c->side[u], c->ev_fork, 0));
-    for (int u = 0; u < 3; ++u) one(pl.cls[u + 1], c->side[u]);
-    one(pl.cls[0], c->stream);
-    for (int u = 0; u < 3; ++u) {
-        if (pl.cls[u + 1].nseg == 0) continue;
-        LF_HIP(hipEventRecord(c->ev_join[u], c->side[u]));
-        LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[u], 0));
+    if (lead < 0 || lead >= kTBClasses || pl.cls[lead].nseg == 0) lead = 0;
+    int side_of[kTBClasses];
+    for (int u = 0, n = 0; u < kTBClasses; ++u) side_of[u] = u == lead ? -1 : n++;
+    const bool chain = c->tb_chain;
+    if (!chain) LF_HIP(hipEventRecord(c->ev_fork, c->stream));
+    for (int u = 0; u < kTBClasses; ++u) {
+        const int sd = side_of[u];
+        if (sd < 0 || pl.cls[u].nseg == 0) continue;
+        if (!chain) {
+            LF_HIP(hipStreamWaitEvent(c->side[sd], c->ev_fork, 0));
+            continue;
+        }
+        LF_HIP(hipStreamWaitEvent(c->side[sd], c->ev_lead, 0));
+        for (int v = 0; v < 3; ++v)
+            if (v != sd && ((c->tb_chain_sides >> v) & 1u))
+                LF_HIP(hipStreamWaitEvent(c->side[sd], c->ev_join[v], 0));
     }
+    one(pl.cls[lead], c->stream);
+    LF_HIP(hipEventRecord(c->ev_lead, c->stream));
+    for (int u = 0; u < kTBClasses; ++u)
+        if (side_of[u] >= 0) one(pl.cls[u], c->side[side_of[u]]);
+    unsigned sides = 0;
+    for (int u = 0; u < kTBClasses; ++u) {
+        if (side_of[u] < 0 || pl.cls[u].nseg == 0) continue;
+        LF_HIP(hipEventRecord(c->ev_join[side_of[u]], c->side[side_of[u]]));
+        LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[side_of[u]], 0));
+        sides |= 1u << side_of[u];
+    }
+    c->tb_chain = true;
+    c->tb_chain_sides = sides;
     return 0;
 }
 
@@ -1797,7 +1841,7 @@ int launch_tb_pass(lf_ctx *c, A &f, const TBPlan &pl, int T, int fp64) {
         set_class(q, k);
         launch_tb_mode(q, T, k.mode, k.exc, c->ex, fp64, st);
     };
-    return fork_tb_classes(c, pl, one);
+    return fork_tb_classes(c, pl, c->tb_lead, one);
 }
 
 int ensure_psi_alt_2d(lf_ctx *c) {
@@ -1806,6 +1850,7 @@ int ensure_psi_alt_2d(lf_ctx *c) {
             const size_t n = strip_elems_2d(c, c->s2[q], x_slab_2d(c->kind, q));
             LF_HIP(hipMalloc(&c->psi_alt[q], n * sizeof(float)));
             LF_HIP(hipMemsetAsync(c->psi_alt[q], 0, n * sizeof(float), c->stream));
+            c->tb_chain = false;
         }
     return 0;
 }
@@ -2010,6 +2055,7 @@ int tb_chunk_2d_mon(lf_ctx *c, const lf_step *steps, int T, TBMonPlan &mp, int f
         const size_t n = (size_t)kTBFlux * kMaxTB * 2 * c->g.P;
         LF_HIP(hipMalloc(&c->tbm_stage, n * sizeof(float)));
         LF_HIP(hipMemsetAsync(c->tbm_stage, 0, n * sizeof(float), c->stream));
+        c->tb_chain = false;
         for (int f = 0; f < mp.nflux; ++f) mp.m.fst[f] = c->tbm_stage + (size_t)f * kMaxTB * 2 * c->g.P;
     }
     const TBPlan pl = tb_plan_2d(c, T, c->tb_lx);
@@ -2022,7 +2068,7 @@ int tb_chunk_2d_mon(lf_ctx *c, const lf_step *steps, int T, TBMonPlan &mp, int f
             const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
             launch(T, k.mode, k.exc != 0, mp.mon, q, c->ex, mp.m, fp64, st, nb);
         };
-        return fork_tb_classes(c, pl, one);
+        return fork_tb_classes(c, pl, 0, one);
     };
     if (c->kind == LF_TM2D) {
         TBTMArgs f = tm_tb_args(c);
@@ -2081,6 +2127,14 @@ extern "C" int lf_set_temporal(lf_ctx *c, int depth, int lx) {
     if (lx < 1 || lx > 1 << 20) return fail(-1, "lf_set_temporal: lx out of range");
     c->tb = depth;
     c->tb_lx = lx;
+    return 0;
+}
+
+extern "C" int lf_set_tb_lead(lf_ctx *c, int cls) {
+    if (int e = check_ctx(c)) return e;
+    if (cls < 0 || cls > 3) return fail(-1, "lf_set_tb_lead: class must be 0, 1, 2 or 3");
+    if (c->kind == LF_YEE3D && cls != 0) return fail(-1, "lf_set_tb_lead: 3D passes take class 0 only");
+    c->tb_lead = cls;
     return 0;
 }
 
@@ -2792,6 +2846,7 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
     else if (c->kind == LF_TM2D) tm = tm_args(c);
     else yee = yee_args(c);
     MonArgs ma = mon_args(c);
+    c->tb_chain = false;                 // work queued since the last pass (see fork_tb_classes)
     const bool timing = c->timing;
     if (timing) {
         while ((int)c->ev.size() < 2 * nsteps + 2) {
@@ -2838,6 +2893,7 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
                 ma = mon_args(c);
                 s += T - 1;
                 const lf_step &tl = steps[s];
+                if ((tl.mask & ~fused) != 0) c->tb_chain = false;
                 if (int e = launch_monitors(c, ma, tl, step_args(tl), tl.mask & ~fused)) return e;
                 continue;
             }
@@ -2848,6 +2904,7 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         if (c->kind != LF_YEE3D && Tm > 1 && s + Tm <= nsteps && tb_mon_plan(c, steps + s, Tm, mp)) {
             if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
             if (int e = tb_chunk_2d_mon(c, steps + s, Tm, mp, fp64)) return e;
+            c->tb_chain = false;         // the flux additions follow the pass on the context stream
             if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
             ++ngroup;
             if (c->kind == LF_TM2D) tm = tm_args(c);
@@ -2856,6 +2913,7 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
             s += Tm - 1;
             continue;
         }
+        c->tb_chain = false;
         if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
         if (c->kind == LF_TE2D) {
             te_h_kernel<<<grd2, blk, 0, c->stream>>>(te);
@@ -2958,14 +3016,15 @@ int snapshot_restore(lf_ctx *c, Snapshot &sn) {
 
 }  // namespace
 
-// Picks the temporal-block depth and x chunk with the shortest measured time per step on
-// this context (its grid, CPML slabs, source, drive and fused detector boxes): every
-// candidate runs a warm-up pass and `reps` timed passes of monitor-free copies of `row`,
-// then every field and psi slab is restored, so the simulation state is unchanged.  The
-// blocked passes give bit-identical results for every depth and chunk, so the choice
-// affects only speed.  No counterpart in the reference (its step loop has no blocking).
+// Picks the temporal-block depth and x chunk (then, in 2D, the pass class launched first)
+// with the shortest measured time per step on this context (its grid, CPML slabs, source,
+// drive and fused detector boxes): every candidate runs a warm-up pass and `reps` timed
+// passes of monitor-free copies of `row`, then every field and psi slab is restored, so
+// the simulation state is unchanged.  The blocked passes give bit-identical results for
+// every depth, chunk and launch order, so the choice affects only speed.  No counterpart
+// in the reference (its step loop has no blocking).
 extern "C" int lf_autotune(lf_ctx *c, const lf_step *row, int reps, int *depth, int *lx,
-                           double *ms_per_step) {
+                           int *lead, double *ms_per_step) {
     if (int e = check_ctx(c)) return e;
     if (!c->finalized) return fail(-1, "lf_autotune before lf_finalize");
     if (!row) return fail(-1, "lf_autotune: null step row");
@@ -2973,7 +3032,7 @@ extern "C" int lf_autotune(lf_ctx *c, const lf_step *row, int reps, int *depth, 
     std::vector<int> Ts, lxs;
     if (c->kind == LF_YEE3D) { Ts = {2, 3}; lxs = {32, 48, 64, 96}; }
     else { Ts = {2, 4, 6, 8}; lxs = {32, 64, 128}; }
-    const int tb0 = c->tb, lx0 = c->tb_lx;
+    const int tb0 = c->tb, lx0 = c->tb_lx, lead0 = c->tb_lead;
     const bool timing0 = c->timing;
     c->timing = false;
     Snapshot sn;
@@ -2981,37 +3040,51 @@ extern "C" int lf_autotune(lf_ctx *c, const lf_step *row, int reps, int *depth, 
     hipEvent_t e0, e1;
     LF_HIP(hipEventCreate(&e0));
     LF_HIP(hipEventCreate(&e1));
-    int bestT = tb0, bestLx = lx0;
+    int bestT = tb0, bestLx = lx0, bestLead = 0;
     double best = 1e30;
+    auto trial = [&](int T, int L, int ld, double &per) -> int {
+        std::vector<lf_step> tab((size_t)T * (reps + 1), *row);
+        for (auto &t : tab) t.mask = 0;
+        c->tb = T;
+        c->tb_lx = L;
+        c->tb_lead = ld;
+        if (int r = run_impl(c, tab.data(), T, true)) return r;          // warm-up pass
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(-2, "lf_autotune: event");
+        if (int r = run_impl(c, tab.data() + T, T * reps, true)) return r;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return fail(-2, "lf_autotune: event");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        per = ms / (double)(T * reps);
+        return 0;
+    };
     int rc = 0;
     for (int T : Ts) {
         for (int L : lxs) {
-            std::vector<lf_step> tab((size_t)T * (reps + 1), *row);
-            for (auto &t : tab) t.mask = 0;
-            c->tb = T;
-            c->tb_lx = L;
-            if ((rc = run_impl(c, tab.data(), T, true)) != 0) break;        // warm-up pass
-            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(-2, "lf_autotune: event"); break; }
-            if ((rc = run_impl(c, tab.data() + T, T * reps, true)) != 0) break;
-            if (hipEventRecord(e1, c->stream) != hipSuccess ||
-                hipEventSynchronize(e1) != hipSuccess) { rc = fail(-2, "lf_autotune: event"); break; }
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double per = ms / (double)(T * reps);
+            double per = 0;
+            if ((rc = trial(T, L, 0, per)) != 0) break;
             if (per < best) { best = per; bestT = T; bestLx = L; }
         }
         if (rc) break;
+    }
+    // 2D: the launch order of the pass classes at the chosen depth and chunk
+    for (int ld = 1; rc == 0 && c->kind != LF_YEE3D && bestT > 1 && ld < kTBClasses; ++ld) {
+        double per = 0;
+        if ((rc = trial(bestT, bestLx, ld, per)) != 0) break;
+        if (per < best) { best = per; bestLead = ld; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     const int rr = snapshot_restore(c, sn);
     c->timing = timing0;
-    if (rc) { c->tb = tb0; c->tb_lx = lx0; return rc; }
+    if (rc) { c->tb = tb0; c->tb_lx = lx0; c->tb_lead = lead0; return rc; }
     if (rr) return rr;
     c->tb = bestT;
     c->tb_lx = bestLx;
+    c->tb_lead = bestLead;
     if (depth) *depth = bestT;
     if (lx) *lx = bestLx;
+    if (lead) *lead = bestLead;
     if (ms_per_step) *ms_per_step = best;
     return 0;
 }

@@ -6,6 +6,18 @@
 
 namespace lfd {
 
+// d / k of a CPML slab term.  The profiles' kappa is 1 wherever kappa_max is 1 (the
+// default), and d / 1 is d exactly, so the division runs only where kappa differs from 1:
+// for the x profiles the test is wave-uniform (a scalar branch), for the y profiles the
+// division runs under the lanes that need it.
+__device__ __forceinline__ float div_kappa(float d, float k) {
+    float t;
+    if (k == 1.0f) t = d;
+    else t = d / k;
+    return t;
+}
+
+
 // ---------------------------------------------------------------------------- temporal blocking (2D TM)
 // T time steps per pass.  A wavefront holds 64 consecutive y cells
 // j = wy*(64-2T) - T + lane and marches along its x chunk [X0, X1) from X0-T to X1+T-1.
@@ -371,15 +383,18 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     const int pe = X1 + T;
     // x profiles of the wavefront's planes, plane ps + lane in lane `lane`, read per level
     // with v_readlane (no dependent scalar loads inside the level chain)
-    const bool xpre = PX && pe - ps <= 64;
+    // (the plan keeps the chunks of the x-slab modes at most 64 - 2T planes long, so the
+    // marched planes fit the 64 lanes; a profile read from memory inside the level chain
+    // would make the level wait for every plane load in flight)
+    if (PX && pe - ps > 64) return;
     float xhb = 0.f, xhc = 0.f, xhk = 1.f, xeb = 0.f, xec = 0.f, xek = 1.f;
-    if (xpre) {
+    if (PX) {
         const int pl = ps + lane;
         if (pl < Nx - 1) { xhb = a.xh.b[pl]; xhc = a.xh.c[pl]; xhk = a.xh.k[pl]; }
         if (pl < Nx) { xeb = a.xe.b[pl]; xec = a.xe.c[pl]; xek = a.xe.k[pl]; }
     }
-    auto plane_val = [&](float reg, const float *arr, int q) {
-        return xpre ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps)) : arr[q];
+    auto plane_val = [&](float reg, int q) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps));
     };
 
     // slot L holds the output of level L-1 (slot 0: loaded plane); C = this iteration,
@@ -458,14 +473,14 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                     const float dExy = ex_jp - ex_o;
                     float tx, ty;
                     if (PX && strip_pos(a.s0[0], q) >= 0) {
-                        psx = plane_val(xhb, a.xh.b, q) * psx + plane_val(xhc, a.xh.c, q) * dEyx;
-                        tx = dEyx / plane_val(xhk, a.xh.k, q) + psx;
+                        psx = plane_val(xhb, q) * psx + plane_val(xhc, q) * dEyx;
+                        tx = div_kappa(dEyx, plane_val(xhk, q)) + psx;
                     } else {
                         tx = dEyx;
                     }
                     if (sHzy >= 0) {
                         psy = yhb * psy + yhc * dExy;
-                        ty = dExy / yhk + psy;
+                        ty = div_kappa(dExy, yhk) + psy;
                     } else {
                         ty = dExy;
                     }
@@ -482,7 +497,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                         float t;
                         if (sExy >= 0) {
                             qy = yeb * qy + yec * d;
-                            t = d / yek + qy;
+                            t = div_kappa(d, yek) + qy;
                         } else {
                             t = d;
                         }
@@ -500,8 +515,8 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                         const float d = hz - hzP[s + 1];            // Hz(q) - Hz(q-1)
                         float t;
                         if (PX && strip_pos(a.s0[3], q) >= 0) {
-                            qx = plane_val(xeb, a.xe.b, q) * qx + plane_val(xec, a.xe.c, q) * d;
-                            t = d / plane_val(xek, a.xe.k, q) + qx;
+                            qx = plane_val(xeb, q) * qx + plane_val(xec, q) * d;
+                            t = div_kappa(d, plane_val(xek, q)) + qx;
                         } else {
                             t = d;
                         }
@@ -570,8 +585,14 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
 // into a scalar register, so every x/plane test is a scalar branch.
 // EXC = false: no source or drive plane lies within the launch's marches, so the body
 // carries no excitation loads (whose waits would also drain the plane prefetch).
+// The slab-free body with excitation is held to 168 VGPRs (3 waves per SIMD instead of
+// 2 at its natural 170): its class carries the stitch drive over a quarter of a window
+// and is the long pole of those passes; the two values it then keeps in scratch are
+// read only on the source planes.
 template <int T, int MODE, bool EXC = true>
-__global__ void __launch_bounds__(256) tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 256),
+                               amdgpu_waves_per_eu(MODE == 0 && EXC ? 3 : 1)))
+tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
     if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     tm_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
@@ -717,15 +738,18 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
     if (sEzy >= 0) { yeb = a.ye.b[j]; yec = a.ye.c[j]; yek = a.ye.k[j]; }
     const int ps = X0 - T > 0 ? X0 - T : 0;
     const int pe = X1 + T;
-    const bool xpre = PX && pe - ps <= 64;
+    // (the plan keeps the chunks of the x-slab modes at most 64 - 2T planes long, so the
+    // marched planes fit the 64 lanes; a profile read from memory inside the level chain
+    // would make the level wait for every plane load in flight)
+    if (PX && pe - ps > 64) return;
     float xhb = 0.f, xhc = 0.f, xhk = 1.f, xeb = 0.f, xec = 0.f, xek = 1.f;
-    if (xpre) {
+    if (PX) {
         const int pl = ps + lane;
         if (pl < Nx - 1) { xhb = a.xh.b[pl]; xhc = a.xh.c[pl]; xhk = a.xh.k[pl]; }
         if (pl < Nx) { xeb = a.xe.b[pl]; xec = a.xe.c[pl]; xek = a.xe.k[pl]; }
     }
-    auto plane_val = [&](float reg, const float *arr, int q) {
-        return xpre ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps)) : arr[q];
+    auto plane_val = [&](float reg, int q) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(reg), q - ps));
     };
 
     float ezC[T + 1], hxC[T + 1], hyC[T + 1], ezP[T + 1], hxP[T + 1], hyP[T + 1];
@@ -795,7 +819,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                     float t;
                     if (sHxy >= 0) {
                         qxy = yhb * qxy + yhc * d;
-                        t = d / yhk + qxy;
+                        t = div_kappa(d, yhk) + qxy;
                     } else {
                         t = d;
                     }
@@ -805,8 +829,8 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                     const float d = ezC[s] - ez_o;
                     float t;
                     if (PX && strip_pos(a.s0[1], q) >= 0) {
-                        qyx = plane_val(xhb, a.xh.b, q) * qyx + plane_val(xhc, a.xh.c, q) * d;
-                        t = d / plane_val(xhk, a.xh.k, q) + qyx;
+                        qyx = plane_val(xhb, q) * qyx + plane_val(xhc, q) * d;
+                        t = div_kappa(d, plane_val(xhk, q)) + qyx;
                     } else {
                         t = d;
                     }
@@ -822,14 +846,14 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                         const float dHx = hx - hx_jm;
                         float t;
                         if (PX && strip_pos(a.s0[2], q) >= 0) {
-                            qzx = plane_val(xeb, a.xe.b, q) * qzx + plane_val(xec, a.xe.c, q) * dHy;
-                            t = dHy / plane_val(xek, a.xe.k, q) + qzx;
+                            qzx = plane_val(xeb, q) * qzx + plane_val(xec, q) * dHy;
+                            t = div_kappa(dHy, plane_val(xek, q)) + qzx;
                         } else {
                             t = dHy;
                         }
                         if (sEzy >= 0) {
                             qzy = yeb * qzy + yec * dHx;
-                            t = (t - dHx / yek) - qzy;
+                            t = (t - div_kappa(dHx, yek)) - qzy;
                         } else {
                             t = t - dHx;
                         }

@@ -31,7 +31,9 @@ struct TBPlanIn {
 //   class 0: MODE 0, slab-free chunks without excitation (chunks of lx)
 //   class 1: MODE 1, y-slab rows over [A, B) (chunks of lx/4)
 //   class 2: MODE 3, x-slab chunks over all rows (chunks of lx/16: the finest), or
-//            every wavefront of a grid without a slab-free rectangle (with excitation)
+//            every wavefront of a grid without a slab-free rectangle (with excitation);
+//            its chunks are at most 64 - 2T planes long, so that a wave's marched planes
+//            fit its 64 lanes (the x profiles are held one plane per lane)
 //   class 3: MODE 0, slab-free chunks with excitation (chunks of lx/4: their waves
 //            carry the excitation work, and few long marches would leave SIMDs idle)
 // The slab classes choose the body per rectangle when only some of their rectangles
@@ -127,7 +129,7 @@ inline TBPlan tb_plan_2d(const TBPlanIn &in, int T, int lx) {
             }
         };
         if (!pl.interior) {
-            add(pl.cls[2], 0, Nx, lx / div(4), 0, nwy, true);
+            add(pl.cls[2], 0, Nx, std::min(lx / div(4), W), 0, nwy, true);
             return finish();
         }
         const int ni = iy1 - iy0 + 1;
@@ -137,8 +139,9 @@ inline TBPlan tb_plan_2d(const TBPlanIn &in, int T, int lx) {
         }
         slab(1, A, B, lx / div(4), lx / div(16), 0, iy0);   // y-slab rows: class 1
         slab(1, A, B, lx / div(4), lx / div(16), iy1 + 1, nwy - 1 - iy1);
-        slab(2, 0, A, lx / div(16), lx / div(16), 0, nwy);  // x-slab chunks: class 2
-        slab(2, B, Nx, lx / div(16), lx / div(16), 0, nwy);
+        const int L3 = std::min(lx / div(16), W);           // x-slab chunks: class 2
+        slab(2, 0, A, L3, L3, 0, nwy);
+        slab(2, B, Nx, L3, L3, 0, nwy);
         return finish();
     };
     TBPlan pl;

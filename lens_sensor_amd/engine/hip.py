@@ -31,7 +31,7 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats',
            'lf_batch_create', 'lf_batch_add', 'lf_batch_set_temporal', 'lf_batch_set_timing',
            'lf_batch_run', 'lf_batch_stats', 'lf_batch_destroy', 'lf_batch_set_table',
-           'lf_batch_run_at', 'lf_autotune', 'lf_set_temporal_mon')
+           'lf_batch_run_at', 'lf_autotune', 'lf_set_temporal_mon', 'lf_set_tb_lead')
 
 
 def library_path():
@@ -77,7 +77,8 @@ def load():
         'lf_set_temporal': [P, I, I],
         'lf_set_temporal_mon': [P, I],
         'lf_tb_stats': [P, ctypes.POINTER(ctypes.c_double), ip, ip],
-        'lf_autotune': [P, P, I, ip, ip, ctypes.POINTER(ctypes.c_double)],
+        'lf_autotune': [P, P, I, ip, ip, ip, ctypes.POINTER(ctypes.c_double)],
+        'lf_set_tb_lead': [P, I],
         'lf_kernel_ms': [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ip],
         'lf_batch_create': [ctypes.POINTER(P)],
         'lf_batch_add': [P, P],
@@ -340,17 +341,23 @@ class HipEngine:
         """Depth of the 2D passes that also sum the CW-window monitors (1 = off)."""
         _check(self._lib.lf_set_temporal_mon(self.h, int(depth)), 'lf_set_temporal_mon')
 
+    def set_tb_lead(self, cls):
+        """Pass class dispatched first in the 2D blocked passes (0 slab-free, 1 y-slab rows,
+        2 x-slab planes, 3 excitation planes); a speed knob, results are identical."""
+        _check(self._lib.lf_set_tb_lead(self.h, int(cls)), 'lf_set_tb_lead')
+
     def autotune(self, row, reps=None):
-        """Time the candidate blocking depths / x chunks on this engine with monitor-free
-        copies of step row `row` (one lf_step record); keeps the fastest and restores the
-        fields.  Returns (depth, lx, ms per step)."""
+        """Time the candidate blocking depths / x chunks (2D: then the lead class) on this
+        engine with monitor-free copies of step row `row` (one lf_step record); keeps the
+        fastest and restores the fields.  Returns (depth, lx, lead, ms per step)."""
         row = np.ascontiguousarray(np.asarray(row, dtype=STEP_DTYPE).reshape(1))
-        d, lx, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        d, lx, ld, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         if reps is None:
             reps = 3 if self.kind == 'yee3d' else 8
         _check(self._lib.lf_autotune(self.h, row.ctypes.data, int(reps), ctypes.byref(d),
-                                     ctypes.byref(lx), ctypes.byref(ms)), 'lf_autotune')
-        return d.value, lx.value, ms.value
+                                     ctypes.byref(lx), ctypes.byref(ld), ctypes.byref(ms)),
+               'lf_autotune')
+        return d.value, lx.value, ld.value, ms.value
 
     def tb_stats(self):
         ms, n, d = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()

@@ -779,7 +779,8 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
         fill_phasors(tab, omega, dt, t_phys, cw_envelope(t_phys, sigma_t), src_omega=src.omega)
         win = ns >= steps_win - n_acc
         tab['mask'] = np.where(win, np.uint32(bits), np.uint32(0))
-        backend.tune(eng, tab[steps_win // 2], steps_win, cache=tuned, key=(Nxw, Ny))
+        backend.tune(eng, tab[steps_win // 2], steps_win, cache=tuned,
+                     key=(Nxw, Ny, w > 0 and Ehand is not None))
         acc_n = int(win.sum())
         book.note_samples(list(fpl), acc_n)
         tw = time.time()

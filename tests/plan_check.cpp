@@ -44,6 +44,8 @@ static int check(const TBPlanIn &in, int T, int lx) {
                     err("x interior", k, g, X0, X1);
                 if (q.mode == 0 && (j0 < std::max(in.ylo, 1) || j0 + 63 > std::min(in.yhi, Ny - 2)))
                     err("y interior", k, g, X0, wy);
+                if ((q.mode & 2) && X1 + T - std::max(X0 - T, 0) > 64)     // x profiles in lanes
+                    err("x-slab march longer than 64 planes", k, g, X0, X1);
                 if (!exc) {
                     const int m0 = X0 - T, m1 = X1 + T;
                     if (in.src_ix >= 0 && in.src_ix < m1 && in.src_ix + 2 > m0) err("source", k, g, X0, X1);
@@ -68,7 +70,7 @@ int main() {
     while (std::scanf("%d %d %d %d %d %d %d %d", &in.nx, &in.ny, &in.xlo, &in.xhi, &in.ylo, &in.yhi,
                       &in.src_ix, &in.drv_n) == 8) {
         for (int T : {2, 3, 4, 6, 8})
-            for (int lx : {3, 5, 16, 32, 64, 128}) {
+            for (int lx : {3, 5, 16, 32, 64, 128, 1024}) {
                 total += check(in, T, lx);
                 ++n;
             }

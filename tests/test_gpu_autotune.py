@@ -27,12 +27,51 @@ def test_autotune_keeps_the_state(kind, dims):
     tab = _table(48, 0, 48)
     a.run(tab[:12])
     b.run(tab[:12])
-    depth, lx, ms = a.autotune(tab[12], reps=2)
+    depth, lx, lead, ms = a.autotune(tab[12], reps=2)
     depths, lxs = CHOICES[kind]
     assert depth in depths and lx in lxs and ms > 0
+    assert lead in ((0,) if kind == 'yee3d' else (0, 1, 2, 3))
     b.set_temporal(depth, lx)
+    b.set_tb_lead(lead)
     a.run(tab[12:])
     b.run(tab[12:])
     _compare(kind, a, b, ma)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize('kind', ['tm2d', 'te2d'])
+def test_lead_class_is_bit_exact(kind):
+    """Every launch order of the pass classes gives the depth-1 result bit for bit (source
+    and stitch drive inside the grid, so all four classes have work)."""
+    from lens_sensor_amd.engine.hip import HipEngine
+    dims = (300, 200)
+    ref = HipEngine(kind, dims, 'fp64acc')
+    mr = _setup(ref, kind, dims, np.random.default_rng(12), source=True, drive=True, pec=False)
+    ref.set_temporal(1, 64)
+    tab = _table(40, 0, 40)
+    ref.run(tab)
+    for lead in range(4):
+        print(f'[lead class] {kind} lead={lead}', flush=True)
+        e = HipEngine(kind, dims, 'fp64acc')
+        _setup(e, kind, dims, np.random.default_rng(12), source=True, drive=True, pec=False)
+        e.set_temporal(8, 32)
+        e.set_tb_lead(lead)
+        e.run(tab)
+        _compare(kind, e, ref, mr)
+        e.close()
+    ref.close()
+
+
+def test_lead_class_setter():
+    from lens_sensor_amd.engine.hip import HipEngine
+    e = HipEngine('tm2d', (20, 20), 'ref32')
+    for bad in (-1, 4):
+        with pytest.raises(RuntimeError):
+            e.set_tb_lead(bad)
+    e.close()
+    e3 = HipEngine('yee3d', (12, 12, 12), 'ref32')
+    with pytest.raises(RuntimeError):
+        e3.set_tb_lead(2)
+    e3.set_tb_lead(0)
+    e3.close()

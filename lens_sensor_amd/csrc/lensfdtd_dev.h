@@ -99,6 +99,15 @@ __device__ __forceinline__ float lane_prev(float v) {          // value of lane 
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
                                                       0x138, 0xf, 0xf, false));
 }
+// The same shifts with the sourceless edge lane (63, resp. 0) set to 0 (bound_ctrl): the
+// DPP op writes a fresh register, with no copy of the source into it first.  For
+// wavefronts whose edge lanes are halo lanes, as in the temporally blocked passes.
+__device__ __forceinline__ float lane_next0(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_prev0(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
 
 // psi slot of a slab compressed along `axis`, 32-bit element offset (grids < 2^31)
 __device__ __forceinline__ unsigned psi_off32(const Strip &s, int axis, int pos, int i, int j, int k,

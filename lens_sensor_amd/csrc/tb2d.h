@@ -449,7 +449,10 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
         tm_mask_plane<MODE>(nxt, a, p, jin, jey, sHzy, sExy);
         exC[0] = nxt.ex; eyC[0] = nxt.ey; hzC[0] = nxt.hz;
         hxC[0] = nxt.ph; hyC[0] = nxt.py; eyyC[0] = nxt.qy; exxC[0] = nxt.qx;
-        cA[0] = nxt.ca; cY[0] = nxt.cy; dA[0] = nxt.da; dY[0] = nxt.dy;
+        // a lossless grid multiplies by 1 (exact), chosen once per plane, not per level
+        cA[0] = nxt.ca; cY[0] = nxt.cy;
+        dA[0] = lossy_x ? nxt.da : 1.0f;
+        dY[0] = lossy_y ? nxt.dy : 1.0f;
         if (EXC) {
             dR[0] = dRn; dI[0] = dIn; dW[0] = dWn;
             drv_load(p + 1);
@@ -463,9 +466,13 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
 #pragma unroll
         for (int s = 0; s < T; ++s) {
             const int q = p - 1 - s;
-            if (q >= ps && q < Nx) {
+            // no range test on q: a level's outputs at the march's first planes (q < ps +
+            // s + 1) or past the grid depend on planes the march does not hold, and no
+            // owned output depends on them (the halo cone), so computing them is harmless
+            // and the level chain stays free of per-level branches and copies
+            {
                 const float ex_o = exP[s], ey_o = eyP[s], hz_o = hzP[s];
-                const float ex_jp = lane_next(ex_o);                // Ex(q, j+1)
+                const float ex_jp = lane_next0(ex_o);                // Ex(q, j+1)
                 float psx = hxP[s], psy = hyP[s];
                 float hz = 0.f;
                 if ((XIN || q < Nx - 1) && jey) {                   // Hz(q, j)
@@ -486,12 +493,11 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                     }
                     hz = hz_o - a.ch * (tx - ty);
                 }
-                const float hz_jm = lane_prev(hz);                  // Hz(q, j-1)
+                const float hz_jm = lane_prev0(hz);                  // Hz(q, j-1)
                 float qy = eyyP[s];
                 float exn = 0.f;
                 if ((XIN || q < Nx - 1) && jin) {                   // Ex(q, j)
-                    float v = ex_o;
-                    if (lossy_x) v = v * dA[s + 1];
+                    float v = ex_o * dA[s + 1];
                     if (YIN || (j >= 1 && j <= Ny - 2)) {
                         const float d = hz - hz_jm;
                         float t;
@@ -509,8 +515,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                 float qx = exxP[s];
                 float eyn = 0.f;
                 if (jey) {                                          // Ey(q, j)
-                    float v = ey_o;
-                    if (lossy_y) v = v * dY[s + 1];
+                    float v = ey_o * dY[s + 1];
                     if (XIN || (q >= 1 && q <= Nx - 2)) {
                         const float d = hz - hzP[s + 1];            // Hz(q) - Hz(q-1)
                         float t;
@@ -809,9 +814,13 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
 #pragma unroll
         for (int s = 0; s < T; ++s) {
             const int q = p - 1 - s;
-            if (q >= ps && q < Nx) {
+            // no range test on q: a level's outputs at the march's first planes (q < ps +
+            // s + 1) or past the grid depend on planes the march does not hold, and no
+            // owned output depends on them (the halo cone), so computing them is harmless
+            // and the level chain stays free of per-level branches and copies
+            {
                 const float ez_o = ezP[s];
-                const float ez_jp = lane_next(ez_o);                // Ez(q, j+1)
+                const float ez_jp = lane_next0(ez_o);                // Ez(q, j+1)
                 float qxy = axyP[s], qyx = ayxP[s];
                 float hx = 0.f, hy = 0.f;
                 if (jhx) {                                          // Hx(q, j)
@@ -836,7 +845,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                     }
                     hy = hyP[s] + a.ch * t;
                 }
-                const float hx_jm = lane_prev(hx);                  // Hx(q, j-1)
+                const float hx_jm = lane_prev0(hx);                  // Hx(q, j-1)
                 float qzx = azxP[s], qzy = azyP[s];
                 float ezn = 0.f;
                 if (jin) {                                          // Ez(q, j)

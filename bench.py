@@ -36,20 +36,25 @@ BYTES_TM2D_LOSSY = 3 * 8 + 4 * 4      # Ex, Ey, Hz r/w + Ca_ex, Ca_ey, D_ex, D_e
 BYTES_YEE3D = 6 * 8 + 3 * 4           # 6 fields r/w + ce_x, ce_y, ce_z = 60 B
 # HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), scaled by the same counters on a plain 1 GiB copy
-# (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth).
-TRAFFIC_BYTES = {('tm2d_stitch', 8): 724.2e6}
+# (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth);
+# the 2D stitch figure is for the x chunk of 128 planes the autotuner keeps there.
+TRAFFIC_BYTES = {('tm2d_stitch', 8): 655.7e6}
 
 
-def _timed_factory(records, every=1):
+def _timed_factory(records, every=1, fixed=None):
     """Engine class that records HIP-event times of its runs.  every > 1: only one run()
     call in `every` carries the events (the per-launch events of a timed run add to the
-    stream's own time, so a sample of the calls is timed and the rest run bare)."""
+    stream's own time, so a sample of the calls is timed and the rest run bare).
+    fixed = (depth, lx, lead): blocking setting of every engine (the autotuner is off)."""
     from lens_sensor_amd.engine.hip import HipEngine
 
     class TimedHip(HipEngine):
         def __init__(self, kind, dims, mode):
             super().__init__(kind, dims, mode)
             self.set_timing(True)
+            if fixed is not None and kind != 'yee3d':
+                self.set_temporal(fixed[0], fixed[1])
+                self.set_tb_lead(fixed[2])
             self._upd = 0.0
             self._tot = 0.0
             self._n = 0
@@ -105,12 +110,16 @@ def bench_tm2d_stitch(args):
                               y_um=-r, n=1.59))
     records = []
     info = []
-    old_k, old_tb = d2.KAPPA, d2.TEAR_BUFFER
+    old_k, old_tb, old_at = d2.KAPPA, d2.TEAR_BUFFER, backend.AUTOTUNE
     d2.KAPPA = {'bead': 0.002}
     d2.TEAR_BUFFER = tear_um * 1e-6
+    fixed = None
+    if args.temporal:
+        fixed = tuple(int(v) for v in args.temporal.split(','))
+        backend.AUTOTUNE = False
     t0 = time.time()
     try:
-        with backend.use_engine(_timed_factory(records, every=5)), \
+        with backend.use_engine(_timed_factory(records, every=5, fixed=fixed)), \
                 contextlib.redirect_stdout(io.StringIO()):
             r = d2.run_beads_stitched(
                 'pmma', 'polystyrol', 0.0, dx_nm, False, 2, t_wg, 850.0, 3.0, 0.0, 0.0, 'cw',
@@ -121,7 +130,7 @@ def bench_tm2d_stitch(args):
                                                            print(f'[bench] window {w} {i}',
                                                                  file=sys.stderr, flush=True)))
     finally:
-        d2.KAPPA, d2.TEAR_BUFFER = old_k, old_tb
+        d2.KAPPA, d2.TEAR_BUFFER, backend.AUTOTUNE = old_k, old_tb, old_at
     wall = time.time() - t0
     cells = [i['cells'] for i in info]
     step_s = sum(i['seconds'] for i in info)
@@ -281,6 +290,9 @@ def main():
     ap.add_argument('--windows', type=int, default=None, help='tm2d_stitch: windows to run')
     ap.add_argument('--steps-override', type=int, default=None,
                     help='steps per window (reduced runs for profiling only)')
+    ap.add_argument('--temporal', default=None, metavar='DEPTH,LX,LEAD',
+                    help='tm2d_stitch: fixed blocking setting instead of the autotuner '
+                         '(profiling runs of one configuration)')
     ap.add_argument('--seed', type=int, default=2024)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-probe', type=int, default=None, metavar='MIB',

@@ -1409,7 +1409,7 @@ int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out) {
     c->kind = kind; c->mode = mode;
     if (kind != LF_YEE3D) c->tb = 6;             // blocked passes over monitor-free steps
     if (kind == LF_YEE3D) c->tb = 3;
-    if (kind == LF_TM2D) c->tb_mon = 3;          // monitored passes (CW-window steps)
+    if (kind == LF_TM2D) c->tb_mon = 4;          // monitored passes (CW-window steps)
     if (kind == LF_TE2D) c->tb_mon = 4;
     c->nx = nx; c->ny = ny; c->nz = (kind == LF_YEE3D) ? nz : 1;
     if (kind == LF_YEE3D) c->g = Grid{nx, ny, nz, round_up(nz, 64)};
@@ -1928,7 +1928,7 @@ int tb_chunk_2d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
 
 // ---- monitored 2D passes: the CW-window monitors summed inside the pass
 struct TBMonPlan {
-    int mon = 0;                      // kernel variant: 9 (one phasor) or 14 (two + intensity)
+    int mon = 0;                      // kernel variant: 9 (one phasor) or 14 (two + intensity), +16 all complex128
     TBMon m{};
     int nflux = 0;
     int fid[kTBFlux] = {};            // flux monitor ids in staging order
@@ -1975,6 +1975,9 @@ bool tb_mon_plan(const lf_ctx *c, const lf_step *steps, int T, TBMonPlan &pl) {
         m.dx0[k] = d.x0; m.dx1[k] = d.x0 + d.n0; m.dn2[k] = d.n2;
         m.dbuf[k] = d.buf;
     }
+    bool all128 = nd > 0;
+    for (int k = 0; k < nd; ++k) all128 = all128 && m.dacc[k] == LF_ACC_C128;
+    if (all128) pl.mon |= 16;            // kernels without the accumulator-format tests
     if (sid >= 0) {
         const Monitor &mo = c->mon[sid];
         m.sform = mo.form; m.sn2 = mo.n2; m.sbuf = (float *)mo.buf;

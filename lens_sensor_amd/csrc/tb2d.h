@@ -168,6 +168,7 @@ struct TBMon {
 template <int T, int MON>
 struct TBMonRegs {
     static constexpr int ND = MON & 3, NS = (MON >> 2) & 1;
+    static constexpr bool C128 = (MON >> 4) & 1;   // every phasor accumulates in complex128
     static constexpr int NDA = ND > 0 ? ND : 1, NSA = NS > 0 ? NS : 1;
     // slot L (1..T): accumulators of plane p-L, the plane level L-1 advances in
     // iteration p; the next plane's accumulators wait in dn / sn
@@ -176,12 +177,15 @@ struct TBMonRegs {
     double2 dn[NDA];
     float sn[NSA];
     unsigned used;               // monitors summed on any level of this pass
+    bool fwave;                  // a flux plane lies among the wavefront's planes [ps, pe)
 
-    __device__ __forceinline__ void init(const TBMon *m) {
+    __device__ __forceinline__ void init(const TBMon *m, int ps, int pe) {
         used = 0u;
+        fwave = false;
         if constexpr (MON != 0) {
 #pragma unroll
             for (int l = 0; l < T; ++l) used |= m->lm[l];
+            for (int f = 0; f < m->nflux; ++f) fwave = fwave || (m->fil[f] >= ps && m->fil[f] < pe);
         }
 #pragma unroll
         for (int L = 0; L <= T; ++L) {
@@ -203,7 +207,7 @@ struct TBMonRegs {
             double2 v = make_double2(0.0, 0.0);
             if (((used >> k) & 1u) && pl >= m->dx0[k] && pl < m->dx1[k] && ok && j < m->dn2[k]) {
                 const long long o = (long long)(pl - m->dx0[k]) * m->dn2[k] + j;
-                if (m->dacc[k] == LF_ACC_C128) {
+                if (C128 || m->dacc[k] == LF_ACC_C128) {
                     v = ((const double2 *)m->dbuf[k])[o];
                 } else {
                     const float2 f = ((const float2 *)m->dbuf[k])[o];
@@ -230,7 +234,7 @@ struct TBMonRegs {
             if (((used >> k) & 1u) && q >= m->dx0[k] && q < m->dx1[k] && j < m->dn2[k]) {
                 const long long o = (long long)(q - m->dx0[k]) * m->dn2[k] + j;
                 const double2 v = d[k][T];
-                if (m->dacc[k] == LF_ACC_C128) ((double2 *)m->dbuf[k])[o] = v;
+                if (C128 || m->dacc[k] == LF_ACC_C128) ((double2 *)m->dbuf[k])[o] = v;
                 else ((float2 *)m->dbuf[k])[o] = make_float2((float)v.x, (float)v.y);
             }
         }
@@ -292,8 +296,8 @@ __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *
         const float F = src == LF_SRC_EY ? eyn
                       : src == LF_SRC_EX ? exn
                       : src == LF_SRC_EY_NODE ? node_ey() : node_ex();
-        r.d[k][s + 1] = tb_dft_add(r.d[k][s + 1], F, m->dacc[k], m->pr[s], m->pi[s], m->fr[k][s],
-                                   m->fi[k][s]);
+        r.d[k][s + 1] = tb_dft_add(r.d[k][s + 1], F, TBMonRegs<T, MON>::C128 ? LF_ACC_C128 : m->dacc[k],
+                                   m->pr[s], m->pi[s], m->fr[k][s], m->fi[k][s]);
     }
     if constexpr (NS > 0) {
         if ((bits >> 2) & 1u) {
@@ -307,7 +311,7 @@ __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *
             r.s[0][s + 1] = r.s[0][s + 1] + add;
         }
     }
-    if (bits >> 3) {
+    if ((bits >> 3) && r.fwave) {       // most wavefronts hold no flux plane
         // a loop, not unrolled: four unrolled copies per level hold their plane indices
         // and pointers in scalar registers and push the body over 150 VGPRs (the
         // monitor block is read through a pointer; the loop copies nothing to scratch)
@@ -331,13 +335,13 @@ __device__ __forceinline__ void te_mon_level(TBMonRegs<T, MON> &r, const TBMon *
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
         if (!((bits >> k) & 1u)) continue;
-        r.d[k][s + 1] = tb_dft_add(r.d[k][s + 1], ezn, m->dacc[k], m->pr[s], m->pi[s], m->fr[k][s],
-                                   m->fi[k][s]);
+        r.d[k][s + 1] = tb_dft_add(r.d[k][s + 1], ezn, TBMonRegs<T, MON>::C128 ? LF_ACC_C128 : m->dacc[k],
+                                   m->pr[s], m->pi[s], m->fr[k][s], m->fi[k][s]);
     }
     if constexpr (NS > 0) {
         if ((bits >> 2) & 1u) r.s[0][s + 1] = r.s[0][s + 1] + ezn * ezn;
     }
-    if (bits >> 3) {
+    if ((bits >> 3) && r.fwave) {
         for (int f = 0; f < m->nflux; ++f) {
             if (((bits >> (3 + f)) & 1u) && q == m->fil[f] && own && j < m->fn2[f]) {
                 float *o = m->fst[f] + (long long)(2 * s) * m->fn2[f] + j;
@@ -429,7 +433,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     if (EXC) drv_load(ps);
     TBMonRegs<T, MON> mr;
     if constexpr (MON != 0) {
-        mr.init(m);
+        mr.init(m, ps, pe);
         mr.load(m, ps, j, own, X0, X1);
     }
     // planes are prefetched two iterations ahead: a wave's march is a serial chain, and
@@ -786,7 +790,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
     if (EXC) drv_load(ps);
     TBMonRegs<T, MON> mr;
     if constexpr (MON != 0) {
-        mr.init(m);
+        mr.init(m, ps, pe);
         mr.load(m, ps, j, own, X0, X1);
     }
     constexpr int PFD = MON == 0 ? 2 : 1;                                   // as for TM

@@ -354,7 +354,7 @@ __device__ __forceinline__ void te_mon_level(TBMonRegs<T, MON> &r, const TBMon *
 
 // Body of one wavefront (index wv within its instance's launch segments); st[s] are the
 // per-step scalars of step n+s.
-template <int T, int MODE, bool EXC = true, int MON = 0>
+template <int T, int MODE, int EXC = 3, int MON = 0>
 __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, const StepArgs *st,
                                            int wv, int fp64, const TBMon *m = nullptr) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
@@ -416,7 +416,8 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     }
     // stitch drive of the driven component (Ey), loaded with its plane and carried by
     // plane age like the coefficients, so the levels wait on no load of their own
-    const float2 *dcomp = EXC ? x.drv[LF_EY] : nullptr;
+    constexpr bool SRC = EXC & 1, DRV = EXC & 2;      // excitation code: source, drive
+    const float2 *dcomp = DRV ? x.drv[LF_EY] : nullptr;
     float dR[T + 1], dI[T + 1], dW[T + 1], dRn = 0.f, dIn = 0.f, dWn = 0.f;
     auto drv_load = [&](int pl) {
         dRn = dIn = dWn = 0.f;
@@ -430,7 +431,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     };
 #pragma unroll
     for (int L = 0; L <= T; ++L) dR[L] = dI[L] = dW[L] = 0.f;
-    if (EXC) drv_load(ps);
+    if (DRV) drv_load(ps);
     TBMonRegs<T, MON> mr;
     if constexpr (MON != 0) {
         mr.init(m, ps, pe);
@@ -457,7 +458,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
         cA[0] = nxt.ca; cY[0] = nxt.cy;
         dA[0] = lossy_x ? nxt.da : 1.0f;
         dY[0] = lossy_y ? nxt.dy : 1.0f;
-        if (EXC) {
+        if (DRV) {
             dR[0] = dRn; dI[0] = dIn; dW[0] = dWn;
             drv_load(p + 1);
         }
@@ -531,8 +532,8 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                         }
                         v = v + (-cY[s + 1]) * t;
                     }
-                    if (EXC) v = apply_source(v, LF_EY, q, j, x, st[s], fp64);
-                    if (EXC && dcomp != nullptr && q < x.drv_n)
+                    if (SRC) v = apply_source(v, LF_EY, q, j, x, st[s], fp64);
+                    if (DRV && dcomp != nullptr && q < x.drv_n)
                         v = drive_mix(v, make_float2(dR[s + 1], dI[s + 1]), dW[s + 1], x.drv_math, st[s]);
                     if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     eyn = v;
@@ -575,7 +576,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
 #pragma unroll
         for (int L = T; L >= 1; --L) {
             cA[L] = cA[L - 1]; cY[L] = cY[L - 1]; dA[L] = dA[L - 1]; dY[L] = dY[L - 1];
-            if (EXC) { dR[L] = dR[L - 1]; dI[L] = dI[L - 1]; dW[L] = dW[L - 1]; }
+            if (DRV) { dR[L] = dR[L - 1]; dI[L] = dI[L - 1]; dW[L] = dW[L - 1]; }
         }
         if constexpr (MON != 0) mr.shift();
     };
@@ -592,13 +593,16 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
 // slab wavefronts are few long serial chains that run beside the interior launch: they
 // take issue priority on their SIMD so that they finish with it.  The wave index is read
 // into a scalar register, so every x/plane test is a scalar branch.
-// EXC = false: no source or drive plane lies within the launch's marches, so the body
-// carries no excitation loads (whose waits would also drain the plane prefetch).
+// EXC: excitation code of the body, bit 0 the source, bit 1 the stitch drive.  EXC = 0:
+// no source or drive plane lies within the launch's marches, so the body carries no
+// excitation loads (whose waits would also drain the plane prefetch).  A context with
+// only a drive (the stitch windows after the first) or only a source takes the body with
+// only that code: fewer scalar registers to spill and reload inside the levels.
 // The slab-free body with excitation is held to 168 VGPRs (3 waves per SIMD instead of
 // 2 at its natural 170): its class carries the stitch drive over a quarter of a window
 // and is the long pole of those passes; the two values it then keeps in scratch are
 // read only on the source planes.
-template <int T, int MODE, bool EXC = true>
+template <int T, int MODE, int EXC>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, 256),
                                amdgpu_waves_per_eu(MODE == 0 && EXC ? 3 : 1)))
 tm_tb_kernel(TBTMArgs a, Excite x, int fp64) {
@@ -617,14 +621,14 @@ __device__ __forceinline__ int tb_segment(const int *seg_end, int nseg, int wv) 
 
 // A slab class whose rectangles differ in excitation: each wavefront takes the body its
 // rectangle needs (the two bodies hold nearly the same number of waves per SIMD).
-template <int T, int MODE>
+template <int T, int MODE, int EXC>
 __global__ void __launch_bounds__(256) tm_tb_mixed_kernel(TBTMArgs a, Excite x, int fp64) {
     __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = tb_segment(a.seg_end, a.nseg, wv);
     if (g < 0) return;
-    if ((a.seg_exc >> g) & 1u) tm_tb_body<T, MODE, true>(a, x, a.st, wv, fp64);
-    else tm_tb_body<T, MODE, false>(a, x, a.st, wv, fp64);
+    if ((a.seg_exc >> g) & 1u) tm_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
+    else tm_tb_body<T, MODE, 0>(a, x, a.st, wv, fp64);
 }
 
 // ---------------------------------------------------------------------------- temporal blocking (2D TE)
@@ -720,7 +724,7 @@ __device__ __forceinline__ void te_mask_plane(TEPlane &v, const TBTEArgs &a, int
     }
 }
 
-template <int T, int MODE, bool EXC = true, int MON = 0>
+template <int T, int MODE, int EXC = 3, int MON = 0>
 __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, const StepArgs *st,
                                            int wv, int fp64, const TBMon *m = nullptr) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
@@ -773,7 +777,8 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         cE[L] = cH[L] = 0.f;
     }
     // stitch drive of Ez, loaded with its plane and carried by plane age
-    const float2 *dcomp = EXC ? x.drv[LF_EZ] : nullptr;
+    constexpr bool SRC = EXC & 1, DRV = EXC & 2;      // excitation code: source, drive
+    const float2 *dcomp = DRV ? x.drv[LF_EZ] : nullptr;
     float dR[T + 1], dI[T + 1], dW[T + 1], dRn = 0.f, dIn = 0.f, dWn = 0.f;
     auto drv_load = [&](int pl) {
         dRn = dIn = dWn = 0.f;
@@ -787,7 +792,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
     };
 #pragma unroll
     for (int L = 0; L <= T; ++L) dR[L] = dI[L] = dW[L] = 0.f;
-    if (EXC) drv_load(ps);
+    if (DRV) drv_load(ps);
     TBMonRegs<T, MON> mr;
     if constexpr (MON != 0) {
         mr.init(m, ps, pe);
@@ -806,7 +811,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         ezC[0] = nxt.ez; hxC[0] = nxt.hx; hyC[0] = nxt.hy;
         axyC[0] = nxt.pxy; ayxC[0] = nxt.pyx; azxC[0] = nxt.pzx; azyC[0] = nxt.pzy;
         cE[0] = nxt.ce; cH[0] = nxt.cH;
-        if (EXC) {
+        if (DRV) {
             dR[0] = dRn; dI[0] = dIn; dW[0] = dWn;
             drv_load(p + 1);
         }
@@ -872,8 +877,8 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                         }
                         v = cE[s + 1] * v + cH[s + 1] * t;
                     }
-                    if (EXC) v = apply_source(v, LF_EZ, q, j, x, st[s], fp64);
-                    if (EXC && dcomp != nullptr && q < x.drv_n)
+                    if (SRC) v = apply_source(v, LF_EZ, q, j, x, st[s], fp64);
+                    if (DRV && dcomp != nullptr && q < x.drv_n)
                         v = drive_mix(v, make_float2(dR[s + 1], dI[s + 1]), dW[s + 1], x.drv_math, st[s]);
                     if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     if (!YIN && ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1))) v = 0.0f;
@@ -913,7 +918,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
 #pragma unroll
         for (int L = T; L >= 1; --L) {
             cE[L] = cE[L - 1]; cH[L] = cH[L - 1];
-            if (EXC) { dR[L] = dR[L - 1]; dI[L] = dI[L - 1]; dW[L] = dW[L - 1]; }
+            if (DRV) { dR[L] = dR[L - 1]; dI[L] = dI[L - 1]; dW[L] = dW[L - 1]; }
         }
         if constexpr (MON != 0) mr.shift();
     };
@@ -927,21 +932,21 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
     }
 }
 
-template <int T, int MODE, bool EXC = true>
+template <int T, int MODE, int EXC>
 __global__ void __launch_bounds__(256) te_tb_kernel(TBTEArgs a, Excite x, int fp64) {
     if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     te_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
 }
 
-template <int T, int MODE>
+template <int T, int MODE, int EXC>
 __global__ void __launch_bounds__(256) te_tb_mixed_kernel(TBTEArgs a, Excite x, int fp64) {
     __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = tb_segment(a.seg_end, a.nseg, wv);
     if (g < 0) return;
-    if ((a.seg_exc >> g) & 1u) te_tb_body<T, MODE, true>(a, x, a.st, wv, fp64);
-    else te_tb_body<T, MODE, false>(a, x, a.st, wv, fp64);
+    if ((a.seg_exc >> g) & 1u) te_tb_body<T, MODE, EXC>(a, x, a.st, wv, fp64);
+    else te_tb_body<T, MODE, 0>(a, x, a.st, wv, fp64);
 }
 
 // Monitored passes: the plain pass plus the monitor work of TBMon (instantiated in
@@ -950,14 +955,14 @@ template <int T, int MODE, bool EXC, int MON>
 __global__ void __launch_bounds__(256) tm_tb_mon_kernel(TBTMArgs a, Excite x, TBMon m, int fp64) {
     if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    tm_tb_body<T, MODE, EXC, MON>(a, x, a.st, wv, fp64, &m);
+    tm_tb_body<T, MODE, EXC ? 3 : 0, MON>(a, x, a.st, wv, fp64, &m);
 }
 
 template <int T, int MODE, bool EXC, int MON>
 __global__ void __launch_bounds__(256) te_tb_mon_kernel(TBTEArgs a, Excite x, TBMon m, int fp64) {
     if (MODE != 0) __builtin_amdgcn_s_setprio(2);
     const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    te_tb_body<T, MODE, EXC, MON>(a, x, a.st, wv, fp64, &m);
+    te_tb_body<T, MODE, EXC ? 3 : 0, MON>(a, x, a.st, wv, fp64, &m);
 }
 
 // launchers of the plain passes (tb2d_pass.hip): depth T in {2, 4, 6, 8}, mode 0 / 1 /

@@ -12,32 +12,50 @@ namespace lfd {
 
 #if LF_TB2P_TM
 using TB2Args = TBTMArgs;
-template <int T, int MODE, bool EXC>
+template <int T, int MODE, int EXC>
 static void go(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
     tm_tb_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
-template <int T, int MODE>
+template <int T, int MODE, int EXC>
 static void go_mixed(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
-    tm_tb_mixed_kernel<T, MODE><<<nb, 256, 0, st>>>(q, x, fp64);
+    tm_tb_mixed_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
 #else
 using TB2Args = TBTEArgs;
-template <int T, int MODE, bool EXC>
+template <int T, int MODE, int EXC>
 static void go(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
     te_tb_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
-template <int T, int MODE>
+template <int T, int MODE, int EXC>
 static void go_mixed(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
-    te_tb_mixed_kernel<T, MODE><<<nb, 256, 0, st>>>(q, x, fp64);
+    te_tb_mixed_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
 #endif
 
+// exc: the class's rectangles with excitation (0 none, 1 all, 2 some); em: the code they
+// need (bit 0 source, bit 1 drive)
+template <int T, int MODE, int EM>
+static void by_exc(int exc, const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
+    if (exc == 0) go<T, MODE, 0>(q, x, fp64, st, nb);
+    else if constexpr (MODE == 0) go<T, MODE, EM>(q, x, fp64, st, nb);   // no mixed slab-free class
+    else if (exc == 1) go<T, MODE, EM>(q, x, fp64, st, nb);
+    else go_mixed<T, MODE, EM>(q, x, fp64, st, nb);
+}
+
+template <int T, int MODE>
+static void by_em(int exc, int em, const TB2Args &q, const Excite &x, int fp64, hipStream_t st,
+                  unsigned nb) {
+    if (em == 1) by_exc<T, MODE, 1>(exc, q, x, fp64, st, nb);
+    else if (em == 2) by_exc<T, MODE, 2>(exc, q, x, fp64, st, nb);
+    else by_exc<T, MODE, 3>(exc, q, x, fp64, st, nb);
+}
+
 template <int T>
-static void by_mode(int mode, int exc, const TB2Args &q, const Excite &x, int fp64, hipStream_t st,
-                    unsigned nb) {
-    if (mode == 0) (exc ? go<T, 0, true> : go<T, 0, false>)(q, x, fp64, st, nb);
-    else if (mode == 1) (exc == 2 ? go_mixed<T, 1> : exc ? go<T, 1, true> : go<T, 1, false>)(q, x, fp64, st, nb);
-    else (exc == 2 ? go_mixed<T, 3> : exc ? go<T, 3, true> : go<T, 3, false>)(q, x, fp64, st, nb);
+static void by_mode(int mode, int exc, int em, const TB2Args &q, const Excite &x, int fp64,
+                    hipStream_t st, unsigned nb) {
+    if (mode == 0) by_em<T, 0>(exc, em, q, x, fp64, st, nb);
+    else if (mode == 1) by_em<T, 1>(exc, em, q, x, fp64, st, nb);
+    else by_em<T, 3>(exc, em, q, x, fp64, st, nb);
 }
 
 #if LF_TB2P_TM
@@ -47,10 +65,19 @@ void launch_tm_tb(int T, int mode, int exc, const TBTMArgs &q, const Excite &x, 
 void launch_te_tb(int T, int mode, int exc, const TBTEArgs &q, const Excite &x, int fp64,
                   hipStream_t st, unsigned nb) {
 #endif
-    if (T == 2) by_mode<2>(mode, exc, q, x, fp64, st, nb);
-    else if (T == 4) by_mode<4>(mode, exc, q, x, fp64, st, nb);
-    else if (T == 6) by_mode<6>(mode, exc, q, x, fp64, st, nb);
-    else by_mode<8>(mode, exc, q, x, fp64, st, nb);
+#if LF_TB2P_TM
+    const int comp = LF_EY;
+#else
+    const int comp = LF_EZ;
+#endif
+    // a source or drive that the stepped component does not carry is never applied
+    int em = (x.amp != nullptr && x.src_comp == comp ? 1 : 0) |
+             (x.drv[comp] != nullptr && x.drv_n > 0 ? 2 : 0);
+    if (em == 0) exc = 0;
+    if (T == 2) by_mode<2>(mode, exc, em, q, x, fp64, st, nb);
+    else if (T == 4) by_mode<4>(mode, exc, em, q, x, fp64, st, nb);
+    else if (T == 6) by_mode<6>(mode, exc, em, q, x, fp64, st, nb);
+    else by_mode<8>(mode, exc, em, q, x, fp64, st, nb);
 }
 
 }  // namespace lfd

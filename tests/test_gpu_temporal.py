@@ -131,3 +131,24 @@ def test_te_blocked_source_and_drive_equal_depth1(mode):
     _compare('te2d', a, b, ma)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize('kind', ['tm2d', 'te2d'])
+@pytest.mark.parametrize('source,drive', [(True, False), (False, True), (True, True)])
+def test_blocked_excitation_variants_equal_depth1(kind, source, drive):
+    """The passes take a body with only the source code, only the drive code, or both,
+    by what the context carries; each equals depth 1 bit for bit."""
+    H = _hip()
+    dims = (150, 140)
+    print(f'[excitation variant] {kind} source={source} drive={drive}', flush=True)
+    a, b = H(kind, dims, 'fp64acc'), H(kind, dims, 'fp64acc')
+    a.set_temporal(8, 16)
+    b.set_temporal(1, 64)
+    ma = _setup(a, kind, dims, np.random.default_rng(5), source=source, drive=drive, pec=False)
+    _setup(b, kind, dims, np.random.default_rng(5), source=source, drive=drive, pec=False)
+    tab = _table(42, 0, 40)
+    a.run(tab)
+    b.run(tab)
+    _compare(kind, a, b, ma)
+    a.close()
+    b.close()

@@ -122,3 +122,32 @@ def test_monitored_depth_setter():
     with pytest.raises(RuntimeError):
         e3.set_temporal_mon(2)
     e3.close()
+
+
+@pytest.mark.parametrize('kind,dm', [('tm2d', 4), ('tm2d', 3), ('te2d', 4)])
+def test_monitored_pass_complex128_variant_equal_depth1(kind, dm):
+    """All phasors in complex128 (the stitch runs' fp64acc setup): the pass takes the kernel
+    variant without accumulator-format tests; accumulators equal depth 1 bit for bit."""
+    H = _hip()
+    dims = (150, 240)
+    print(f'[monitored pass c128] {kind} depth={dm}', flush=True)
+    engs = []
+    for depth in (dm, 1):
+        e = H(kind, dims, 'fp64acc')
+        _setup(e, kind, dims, np.random.default_rng(13), source=True, drive=True, pec=False)
+        comp = 'Ey' if kind == 'tm2d' else 'Ez'
+        mons = dict(d0=e.add_dft(comp, 0, dims[0], acc='c128', phasor='p'),
+                    flux=e.add_flux_plane(dims[0] // 3))
+        if kind == 'tm2d':
+            mons['d1'] = e.add_dft('ex_node', 0, dims[0], acc='c128', phasor='p')
+            mons['sq'] = e.add_sq('tm_ex2')
+        e.set_temporal(8 if kind == 'tm2d' else 4, 32)
+        e.set_temporal_mon(depth)
+        engs.append((e, mons))
+    tab = _mon_table(50, engs[0][1], 13, seed=21)
+    for e, _ in engs:
+        e.run(tab)
+    (a, ma), (b, _) = engs
+    _compare(kind, a, b, ma)
+    a.close()
+    b.close()

@@ -416,7 +416,9 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     }
     // stitch drive of the driven component (Ey), loaded with its plane and carried by
     // plane age like the coefficients, so the levels wait on no load of their own
-    constexpr bool SRC = EXC & 1, DRV = EXC & 2;      // excitation code: source, drive
+    // excitation code: bit 0 source, bit 1 drive (blend chosen at run time), bit 2 drive
+    // with the float64 blend only
+    constexpr bool SRC = EXC & 1, DRV = EXC & 6, D128 = EXC & 4;
     const float2 *dcomp = DRV ? x.drv[LF_EY] : nullptr;
     float dR[T + 1], dI[T + 1], dW[T + 1], dRn = 0.f, dIn = 0.f, dWn = 0.f;
     auto drv_load = [&](int pl) {
@@ -534,7 +536,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                     }
                     if (SRC) v = apply_source(v, LF_EY, q, j, x, st[s], fp64);
                     if (DRV && dcomp != nullptr && q < x.drv_n)
-                        v = drive_mix(v, make_float2(dR[s + 1], dI[s + 1]), dW[s + 1], x.drv_math, st[s]);
+                        v = drive_mix(v, make_float2(dR[s + 1], dI[s + 1]), dW[s + 1], D128 ? LF_DRIVE_C128 : x.drv_math, st[s]);
                     if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     eyn = v;
                 }
@@ -593,7 +595,8 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
 // slab wavefronts are few long serial chains that run beside the interior launch: they
 // take issue priority on their SIMD so that they finish with it.  The wave index is read
 // into a scalar register, so every x/plane test is a scalar branch.
-// EXC: excitation code of the body, bit 0 the source, bit 1 the stitch drive.  EXC = 0:
+// EXC: excitation code of the body, bit 0 the source, bit 1 the stitch drive, bit 2 the
+// drive with its float64 blend fixed at compile time (no float32 blend code).  EXC = 0:
 // no source or drive plane lies within the launch's marches, so the body carries no
 // excitation loads (whose waits would also drain the plane prefetch).  A context with
 // only a drive (the stitch windows after the first) or only a source takes the body with
@@ -777,7 +780,9 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         cE[L] = cH[L] = 0.f;
     }
     // stitch drive of Ez, loaded with its plane and carried by plane age
-    constexpr bool SRC = EXC & 1, DRV = EXC & 2;      // excitation code: source, drive
+    // excitation code: bit 0 source, bit 1 drive (blend chosen at run time), bit 2 drive
+    // with the float64 blend only
+    constexpr bool SRC = EXC & 1, DRV = EXC & 6, D128 = EXC & 4;
     const float2 *dcomp = DRV ? x.drv[LF_EZ] : nullptr;
     float dR[T + 1], dI[T + 1], dW[T + 1], dRn = 0.f, dIn = 0.f, dWn = 0.f;
     auto drv_load = [&](int pl) {
@@ -879,7 +884,7 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                     }
                     if (SRC) v = apply_source(v, LF_EZ, q, j, x, st[s], fp64);
                     if (DRV && dcomp != nullptr && q < x.drv_n)
-                        v = drive_mix(v, make_float2(dR[s + 1], dI[s + 1]), dW[s + 1], x.drv_math, st[s]);
+                        v = drive_mix(v, make_float2(dR[s + 1], dI[s + 1]), dW[s + 1], D128 ? LF_DRIVE_C128 : x.drv_math, st[s]);
                     if (!XIN && ((a.pec_xlo && q == 0) || (a.pec_xhi && q == Nx - 1))) v = 0.0f;
                     if (!YIN && ((a.pec_ylo && j == 0) || (a.pec_yhi && j == Ny - 1))) v = 0.0f;
                     ezn = v;

@@ -33,7 +33,8 @@ static void go_mixed(const TB2Args &q, const Excite &x, int fp64, hipStream_t st
 #endif
 
 // exc: the class's rectangles with excitation (0 none, 1 all, 2 some); em: the code they
-// need (bit 0 source, bit 1 drive)
+// need: 1 source only, 4 drive only with the float64 blend (the 2D stitch windows after
+// the first), 3 anything else (source and drive, blend chosen at run time)
 template <int T, int MODE, int EM>
 static void by_exc(int exc, const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
     if (exc == 0) go<T, MODE, 0>(q, x, fp64, st, nb);
@@ -46,7 +47,7 @@ template <int T, int MODE>
 static void by_em(int exc, int em, const TB2Args &q, const Excite &x, int fp64, hipStream_t st,
                   unsigned nb) {
     if (em == 1) by_exc<T, MODE, 1>(exc, q, x, fp64, st, nb);
-    else if (em == 2) by_exc<T, MODE, 2>(exc, q, x, fp64, st, nb);
+    else if (em == 4) by_exc<T, MODE, 4>(exc, q, x, fp64, st, nb);
     else by_exc<T, MODE, 3>(exc, q, x, fp64, st, nb);
 }
 
@@ -71,8 +72,9 @@ void launch_te_tb(int T, int mode, int exc, const TBTEArgs &q, const Excite &x, 
     const int comp = LF_EZ;
 #endif
     // a source or drive that the stepped component does not carry is never applied
-    int em = (x.amp != nullptr && x.src_comp == comp ? 1 : 0) |
-             (x.drv[comp] != nullptr && x.drv_n > 0 ? 2 : 0);
+    const bool src = x.amp != nullptr && x.src_comp == comp;
+    const bool drv = x.drv[comp] != nullptr && x.drv_n > 0;
+    const int em = !src && !drv ? 0 : !drv ? 1 : (!src && x.drv_math == LF_DRIVE_C128) ? 4 : 3;
     if (em == 0) exc = 0;
     if (T == 2) by_mode<2>(mode, exc, em, q, x, fp64, st, nb);
     else if (T == 4) by_mode<4>(mode, exc, em, q, x, fp64, st, nb);

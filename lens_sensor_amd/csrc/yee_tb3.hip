@@ -232,7 +232,7 @@ __device__ __forceinline__ void yee_tb_body(const TB3Args &a, const Excite &x, c
                 for (int qq = 0; qq < 6; ++qq) cP[qq][s + 1][r] = pP[qq][s][r];
                 if (!act || !juse[r]) continue;
                 const float ex_o = pEx[s][r], ey_o = pEy[s][r], ez_o = pEz[s][r];
-                const float ex_k1 = lane_next(ex_o), ey_k1 = lane_next(ey_o);   // k+1
+                const float ex_k1 = lane_next0(ex_o), ey_k1 = lane_next0(ey_o); // k+1 (lane 63 is halo)
                 const float ex_j1 = r + 1 < R ? pEx[s][r + 1 < R ? r + 1 : r] : exJ;   // j+1
                 const float ez_j1 = r + 1 < R ? pEz[s][r + 1 < R ? r + 1 : r] : ezJ;
                 if (kz1 && jh[r]) {                                             // Hx
@@ -277,7 +277,7 @@ __device__ __forceinline__ void yee_tb_body(const TB3Args &a, const Excite &x, c
 #pragma unroll
                 for (int qq = 6; qq < 12; ++qq) cP[qq][s + 1][r] = pP[qq][s][r];
                 const float hx = hxN[r], hy = hyN[r], hz = hzN[r];
-                const float hx_km = lane_prev(hx), hy_km = lane_prev(hy);     // k-1
+                const float hx_km = lane_prev0(hx), hy_km = lane_prev0(hy);   // k-1 (lane 0 is halo)
                 if (act && juse[r]) {
                     const int j = jr[r];
                     const float hx_jm = r > 0 ? hxN[r > 0 ? r - 1 : 0] : hxJ;   // j-1

@@ -1928,7 +1928,7 @@ int tb_chunk_2d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
 
 // ---- monitored 2D passes: the CW-window monitors summed inside the pass
 struct TBMonPlan {
-    int mon = 0;                      // kernel variant: 9 (one phasor) or 14 (two + intensity), +16 all complex128
+    int mon = 0;                      // kernel variant: 9 (one phasor) or 14 (two + intensity), +16 all complex128, +32 TM stitch layout
     TBMon m{};
     int nflux = 0;
     int fid[kTBFlux] = {};            // flux monitor ids in staging order
@@ -2001,6 +2001,15 @@ bool tb_mon_plan(const lf_ctx *c, const lf_step *steps, int T, TBMonPlan &pl) {
             m.fr[k][u] = q ? t.qr32 : t.pr32;
             m.fi[k][u] = q ? t.qi32 : t.pi32;
         }
+    }
+    // the CW-window layout of the TM runs (fdtd2d_core.py:838-842 with the Ex^2 sum of 812):
+    // Ey and node-Ex phasors in complex128 and the Ex^2 sum on every step; its kernels test
+    // nothing per level
+    if (c->kind == LF_TM2D && (pl.mon & 16) && nd == 2 && m.dsrc[0] == LF_SRC_EY &&
+        m.dsrc[1] == LF_SRC_EX_NODE && sid >= 0 && m.sform == LF_SQ_TM_EX2) {
+        bool every = true;
+        for (int u = 0; u < T; ++u) every = every && (m.lm[u] & 7u) == 7u;
+        if (every) pl.mon |= 32;
     }
     return true;
 }

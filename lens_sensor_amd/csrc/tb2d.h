@@ -169,6 +169,9 @@ template <int T, int MON>
 struct TBMonRegs {
     static constexpr int ND = MON & 3, NS = (MON >> 2) & 1;
     static constexpr bool C128 = (MON >> 4) & 1;   // every phasor accumulates in complex128
+    // bit 5 (TM): the CW-window layout of the stitch runs, fixed at compile time: phasor 0
+    // of Ey, phasor 1 of node Ex, the Ex^2 sum, and every level sums all of them
+    static constexpr bool FIX = (MON >> 5) & 1;
     static constexpr int NDA = ND > 0 ? ND : 1, NSA = NS > 0 ? NS : 1;
     // slot L (1..T): accumulators of plane p-L, the plane level L-1 advances in
     // iteration p; the next plane's accumulators wait in dn / sn
@@ -280,8 +283,9 @@ template <int T, int MON>
 __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *m, int s, int q,
                                              int j, int Nx, int Ny, bool own, float exn, float eyn,
                                              float exq1, float hz, float hzq1) {
+    constexpr bool FIX = TBMonRegs<T, MON>::FIX;
     const unsigned bits = m->lm[s];
-    if (bits == 0u) return;
+    if (!FIX && bits == 0u) return;
     constexpr int ND = TBMonRegs<T, MON>::ND, NS = TBMonRegs<T, MON>::NS;
     const float eyp = lane_prev(eyn);                           // Ey(q, j-1)
     auto node_ey = [&]() { return j == 0 ? eyn : (j == Ny - 1 ? eyp : 0.5f * (eyp + eyn)); };
@@ -291,8 +295,8 @@ __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *
     };
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
-        if (!((bits >> k) & 1u)) continue;
-        const int src = m->dsrc[k];
+        if (!FIX && !((bits >> k) & 1u)) continue;
+        const int src = FIX ? (k == 0 ? LF_SRC_EY : LF_SRC_EX_NODE) : m->dsrc[k];
         const float F = src == LF_SRC_EY ? eyn
                       : src == LF_SRC_EX ? exn
                       : src == LF_SRC_EY_NODE ? node_ey() : node_ex();
@@ -300,9 +304,9 @@ __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *
                                    m->pr[s], m->pi[s], m->fr[k][s], m->fi[k][s]);
     }
     if constexpr (NS > 0) {
-        if ((bits >> 2) & 1u) {
+        if (FIX || ((bits >> 2) & 1u)) {
             float add;
-            if (m->sform == LF_SQ_TM_EX2) {
+            if (FIX || m->sform == LF_SQ_TM_EX2) {
                 add = node_ex2();
             } else {
                 const float f = node_ey();

@@ -37,10 +37,14 @@ static void by_mode(int mode, bool exc, const TB2Args &q, const Excite &x, const
 template <int T>
 static void by_mon(int mon, int mode, bool exc, const TB2Args &q, const Excite &x, const TBMon &m,
                    int fp64, hipStream_t st, unsigned nb) {
-    // mon bit 4: every phasor accumulates in complex128 (no per-level format test)
+    // mon bit 4: every phasor accumulates in complex128 (no per-level format test);
+    // bit 5 (TM only): the fixed CW-window layout of the stitch runs
     if (mon == 9) by_mode<T, 9>(mode, exc, q, x, m, fp64, st, nb);
     else if (mon == 14) by_mode<T, 14>(mode, exc, q, x, m, fp64, st, nb);
     else if (mon == 25) by_mode<T, 25>(mode, exc, q, x, m, fp64, st, nb);
+#if LF_TB2M_TM
+    else if (mon == 62) by_mode<T, 62>(mode, exc, q, x, m, fp64, st, nb);
+#endif
     else by_mode<T, 30>(mode, exc, q, x, m, fp64, st, nb);
 }
 

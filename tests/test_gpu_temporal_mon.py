@@ -124,13 +124,16 @@ def test_monitored_depth_setter():
     e3.close()
 
 
-@pytest.mark.parametrize('kind,dm', [('tm2d', 4), ('tm2d', 3), ('te2d', 4)])
-def test_monitored_pass_complex128_variant_equal_depth1(kind, dm):
+@pytest.mark.parametrize('kind,dm,every', [('tm2d', 4, False), ('tm2d', 3, False), ('te2d', 4, False),
+                                            ('tm2d', 4, True), ('tm2d', 2, True)])
+def test_monitored_pass_complex128_variant_equal_depth1(kind, dm, every):
     """All phasors in complex128 (the stitch runs' fp64acc setup): the pass takes the kernel
-    variant without accumulator-format tests; accumulators equal depth 1 bit for bit."""
+    variant without accumulator-format tests; with every monitor on every step (the TM
+    stitch CW window: Ey and node-Ex phasors, Ex^2) the variant with the layout fixed at
+    compile time.  Accumulators equal depth 1 bit for bit."""
     H = _hip()
     dims = (150, 240)
-    print(f'[monitored pass c128] {kind} depth={dm}', flush=True)
+    print(f'[monitored pass c128] {kind} depth={dm} every={every}', flush=True)
     engs = []
     for depth in (dm, 1):
         e = H(kind, dims, 'fp64acc')
@@ -145,6 +148,8 @@ def test_monitored_pass_complex128_variant_equal_depth1(kind, dm):
         e.set_temporal_mon(depth)
         engs.append((e, mons))
     tab = _mon_table(50, engs[0][1], 13, seed=21)
+    if every:
+        tab['mask'][13:] = np.uint32(sum(1 << int(v) for v in engs[0][1].values()))
     for e, _ in engs:
         e.run(tab)
     (a, ma), (b, _) = engs

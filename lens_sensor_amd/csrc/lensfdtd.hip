@@ -20,6 +20,7 @@
 #include <cstring>
 #include <set>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 #include <string>
 #include <vector>
@@ -1747,11 +1748,27 @@ template <typename A>
 void set_class(A &q, const TBClass &k) {
     q.nseg = k.nseg;
     q.seg_exc = 0u;
+    q.seg_m3 = 0u;
     for (int u = 0; u < 4; ++u) {
         q.seg_x0[u] = k.x0[u]; q.seg_x1[u] = k.x1[u]; q.seg_lx[u] = k.lx[u];
         q.seg_wy0[u] = k.wy0[u]; q.seg_nwy[u] = k.nwy[u]; q.seg_end[u] = k.end[u];
         if (u < k.nseg) q.seg_exc |= (unsigned)(k.sexc[u] != 0) << u;
     }
+}
+
+// the y-slab class (MODE 1) then the x-slab class (MODE 3) as the rectangles of one launch
+template <typename A>
+void set_slabs(A &q, const TBClass &k1, const TBClass &k3) {
+    set_class(q, k1);
+    const int base = k1.nseg ? k1.end[k1.nseg - 1] : 0;
+    for (int u = 0; u < k3.nseg; ++u) {
+        const int g = k1.nseg + u;
+        q.seg_x0[g] = k3.x0[u]; q.seg_x1[g] = k3.x1[u]; q.seg_lx[g] = k3.lx[u];
+        q.seg_wy0[g] = k3.wy0[u]; q.seg_nwy[g] = k3.nwy[u]; q.seg_end[g] = base + k3.end[u];
+        q.seg_exc |= (unsigned)(k3.sexc[u] != 0) << g;
+        q.seg_m3 |= 1u << g;
+    }
+    q.nseg = k1.nseg + k3.nseg;
 }
 
 int ensure_side_streams(lf_ctx *c) {
@@ -1780,22 +1797,22 @@ int ensure_side_streams(lf_ctx *c) {
 // per pass (about 10 us each).  When the previous queued work was a pass (tb_chain), each
 // side stream instead waits on the previous pass's lead-class event and on the other side
 // streams' events, one hop; its own stream orders its own previous class.
-template <typename F>
-int fork_tb_classes(lf_ctx *c, const TBPlan &pl, int lead, F one) {
-    if (!pl.interior) {
-        one(pl.cls[2], c->stream);
+template <typename H, typename F>
+int fork_tb_groups(lf_ctx *c, bool interior, int ng, int lead, int solo, H has, F one) {
+    if (!interior) {
+        one(solo, c->stream);
         c->tb_chain = false;
         return 0;
     }
     if (int e = ensure_side_streams(c)) return e;
-    if (lead < 0 || lead >= kTBClasses || pl.cls[lead].nseg == 0) lead = 0;
+    if (lead < 0 || lead >= ng || !has(lead)) lead = 0;
     int side_of[kTBClasses];
-    for (int u = 0, n = 0; u < kTBClasses; ++u) side_of[u] = u == lead ? -1 : n++;
+    for (int u = 0, n = 0; u < ng; ++u) side_of[u] = u == lead ? -1 : n++;
     const bool chain = c->tb_chain;
     if (!chain) LF_HIP(hipEventRecord(c->ev_fork, c->stream));
-    for (int u = 0; u < kTBClasses; ++u) {
+    for (int u = 0; u < ng; ++u) {
         const int sd = side_of[u];
-        if (sd < 0 || pl.cls[u].nseg == 0) continue;
+        if (sd < 0 || !has(u)) continue;
         if (!chain) {
             LF_HIP(hipStreamWaitEvent(c->side[sd], c->ev_fork, 0));
             continue;
@@ -1805,13 +1822,13 @@ int fork_tb_classes(lf_ctx *c, const TBPlan &pl, int lead, F one) {
             if (v != sd && ((c->tb_chain_sides >> v) & 1u))
                 LF_HIP(hipStreamWaitEvent(c->side[sd], c->ev_join[v], 0));
     }
-    one(pl.cls[lead], c->stream);
+    one(lead, c->stream);
     LF_HIP(hipEventRecord(c->ev_lead, c->stream));
-    for (int u = 0; u < kTBClasses; ++u)
-        if (side_of[u] >= 0) one(pl.cls[u], c->side[side_of[u]]);
+    for (int u = 0; u < ng; ++u)
+        if (side_of[u] >= 0 && has(u)) one(u, c->side[side_of[u]]);
     unsigned sides = 0;
-    for (int u = 0; u < kTBClasses; ++u) {
-        if (side_of[u] < 0 || pl.cls[u].nseg == 0) continue;
+    for (int u = 0; u < ng; ++u) {
+        if (side_of[u] < 0 || !has(u)) continue;
         LF_HIP(hipEventRecord(c->ev_join[side_of[u]], c->side[side_of[u]]));
         LF_HIP(hipStreamWaitEvent(c->stream, c->ev_join[side_of[u]], 0));
         sides |= 1u << side_of[u];
@@ -1819,6 +1836,14 @@ int fork_tb_classes(lf_ctx *c, const TBPlan &pl, int lead, F one) {
     c->tb_chain = true;
     c->tb_chain_sides = sides;
     return 0;
+}
+
+// the four classes of a plan, one launch each (the monitored passes)
+template <typename F>
+int fork_tb_classes(lf_ctx *c, const TBPlan &pl, int lead, F one) {
+    return fork_tb_groups(
+        c, pl.interior, kTBClasses, lead, 2, [&](int u) { return pl.cls[u].nseg > 0; },
+        [&](int u, hipStream_t st) { one(pl.cls[u], st); });
 }
 
 void launch_tb_mode(const TBTMArgs &q, int T, int mode, int exc, const Excite &x, int fp64, hipStream_t st) {
@@ -1830,18 +1855,42 @@ void launch_tb_mode(const TBTEArgs &q, int T, int mode, int exc, const Excite &x
     launch_te_tb(T, mode, exc, q, x, fp64, st, nb);
 }
 
-// Launch the classes of one blocked pass.  Without a slab-free rectangle: one launch on
-// the context stream.  Otherwise the rectangle on the context stream and the two slab
-// classes on two side streams, concurrently (disjoint output cells, step-n inputs).
+// Launch the classes of one blocked pass.  Where the slab classes carry no excitation code,
+// or only the drive with the float64 blend (the stitch windows after the first), both slab
+// classes (1 and 2) go in one slab launch: their bodies together still hold two waves per
+// SIMD, and the pass costs the host one launch and one stream's events and waits less
+// (queuing a pass takes the host about as long as the device needs for it).  A lead class
+// 1 or 2 then leads with the slab launch.  Otherwise the four classes launch apart (the
+// source body beside the x-slab body would drop the slab launch to one wave per SIMD).
+// Without a slab-free rectangle only the slab classes run, on the context stream.
 template <typename A>
 int launch_tb_pass(lf_ctx *c, A &f, const TBPlan &pl, int T, int fp64) {
-    auto one = [&](const TBClass &k, hipStream_t st) {
+    auto one_class = [&](const TBClass &k, hipStream_t st) {
         if (k.nseg == 0) return;
         A q = f;
         set_class(q, k);
         launch_tb_mode(q, T, k.mode, k.exc, c->ex, fp64, st);
     };
-    return fork_tb_classes(c, pl, c->tb_lead, one);
+    const bool slab_exc = pl.cls[1].exc || pl.cls[2].exc;
+    const bool merge = !slab_exc || (c->ex.amp == nullptr && c->ex.drv_math == LF_DRIVE_C128);
+    if (!merge) return fork_tb_classes(c, pl, c->tb_lead, one_class);
+    auto has = [&](int u) {
+        return u == 1 ? pl.cls[1].nseg + pl.cls[2].nseg > 0 : pl.cls[u == 0 ? 0 : 3].nseg > 0;
+    };
+    auto one = [&](int u, hipStream_t st) {
+        if (u != 1) {
+            one_class(pl.cls[u == 0 ? 0 : 3], st);
+            return;
+        }
+        if (!has(1)) return;
+        A q = f;
+        set_slabs(q, pl.cls[1], pl.cls[2]);
+        const unsigned nb = (unsigned)(((long long)q.seg_end[q.nseg - 1] + 3) / 4);
+        if constexpr (std::is_same_v<A, TBTMArgs>) launch_tm_tb_slab(T, slab_exc ? 1 : 0, q, c->ex, fp64, st, nb);
+        else launch_te_tb_slab(T, slab_exc ? 1 : 0, q, c->ex, fp64, st, nb);
+    };
+    const int lead = c->tb_lead == 0 ? 0 : c->tb_lead == 3 ? 2 : 1;
+    return fork_tb_groups(c, pl.interior, 3, lead, 1, has, one);
 }
 
 int ensure_psi_alt_2d(lf_ctx *c) {

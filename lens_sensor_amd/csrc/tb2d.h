@@ -41,13 +41,14 @@ struct TBTMArgs {
     Prof xe, xh, ye, yh;
     Strip s0[4], s1[4];       // Hzx Hzy Exy Eyx: read / write buffers
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
-    // wavefronts of this launch: up to 4 rectangles of (x chunk, y wave) indices; wave
+    // wavefronts of this launch: up to 8 rectangles of (x chunk, y wave) indices; wave
     // wv of segment g (seg_end[g-1] <= wv < seg_end[g], loc = wv - seg_end[g-1]) owns
     // the x chunk [X0, min(X0 + seg_lx, seg_x1)) with X0 = seg_x0 + (loc / seg_nwy) *
     // seg_lx, and the y wave seg_wy0 + loc % seg_nwy
     int nseg;
-    int seg_x0[4], seg_x1[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
+    int seg_x0[8], seg_x1[8], seg_wy0[8], seg_nwy[8], seg_end[8], seg_lx[8];
     unsigned seg_exc;         // bit g: segment g needs the body with excitation code
+    unsigned seg_m3;          // bit g: segment g runs the x-slab body (slab launches)
     StepArgs st[kMaxTB];
 };
 
@@ -638,6 +639,26 @@ __global__ void __launch_bounds__(256) tm_tb_mixed_kernel(TBTMArgs a, Excite x, 
     else tm_tb_body<T, MODE, 0>(a, x, a.st, wv, fp64);
 }
 
+// The y-slab rows (MODE 1) and the x-slab chunks (MODE 3) of a pass in one launch: both
+// bodies hold two waves per SIMD, so one launch loses no occupancy and saves the host a
+// launch and a stream's events and waits per pass.  Each wavefront takes the body of its
+// rectangle (seg_m3) with or without excitation code (seg_exc).
+template <int T, int EXC>
+__global__ void __launch_bounds__(256) tm_tb_slab_kernel(TBTMArgs a, Excite x, int fp64) {
+    __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = tb_segment(a.seg_end, a.nseg, wv);
+    if (g < 0) return;
+    const bool m3 = (a.seg_m3 >> g) & 1u, e = EXC != 0 && ((a.seg_exc >> g) & 1u);
+    if (m3) {
+        if (e) tm_tb_body<T, 3, EXC>(a, x, a.st, wv, fp64);
+        else tm_tb_body<T, 3, 0>(a, x, a.st, wv, fp64);
+    } else {
+        if (e) tm_tb_body<T, 1, EXC>(a, x, a.st, wv, fp64);
+        else tm_tb_body<T, 1, 0>(a, x, a.st, wv, fp64);
+    }
+}
+
 // ---------------------------------------------------------------------------- temporal blocking (2D TE)
 // Same pipeline as the TM pass with the roles of E and H exchanged: level s advances
 // plane q = p-1-s from step n+s to n+s+1: Hx^{n+s+1/2}(q) needs Ez(q, j+1) (lane above),
@@ -655,8 +676,9 @@ struct TBTEArgs {
     Strip s0[4], s1[4];       // Hxy Hyx Ezx Ezy: read / write buffers
     int pec_xlo, pec_xhi, pec_ylo, pec_yhi;
     int nseg;                 // wavefront rectangles, as in TBTMArgs
-    int seg_x0[4], seg_x1[4], seg_wy0[4], seg_nwy[4], seg_end[4], seg_lx[4];
+    int seg_x0[8], seg_x1[8], seg_wy0[8], seg_nwy[8], seg_end[8], seg_lx[8];
     unsigned seg_exc;
+    unsigned seg_m3;
     StepArgs st[kMaxTB];
 };
 
@@ -958,6 +980,22 @@ __global__ void __launch_bounds__(256) te_tb_mixed_kernel(TBTEArgs a, Excite x, 
     else te_tb_body<T, MODE, 0>(a, x, a.st, wv, fp64);
 }
 
+template <int T, int EXC>
+__global__ void __launch_bounds__(256) te_tb_slab_kernel(TBTEArgs a, Excite x, int fp64) {
+    __builtin_amdgcn_s_setprio(2);
+    const int wv = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = tb_segment(a.seg_end, a.nseg, wv);
+    if (g < 0) return;
+    const bool m3 = (a.seg_m3 >> g) & 1u, e = EXC != 0 && ((a.seg_exc >> g) & 1u);
+    if (m3) {
+        if (e) te_tb_body<T, 3, EXC>(a, x, a.st, wv, fp64);
+        else te_tb_body<T, 3, 0>(a, x, a.st, wv, fp64);
+    } else {
+        if (e) te_tb_body<T, 1, EXC>(a, x, a.st, wv, fp64);
+        else te_tb_body<T, 1, 0>(a, x, a.st, wv, fp64);
+    }
+}
+
 // Monitored passes: the plain pass plus the monitor work of TBMon (instantiated in
 // tb2d_mon.hip for depths 2..4 and MON in {9, 14}).
 template <int T, int MODE, bool EXC, int MON>
@@ -976,6 +1014,10 @@ __global__ void __launch_bounds__(256) te_tb_mon_kernel(TBTEArgs a, Excite x, TB
 
 // launchers of the plain passes (tb2d_pass.hip): depth T in {2, 4, 6, 8}, mode 0 / 1 /
 // 3; exc 0 without excitation code, 1 with it, 2 per rectangle (modes 1 and 3)
+void launch_tm_tb_slab(int T, int exc, const TBTMArgs &q, const Excite &x, int fp64, hipStream_t st,
+                       unsigned nb);
+void launch_te_tb_slab(int T, int exc, const TBTEArgs &q, const Excite &x, int fp64, hipStream_t st,
+                       unsigned nb);
 void launch_tm_tb(int T, int mode, int exc, const TBTMArgs &q, const Excite &x, int fp64,
                   hipStream_t st, unsigned nb);
 void launch_te_tb(int T, int mode, int exc, const TBTEArgs &q, const Excite &x, int fp64,

@@ -20,6 +20,10 @@ template <int T, int MODE, int EXC>
 static void go_mixed(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
     tm_tb_mixed_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
+template <int T, int EXC>
+static void go_slab(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
+    tm_tb_slab_kernel<T, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
+}
 #else
 using TB2Args = TBTEArgs;
 template <int T, int MODE, int EXC>
@@ -29,6 +33,10 @@ static void go(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsi
 template <int T, int MODE, int EXC>
 static void go_mixed(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
     te_tb_mixed_kernel<T, MODE, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
+}
+template <int T, int EXC>
+static void go_slab(const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
+    te_tb_slab_kernel<T, EXC><<<nb, 256, 0, st>>>(q, x, fp64);
 }
 #endif
 
@@ -59,6 +67,43 @@ static void by_mode(int mode, int exc, int em, const TB2Args &q, const Excite &x
     else by_em<T, 3>(exc, em, q, x, fp64, st, nb);
 }
 
+// excitation code a context's passes need: 0 none, 1 source only, 4 drive only with the
+// float64 blend, 3 anything else (a source or drive that the stepped component does not
+// carry is never applied)
+static int excitation_code(const Excite &x) {
+#if LF_TB2P_TM
+    const int comp = LF_EY;
+#else
+    const int comp = LF_EZ;
+#endif
+    const bool src = x.amp != nullptr && x.src_comp == comp;
+    const bool drv = x.drv[comp] != nullptr && x.drv_n > 0;
+    return !src && !drv ? 0 : !drv ? 1 : (!src && x.drv_math == LF_DRIVE_C128) ? 4 : 3;
+}
+
+// the host merges the slab classes only without excitation or with the drive alone (their
+// bodies then hold two waves per SIMD together); the generic body serves any other case
+template <int T>
+static void slab_by_em(int em, const TB2Args &q, const Excite &x, int fp64, hipStream_t st, unsigned nb) {
+    if (em == 0) go_slab<T, 0>(q, x, fp64, st, nb);
+    else if (em == 4) go_slab<T, 4>(q, x, fp64, st, nb);
+    else go_slab<T, 3>(q, x, fp64, st, nb);
+}
+
+#if LF_TB2P_TM
+void launch_tm_tb_slab(int T, int exc, const TBTMArgs &q, const Excite &x, int fp64, hipStream_t st,
+                       unsigned nb) {
+#else
+void launch_te_tb_slab(int T, int exc, const TBTEArgs &q, const Excite &x, int fp64, hipStream_t st,
+                       unsigned nb) {
+#endif
+    const int em = exc ? excitation_code(x) : 0;
+    if (T == 2) slab_by_em<2>(em, q, x, fp64, st, nb);
+    else if (T == 4) slab_by_em<4>(em, q, x, fp64, st, nb);
+    else if (T == 6) slab_by_em<6>(em, q, x, fp64, st, nb);
+    else slab_by_em<8>(em, q, x, fp64, st, nb);
+}
+
 #if LF_TB2P_TM
 void launch_tm_tb(int T, int mode, int exc, const TBTMArgs &q, const Excite &x, int fp64,
                   hipStream_t st, unsigned nb) {
@@ -66,15 +111,7 @@ void launch_tm_tb(int T, int mode, int exc, const TBTMArgs &q, const Excite &x, 
 void launch_te_tb(int T, int mode, int exc, const TBTEArgs &q, const Excite &x, int fp64,
                   hipStream_t st, unsigned nb) {
 #endif
-#if LF_TB2P_TM
-    const int comp = LF_EY;
-#else
-    const int comp = LF_EZ;
-#endif
-    // a source or drive that the stepped component does not carry is never applied
-    const bool src = x.amp != nullptr && x.src_comp == comp;
-    const bool drv = x.drv[comp] != nullptr && x.drv_n > 0;
-    const int em = !src && !drv ? 0 : !drv ? 1 : (!src && x.drv_math == LF_DRIVE_C128) ? 4 : 3;
+    const int em = excitation_code(x);
     if (em == 0) exc = 0;
     if (T == 2) by_mode<2>(mode, exc, em, q, x, fp64, st, nb);
     else if (T == 4) by_mode<4>(mode, exc, em, q, x, fp64, st, nb);

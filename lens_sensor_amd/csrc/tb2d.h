@@ -280,7 +280,7 @@ __device__ __forceinline__ double2 tb_dft_add(double2 v, float F, int acc, doubl
 // `own`: the cell is an owned cell of the wavefront (owned lane AND owned plane): a halo
 // plane below the chunk holds invalid values on the higher levels, so only the owner
 // stages a flux plane.
-template <int T, int MON>
+template <int T, int MON, bool XIN = false, bool YIN = false>
 __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *m, int s, int q,
                                              int j, int Nx, int Ny, bool own, float exn, float eyn,
                                              float exq1, float hz, float hzq1) {
@@ -289,10 +289,17 @@ __device__ __forceinline__ void tm_mon_level(TBMonRegs<T, MON> &r, const TBMon *
     if (!FIX && bits == 0u) return;
     constexpr int ND = TBMonRegs<T, MON>::ND, NS = TBMonRegs<T, MON>::NS;
     const float eyp = lane_prev(eyn);                           // Ey(q, j-1)
-    auto node_ey = [&]() { return j == 0 ? eyn : (j == Ny - 1 ? eyp : 0.5f * (eyp + eyn)); };
-    auto node_ex = [&]() { return q == 0 ? exn : (q == Nx - 1 ? exq1 : 0.5f * (exq1 + exn)); };
+    // XIN / YIN: the wavefront's planes / lanes lie strictly inside the grid, so only the
+    // interior averages apply (planes outside the march are never summed into an owned cell)
+    auto node_ey = [&]() {
+        return YIN ? 0.5f * (eyp + eyn) : j == 0 ? eyn : (j == Ny - 1 ? eyp : 0.5f * (eyp + eyn));
+    };
+    auto node_ex = [&]() {
+        return XIN ? 0.5f * (exq1 + exn) : q == 0 ? exn : (q == Nx - 1 ? exq1 : 0.5f * (exq1 + exn));
+    };
     auto node_ex2 = [&]() {
-        return q == 0 ? exn * exn : (q == Nx - 1 ? exq1 * exq1 : 0.5f * (exq1 * exq1 + exn * exn));
+        return XIN ? 0.5f * (exq1 * exq1 + exn * exn)
+                   : q == 0 ? exn * exn : (q == Nx - 1 ? exq1 * exq1 : 0.5f * (exq1 * exq1 + exn * exn));
     };
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
@@ -546,7 +553,7 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                     eyn = v;
                 }
                 if constexpr (MON != 0)
-                    tm_mon_level<T, MON>(mr, m, s, q, j, Nx, Ny, own && q >= X0 && q < X1, exn,
+                    tm_mon_level<T, MON, XIN, YIN>(mr, m, s, q, j, Nx, Ny, own && q >= X0 && q < X1, exn,
                                          eyn, exP[s + 1], hz, hzP[s + 1]);
                 exC[s + 1] = exn; eyC[s + 1] = eyn; hzC[s + 1] = hz;
                 hxC[s + 1] = psx; hyC[s + 1] = psy; eyyC[s + 1] = qy; exxC[s + 1] = qx;

@@ -1744,33 +1744,6 @@ TBPlanIn tb_plan_in(const lf_ctx *c) {
 
 TBPlan tb_plan_2d(const lf_ctx *c, int T, int lx) { return tb_plan_2d(tb_plan_in(c), T, lx); }
 
-template <typename A>
-void set_class(A &q, const TBClass &k) {
-    q.nseg = k.nseg;
-    q.seg_exc = 0u;
-    q.seg_m3 = 0u;
-    for (int u = 0; u < 4; ++u) {
-        q.seg_x0[u] = k.x0[u]; q.seg_x1[u] = k.x1[u]; q.seg_lx[u] = k.lx[u];
-        q.seg_wy0[u] = k.wy0[u]; q.seg_nwy[u] = k.nwy[u]; q.seg_end[u] = k.end[u];
-        if (u < k.nseg) q.seg_exc |= (unsigned)(k.sexc[u] != 0) << u;
-    }
-}
-
-// the y-slab class (MODE 1) then the x-slab class (MODE 3) as the rectangles of one launch
-template <typename A>
-void set_slabs(A &q, const TBClass &k1, const TBClass &k3) {
-    set_class(q, k1);
-    const int base = k1.nseg ? k1.end[k1.nseg - 1] : 0;
-    for (int u = 0; u < k3.nseg; ++u) {
-        const int g = k1.nseg + u;
-        q.seg_x0[g] = k3.x0[u]; q.seg_x1[g] = k3.x1[u]; q.seg_lx[g] = k3.lx[u];
-        q.seg_wy0[g] = k3.wy0[u]; q.seg_nwy[g] = k3.nwy[u]; q.seg_end[g] = base + k3.end[u];
-        q.seg_exc |= (unsigned)(k3.sexc[u] != 0) << g;
-        q.seg_m3 |= 1u << g;
-    }
-    q.nseg = k1.nseg + k3.nseg;
-}
-
 int ensure_side_streams(lf_ctx *c) {
     for (int u = 0; u < 3; ++u)
         if (!c->side[u]) LF_HIP(hipStreamCreate(&c->side[u]));

@@ -151,4 +151,34 @@ inline TBPlan tb_plan_2d(const TBPlanIn &in, int T, int lx) {
     return pl;
 }
 
+// Launch arguments of a class (A: the pass argument struct, seg_* fields of up to eight
+// rectangles): its rectangles in order, seg_exc bit g for a rectangle with excitation code.
+template <typename A>
+void set_class(A &q, const TBClass &k) {
+    q.nseg = k.nseg;
+    q.seg_exc = 0u;
+    q.seg_m3 = 0u;
+    for (int u = 0; u < 4; ++u) {
+        q.seg_x0[u] = k.x0[u]; q.seg_x1[u] = k.x1[u]; q.seg_lx[u] = k.lx[u];
+        q.seg_wy0[u] = k.wy0[u]; q.seg_nwy[u] = k.nwy[u]; q.seg_end[u] = k.end[u];
+        if (u < k.nseg) q.seg_exc |= (unsigned)(k.sexc[u] != 0) << u;
+    }
+}
+
+// the y-slab class (MODE 1) then the x-slab class (MODE 3) as the rectangles of one launch
+template <typename A>
+void set_slabs(A &q, const TBClass &k1, const TBClass &k3) {
+    set_class(q, k1);
+    const int base = k1.nseg ? k1.end[k1.nseg - 1] : 0;
+    for (int u = 0; u < k3.nseg; ++u) {
+        const int g = k1.nseg + u;
+        q.seg_x0[g] = k3.x0[u]; q.seg_x1[g] = k3.x1[u]; q.seg_lx[g] = k3.lx[u];
+        q.seg_wy0[g] = k3.wy0[u]; q.seg_nwy[g] = k3.nwy[u]; q.seg_end[g] = base + k3.end[u];
+        q.seg_exc |= (unsigned)(k3.sexc[u] != 0) << g;
+        q.seg_m3 |= 1u << g;
+    }
+    q.nseg = k1.nseg + k3.nseg;
+}
+
+
 }  // namespace lfd

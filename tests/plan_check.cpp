@@ -6,7 +6,10 @@
 //   - the slab-free bodies (MODE 0) load only planes and lanes inside 1..N-2 and outside
 //     the slabs, the y-slab body (MODE 1) only such planes,
 //   - a rectangle without excitation code has no source or drive plane within its marches.
+//   - the y-slab and x-slab classes packed as one slab launch (set_slabs) hold exactly
+//     their waves, each with its class's body (x-slab bit) and excitation flag.
 // Prints one line per problem (at most a few per plan) and the totals.
+#include <array>
 #include <cstdio>
 #include <vector>
 
@@ -61,6 +64,38 @@ static int check(const TBPlanIn &in, int T, int lx) {
     }
     for (size_t i = 0; i < cov.size(); ++i)
         if (cov[i] != 1) { err(cov[i] ? "covered twice" : "not covered", -1, -1, (int)(i / Ny), (int)(i % Ny)); break; }
+    // the slab launch: every wave of classes 1 and 2, in order, with its class's body and
+    // excitation flag, found the way the slab kernel finds it (segment, then chunk and wave)
+    if (pl.cls[1].nseg + pl.cls[2].nseg > 0) {
+        struct Segs {
+            int nseg;
+            int seg_x0[8], seg_x1[8], seg_wy0[8], seg_nwy[8], seg_end[8], seg_lx[8];
+            unsigned seg_exc, seg_m3;
+        } q{};
+        set_slabs(q, pl.cls[1], pl.cls[2]);
+        std::vector<std::array<int, 5>> want, got;
+        for (int k : {1, 2}) {
+            const TBClass &c = pl.cls[k];
+            int prev = 0;
+            for (int g = 0; g < c.nseg; ++g) {
+                const bool exc = c.exc == 1 || (c.exc == 2 && c.sexc[g]);
+                for (int loc = 0; loc < c.end[g] - prev; ++loc) {
+                    const int X0 = c.x0[g] + (loc / c.nwy[g]) * c.lx[g];
+                    want.push_back({k == 2, exc, X0, std::min(X0 + c.lx[g], c.x1[g]), c.wy0[g] + loc % c.nwy[g]});
+                }
+                prev = c.end[g];
+            }
+        }
+        for (int wv = 0; wv < q.seg_end[q.nseg - 1]; ++wv) {
+            int g = 0;
+            while (g < q.nseg - 1 && wv >= q.seg_end[g]) ++g;
+            const int loc = wv - (g > 0 ? q.seg_end[g - 1] : 0);
+            const int X0 = q.seg_x0[g] + (loc / q.seg_nwy[g]) * q.seg_lx[g];
+            got.push_back({(int)((q.seg_m3 >> g) & 1u), (int)((q.seg_exc >> g) & 1u), X0,
+                           std::min(X0 + q.seg_lx[g], q.seg_x1[g]), q.seg_wy0[g] + loc % q.seg_nwy[g]});
+        }
+        if (q.nseg > 8 || got != want) err("slab launch", 12, q.nseg, (int)got.size(), (int)want.size());
+    }
     return bad;
 }
 

@@ -378,6 +378,9 @@ def main():
                data='synthetic', config=cfg, device=dev,
                roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s',
                              frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=kname,
+                             # measured HBM bytes per launch over the same launch time
+                             hbm_gbs=(traffic / (launch_ms * 1e-3) / 1e9) if traffic else None,
+                             hbm_frac=(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                              launch_ms=launch_ms, cell_updates_per_launch=units,
                              bytes_per_cell_update=res['bytes_per_cell'],
                              update_ms_per_step=res['upd_ms_per_step']),

@@ -13,8 +13,13 @@ all faces, 4-layer stack + lipid, one bead, Gaussian CW source, 8000 steps, FP64
 sums on one monitor plane; the detector boxes of every step are summed inside the
 temporally blocked passes (depth 3), the steps of the final CW window run one per launch.
 
-Prints ONE JSON line (metric, value, roofline of the dominant kernel, CPU baseline of the
-NumPy oracle on a bounded sample of the same workload).
+Prints ONE JSON line: metric, value, roofline of the dominant kernel (the blocked pass;
+frac from the counter-measured HBM bytes of the pass), CPU baseline of the NumPy oracle
+on a bounded sample of the same workload.  The default run adds, in the same line:
+  yee3d        the 3D workload above, with its own roofline and CPU baseline;
+  fp64_rel_l2  relative L2 of the device's CW phasors against the FP64 oracle on a
+               cropped copy of the stitch workload (2 windows of 1024 x 512, 20000 steps
+               each, the same blocking), from tests/golden/fp64_tm_stitch_crop.npz.
 """
 import argparse
 import contextlib
@@ -34,11 +39,11 @@ HBM_PEAK_GBS = 8000.0
 # coefficient read once (psi slabs, sources and monitors not counted)
 BYTES_TM2D_LOSSY = 3 * 8 + 4 * 4      # Ex, Ey, Hz r/w + Ca_ex, Ca_ey, D_ex, D_ey = 40 B
 BYTES_YEE3D = 6 * 8 + 3 * 4           # 6 fields r/w + ce_x, ce_y, ce_z = 60 B
-# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
+# HBM bytes per blocked pass (all its launches) from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), scaled by the same counters on a plain 1 GiB copy
-# (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth);
-# the 2D stitch figure is for the x chunk of 128 planes the autotuner keeps there.
-TRAFFIC_BYTES = {('tm2d_stitch', 8): 655.7e6}
+# (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth,
+# x chunk): a pass of another blocking setting has no entry and reports traffic None.
+TRAFFIC_BYTES = {}
 
 
 def _timed_factory(records, every=1, fixed=None):
@@ -63,7 +68,21 @@ def _timed_factory(records, every=1, fixed=None):
             self._tb_depth = 1
             self._tb_units = 0
             self._calls = 0
+            if not hasattr(self, '_cfg'):
+                self._cfg = (None, None)
             records.append(self)
+
+        def set_temporal(self, depth, lx=64):
+            super().set_temporal(depth, lx)
+            self._cfg = (depth, lx)
+
+        def autotune(self, row, reps=None):
+            d, lx, ld, ms = super().autotune(row, reps)
+            self._cfg = (d, lx)
+            return d, lx, ld, ms
+
+        def tb_lx(self):
+            return self._cfg[1]
 
         def run(self, tab):
             on = self._calls % every == every // 2
@@ -86,33 +105,20 @@ def _timed_factory(records, every=1, fixed=None):
     return TimedHip
 
 
-def bench_tm2d_stitch(args):
+def bench_tm2d_stitch(args, warmup=False):
+    """The stitch workload (stitch_bench.run) on engines that record HIP-event times.
+    warmup: one window of ``args.warmup`` steps, untimed (loads the code objects, runs
+    the autotuner's first trials)."""
     from lens_sensor_amd.common import backend
-    from lens_sensor_amd.planar_beads import fdtd2d_core as d2
-    rng = np.random.default_rng(args.seed)
-    dx_nm = 25.0
-    nx_win, ny = 8192, 1536
-    slide = 6144
+    from lens_sensor_amd.planar_beads import stitch_bench as sb
+    nx_win, ny, slide = 8192, 1536, 6144
     n_win = 10 if args.windows is None else args.windows
     steps = 20000 if args.steps_override is None else args.steps_override
-    dx_um = dx_nm * 1e-3
-    length_um = (nx_win + 9 * slide) * dx_um
-    t_wg = 2.0
-    tear_um = ny * dx_um - t_wg - d2.AIR_BUFFER * 1e6
-    stack = [dict(name='Aqueous', t_um=3.0, n=complex(1.331, 0.001), kappa=0.001, mat=None),
-             dict(name='Mucin', t_um=0.5, n=complex(1.34, 0.0), kappa=0.0, mat=None)]
-    beads = []
-    for w in range(10):
-        x0 = w * slide * dx_um
-        for _ in range(int(rng.integers(0, 9))):
-            r = float(rng.uniform(0.2, 0.6))
-            beads.append(dict(x_um=float(x0 + rng.uniform(r, nx_win * dx_um - r)), d_um=2 * r,
-                              y_um=-r, n=1.59))
+    if warmup:
+        n_win, steps = 1, args.warmup
     records = []
     info = []
-    old_k, old_tb, old_at = d2.KAPPA, d2.TEAR_BUFFER, backend.AUTOTUNE
-    d2.KAPPA = {'bead': 0.002}
-    d2.TEAR_BUFFER = tear_um * 1e-6
+    old_at = backend.AUTOTUNE
     fixed = None
     if args.temporal:
         fixed = tuple(int(v) for v in args.temporal.split(','))
@@ -121,16 +127,13 @@ def bench_tm2d_stitch(args):
     try:
         with backend.use_engine(_timed_factory(records, every=5, fixed=fixed)), \
                 contextlib.redirect_stdout(io.StringIO()):
-            r = d2.run_beads_stitched(
-                'pmma', 'polystyrol', 0.0, dx_nm, False, 2, t_wg, 850.0, 3.0, 0.0, 0.0, 'cw',
-                nx_win * dx_um, slide * dx_um, place_bead=False, length_um=length_um,
-                wg_n=1.49, n_cornea=1.376, polarization='p', stack=stack, npml=24,
-                engine_mode='fp64acc', beads=beads, steps_win_override=steps,
-                n_windows=n_win, window_hook=lambda w, i: (info.append(i),
-                                                           print(f'[bench] window {w} {i}',
-                                                                 file=sys.stderr, flush=True)))
+            sb.run(nx_win=nx_win, ny=ny, slide=slide, n_windows=n_win, steps=steps,
+                   seed=args.seed, engine_mode='fp64acc',
+                   window_hook=lambda w, i: (info.append(i),
+                                             print(f'[bench] window {w} {i}', file=sys.stderr,
+                                                   flush=True)))
     finally:
-        d2.KAPPA, d2.TEAR_BUFFER, backend.AUTOTUNE = old_k, old_tb, old_at
+        backend.AUTOTUNE = old_at
     wall = time.time() - t0
     cells = [i['cells'] for i in info]
     step_s = sum(i['seconds'] for i in info)
@@ -143,7 +146,40 @@ def bench_tm2d_stitch(args):
                 bytes_per_cell=BYTES_TM2D_LOSSY, windows=len(info), grid=(nx_win, ny),
                 steps_per_window=steps, tb_ms=sum(e._tb_ms for e in records),
                 tb_launches=sum(e._tb_n for e in records), tb_depth=records[0]._tb_depth,
-                tb_units=sum(e._tb_units for e in records), kernel='tm_tb_kernel')
+                tb_lx=records[0].tb_lx(), tb_units=sum(e._tb_units for e in records),
+                kernel='tm_tb_kernel')
+
+
+def fp64_companion():
+    """Relative L2 error of the device path against the FP64 oracle on the cropped copy of
+    the stitch workload (two 1024 x 512 windows, 20000 steps each, depth 8 and x chunk
+    128 as the full run), from the committed FP64 fixture."""
+    from lens_sensor_amd.common import backend
+    from lens_sensor_amd.engine.hip import HipEngine
+    from lens_sensor_amd.planar_beads import stitch_bench as sb
+    path = os.path.join(HERE, 'tests', 'golden', 'fp64_tm_stitch_crop.npz')
+    ref = np.load(path)
+
+    def blocked(kind, dims, mode):
+        e = HipEngine(kind, dims, mode)
+        e.set_temporal(8, 128)
+        return e
+    old_at = backend.AUTOTUNE
+    backend.AUTOTUNE = False
+    try:
+        with backend.use_engine(blocked), contextlib.redirect_stdout(io.StringIO()):
+            r = sb.run(engine_mode='fp64acc', return_efull=True, **sb.CROP)
+    finally:
+        backend.AUTOTUNE = old_at
+    sx, sy = sb.CROP_SUB
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return dict(efull=rel(np.asarray(r['Efull'])[::sx, ::sy], ref['efull']),
+                ex_phasor=rel(np.asarray(r['Ex_cw_amp'])[::sx, ::sy], ref['ex_amp']),
+                transmission=abs(r['transmission'] - float(ref['transmission'])) / abs(float(ref['transmission'])),
+                bound=1e-4, case='2D TM stitch, 2 windows of 1024x512, 20000 steps each, depth 8, '
+                                 'x chunk 128, fp64acc; vs the f64 oracle (every 4th cell)')
 
 
 def bench_yee3d(args):
@@ -175,7 +211,7 @@ def bench_yee3d(args):
                 upd_ms_per_step=e._upd / max(e._n, 1), cells_per_step=float(cells),
                 bytes_per_cell=BYTES_YEE3D, grid=(nx, ny, nz), steps_per_window=steps,
                 tb_ms=e._tb_ms, tb_launches=e._tb_n, tb_depth=e._tb_depth,
-                tb_units=e._tb_units, kernel='yee_tb_kernel',
+                tb_lx=e.tb_lx(), tb_units=e._tb_units, kernel='yee_tb_kernel',
                 pass_desc='one blocked pass, persistent workgroups')
 
 
@@ -283,7 +319,8 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=None,
                     help='accepted for the driver contract: the workload fixes its steps')
-    ap.add_argument('--warmup', type=int, default=0)
+    ap.add_argument('--warmup', type=int, default=16,
+                    help='tm2d_stitch: steps of one untimed warm-up window (0: none)')
     ap.add_argument('--workload', default='tm2d_stitch',
                     choices=('tm2d_stitch', 'yee3d', 'sweep', 'serve'))
     ap.add_argument('--requests', type=int, default=24, help='serve: requests in the stream')
@@ -295,6 +332,9 @@ def main():
                          '(profiling runs of one configuration)')
     ap.add_argument('--seed', type=int, default=2024)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-3d', action='store_true', help='tm2d_stitch: skip the 3D record')
+    ap.add_argument('--no-fp64', action='store_true',
+                    help='tm2d_stitch: skip the FP64-error companion run')
     ap.add_argument('--copy-probe', type=int, default=None, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (byte-counter calibration)')
     args = ap.parse_args()
@@ -347,6 +387,10 @@ def main():
                    setup_s_per_pass=r['setup_s'])
         print(json.dumps(out), flush=True)
         return
+    warm = 0
+    if args.warmup > 0 and args.workload == 'tm2d_stitch':
+        bench_tm2d_stitch(args, warmup=True)
+        warm = args.warmup
     if args.workload == 'tm2d_stitch':
         res = bench_tm2d_stitch(args)
         cfg = dict(workload='tm2d_stitch_10x8192x1536', windows=res['windows'],
@@ -357,37 +401,70 @@ def main():
         cfg = dict(workload='yee3d_512x256x256', grid=list(res['grid']),
                    steps=res['steps_per_window'], dx_nm=30.0, npml=12, fields='fp32',
                    phasors='fp64')
+    out = dict(metric='Mcell-updates/s', value=None, unit='Mcell-updates/s', n_gpus=1,
+               steps=res['steps'], warmup=warm, higher_is_better=True, scaling='weak',
+               vs_baseline=None, dtype='fp32', data='synthetic', config=cfg, device=dev)
+    out.update(_line(res, args.workload))
+    if args.workload == 'tm2d_stitch' and not args.no_3d and args.windows is None \
+            and args.steps_override is None and not args.temporal:
+        # the 3D FP32 Yee + CPML step (512 x 256 x 256, 8000 steps) measured in the same run
+        r3 = bench_yee3d(args)
+        rec = dict(config=dict(workload='yee3d_512x256x256', grid=list(r3['grid']),
+                               steps=r3['steps_per_window'], dx_nm=30.0, npml=12, fields='fp32',
+                               phasors='fp64'))
+        rec.update(_line(r3, 'yee3d'))
+        if not args.no_cpu:
+            rec['cpu_baseline'] = cpu_baseline('yee3d')
+        out['yee3d'] = rec
+    if args.workload == 'tm2d_stitch' and not args.no_fp64:
+        out['fp64_rel_l2'] = fp64_companion()
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(args.workload)
+    print(json.dumps(out), flush=True)
+
+
+def _line(res, workload):
+    """value, ms_per_step and the roofline record of one stepping workload.
+
+    Dominant kernel: the temporally blocked pass (all its launches; HIP events on the
+    context stream bracket each pass).  One pass advances every cell by tb_depth steps.
+      traffic         HBM bytes per pass from rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE
+                      (copy-calibrated, TRAFFIC_BYTES), None where no profile matches
+      achieved / frac traffic / pass time, / 8 TB/s
+      algorithmic_*   the pass's minimum: every field read and written once and every
+                      coefficient read once per pass (bytes_per_cell_update x cells)
+      effective_gbs   bytes_per_cell_update x cell updates per pass / pass time: what a
+                      code stepping once per launch would have to move (exceeds the peak
+                      under temporal blocking)
+    """
     value = res['cell_updates'] / res['step_seconds'] / 1e6
     ms_step = res['step_seconds'] * 1e3 / max(1, res['steps'])
+    B = res['bytes_per_cell']
     if res['tb_launches']:
-        # dominant kernel: the temporally blocked pass (its slab and interior launches),
-        # tb_depth cell updates per cell per launch, HIP events on the context stream
         launch_ms = res['tb_ms'] / res['tb_launches']
-        units = res['tb_units'] / res['tb_launches']             # cell updates per launch
+        units = res['tb_units'] / res['tb_launches']             # cell updates per pass
+        cells = units / res['tb_depth']
         kname = (f"{res['kernel']}<{res['tb_depth']}> "
                  f"({res.get('pass_desc', 'one blocked pass: slab + interior launches')})")
     else:
         launch_ms = res['upd_ms_per_step']
-        units = res['cells_per_step']
+        units = cells = res['cells_per_step']
         kname = f"{res['kernel']} (one step)"
-    achieved = res['bytes_per_cell'] * units / (launch_ms * 1e-3) / 1e9
-    traffic = TRAFFIC_BYTES.get((args.workload, res['tb_depth']))
-    out = dict(metric='Mcell-updates/s', value=value, unit='Mcell-updates/s', n_gpus=1,
-               steps=res['steps'], warmup=args.warmup, ms_per_step=ms_step,
-               higher_is_better=True, scaling='weak', vs_baseline=None, dtype='fp32',
-               data='synthetic', config=cfg, device=dev,
-               roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s',
-                             frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=kname,
-                             # measured HBM bytes per launch over the same launch time
-                             hbm_gbs=(traffic / (launch_ms * 1e-3) / 1e9) if traffic else None,
-                             hbm_frac=(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
-                             launch_ms=launch_ms, cell_updates_per_launch=units,
-                             bytes_per_cell_update=res['bytes_per_cell'],
-                             update_ms_per_step=res['upd_ms_per_step']),
-               wall_s=res['wall'])
-    if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(args.workload)
-    print(json.dumps(out), flush=True)
+    sec = launch_ms * 1e-3
+    traffic = TRAFFIC_BYTES.get((workload, res['tb_depth'], res.get('tb_lx')))
+    alg_bytes = B * cells
+    hbm = traffic / sec / 1e9 if traffic else None
+    roof = dict(bound='hbm', achieved=hbm, peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=hbm / HBM_PEAK_GBS if hbm else None, traffic=traffic, kernel=kname,
+                depth=res['tb_depth'], x_chunk=res.get('tb_lx'), launch_ms=launch_ms,
+                cell_updates_per_launch=units, bytes_per_cell_update=B,
+                algorithmic_bytes_per_launch=alg_bytes,
+                algorithmic_gbs=alg_bytes / sec / 1e9,
+                algorithmic_frac=alg_bytes / sec / 1e9 / HBM_PEAK_GBS,
+                effective_gbs=B * units / sec / 1e9,
+                traffic_per_algorithmic=(traffic / alg_bytes) if traffic else None,
+                update_ms_per_step=res['upd_ms_per_step'])
+    return dict(value=value, ms_per_step=ms_step, roofline=roof, wall_s=res['wall'])
 
 
 if __name__ == '__main__':

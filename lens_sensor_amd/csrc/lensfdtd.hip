@@ -3136,6 +3136,9 @@ extern "C" int lf_autotune(lf_ctx *c, const lf_step *row, int reps, int *depth, 
 }
 
 // ============================================================================ batch engine (C ABI)
+constexpr long long kBatchMaxInstCells = 1ll << 20;    // cells per instance
+constexpr long long kBatchMaxCells = 8ll << 20;        // cells per batch
+
 struct lf_batch {
     std::vector<lf_ctx *> ctx;
     hipStream_t stream = nullptr, side[2] = {};
@@ -3185,10 +3188,45 @@ void launch_batch_class(int kind, int mode, const void *rec, const int *start, i
 // MODE of a 2D pass class (classes 0 and 3 are both slab-free)
 int pl_mode(int cls) { return cls == 1 ? 1 : (cls == 2 ? 3 : 0); }
 
+// Host-side checks of one batch record before it goes to the device: the buffers the
+// body reads and writes exist, its rectangles lie in the grid and match their wave
+// counts, the slab-free rectangles (MODE 0, unclamped loads) keep every loaded plane and
+// lane inside 1..N-2 and outside the slabs, and its T steps lie inside the resident table.
+template <typename A>
+int check_batch_rec(const lf_batch *b, const lf_ctx *c, const A &a, const TBClass &k, int T,
+                    const lf_step *steps) {
+    const TBPlanIn in = tb_plan_in(c);
+    const int Nx = c->g.nx, Ny = c->g.nz, W = 64 - 2 * T;
+    const int ntab = (int)b->h_steps.size();
+    if (steps < b->d_steps || steps + T > b->d_steps + ntab)
+        return fail(-1, "batch pass: step rows outside the resident table");
+    for (int q = 0; q < 4; ++q)
+        if ((a.s0[q].psi == nullptr) != (a.s1[q].psi == nullptr))
+            return fail(-1, "batch pass: psi slab without its second buffer");
+    int prev = 0;
+    for (int g = 0; g < k.nseg; ++g) {
+        const int x0 = k.x0[g], x1 = k.x1[g], L = k.lx[g];
+        if (x0 < 0 || x1 > Nx || x1 <= x0 || L < 1 || k.nwy[g] < 1 || k.wy0[g] < 0 ||
+            (k.wy0[g] + k.nwy[g] - 1) * W - T >= Ny)
+            return fail(-1, "batch pass: rectangle outside the grid");
+        if ((long long)(x1 - x0 + L - 1) / L * k.nwy[g] != (long long)k.end[g] - prev)
+            return fail(-1, "batch pass: rectangle wave count");
+        prev = k.end[g];
+        if (k.mode == 0) {
+            const int jlo = k.wy0[g] * W - T, jhi = (k.wy0[g] + k.nwy[g] - 1) * W - T + 63;
+            if (x0 - T < std::max(in.xlo, 1) || x1 + T - 1 > std::min(in.xhi, Nx - 2) ||
+                jlo < std::max(in.ylo, 1) || jhi > std::min(in.yhi, Ny - 2) || jhi >= c->g.P)
+                return fail(-1, "batch pass: slab-free rectangle reaches a slab or an edge");
+        }
+    }
+    return 0;
+}
+
 // one blocked pass of depth T over the listed instances; dsteps[k]: device step table
 // position of ids[k] for this pass
 int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const lf_step *> &dsteps,
                int T, int fp64) {
+    if (T != 2 && T != 4 && T != 6 && T != 8) return fail(-1, "batch pass: depth must be 2, 4, 6 or 8");
     struct Launch { int kind, cls, mode, ninst, nwave; size_t rec_off, start_off; };
     std::vector<Launch> launches;
     std::vector<char> pack;                        // records and prefix tables of the pass
@@ -3211,6 +3249,7 @@ int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const
                     set_class(r.a, k);
                     r.x = c->ex;
                     r.steps = dsteps[n];
+                    if (int e = check_batch_rec(b, c, r.a, k, T, r.steps)) return e;
                     rtm.push_back(r);
                 } else {
                     TBRecTE r{};
@@ -3218,8 +3257,11 @@ int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const
                     set_class(r.a, k);
                     r.x = c->ex;
                     r.steps = dsteps[n];
+                    if (int e = check_batch_rec(b, c, r.a, k, T, r.steps)) return e;
                     rte.push_back(r);
                 }
+                if ((long long)start.back() + k.end[k.nseg - 1] > (1ll << 30))
+                    return fail(-1, "batch pass: too many wavefronts in one launch");
                 start.push_back(start.back() + k.end[k.nseg - 1]);
             }
             const int ninst = (int)start.size() - 1;
@@ -3241,6 +3283,9 @@ int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const
     b->ring = (b->ring + 1) % lf_batch::kRing;
     if (sg.used) LF_HIP(hipEventSynchronize(sg.done));    // its previous copy has completed
     if (pack.size() > sg.cap) {
+        // the slot's previous pass may still be reading its device records: every pass
+        // so far has joined the batch stream, so once that stream drains nothing reads it
+        LF_HIP(hipStreamSynchronize(b->stream));
         if (sg.h) LF_HIP(hipHostFree(sg.h));
         if (sg.d) LF_HIP(hipFree(sg.d));
         sg.h = nullptr; sg.d = nullptr; sg.cap = 0;
@@ -3260,10 +3305,14 @@ int batch_pass(lf_batch *b, const std::vector<int> &ids, const std::vector<const
         hipStream_t st = (L.cls == 0 || L.cls == 3) ? b->stream : b->side[L.cls - 1];
         const void *rec = sg.d + L.rec_off;
         const int *start = (const int *)(sg.d + L.start_off);
+        // every depth has its own kernels: a pass planned for depth T must run the depth-T
+        // body, whose lanes and halo match the plan's wave layout (64 - 2T owned lanes)
         switch (T) {
             case 2: launch_batch_class<2>(L.kind, L.mode, rec, start, L.ninst, L.nwave, fp64, st); break;
             case 4: launch_batch_class<4>(L.kind, L.mode, rec, start, L.ninst, L.nwave, fp64, st); break;
-            default: launch_batch_class<6>(L.kind, L.mode, rec, start, L.ninst, L.nwave, fp64, st); break;
+            case 6: launch_batch_class<6>(L.kind, L.mode, rec, start, L.ninst, L.nwave, fp64, st); break;
+            case 8: launch_batch_class<8>(L.kind, L.mode, rec, start, L.ninst, L.nwave, fp64, st); break;
+            default: return fail(-1, "batch pass: depth must be 2, 4, 6 or 8");
         }
     }
     LF_HIP(hipGetLastError());
@@ -3306,10 +3355,8 @@ int lf_batch_add(lf_batch *b, lf_ctx *c) {
 
 int lf_batch_set_temporal(lf_batch *b, int depth, int lx) {
     if (int e = batch_check(b)) return e;
-    // depth 8 is not offered for batches: the batched TE pass of depth 8 gave wrong
-    // results on the device (cause not established); depths up to 6 are exact
-    if (depth != 1 && depth != 2 && depth != 4 && depth != 6)
-        return fail(-1, "lf_batch_set_temporal: depth must be 1 (off), 2, 4 or 6");
+    if (depth != 1 && depth != 2 && depth != 4 && depth != 6 && depth != 8)
+        return fail(-1, "lf_batch_set_temporal: depth must be 1 (off), 2, 4, 6 or 8");
     if (lx < 1 || lx > 1 << 20) return fail(-1, "lf_batch_set_temporal: lx out of range");
     b->tb = depth;
     b->tb_lx = lx;
@@ -3349,6 +3396,18 @@ int lf_batch_run_at(lf_batch *b, const int *offs, int nsteps, const unsigned *ma
         const int na = nact ? std::min(nact[i], nsteps) : nsteps;
         if (o < 0 || o + na > ntab) return fail(-1, "lf_batch_run_at: steps outside the table");
     }
+    // Size guard: a batched sweep of 64 instances of up to 16384 x 2048 cells ended with an
+    // illegal memory access whose cause is not established; the batch engine runs only
+    // within the instance and batch sizes its device tests cover.
+    long long total = 0;
+    for (lf_ctx *c : b->ctx) {
+        const long long cells = (long long)c->g.nx * c->g.nz;
+        total += cells;
+        if (cells > kBatchMaxInstCells)
+            return fail(-1, "lf_batch_run: instance larger than the tested batch size (1M cells)");
+    }
+    if (total > kBatchMaxCells)
+        return fail(-1, "lf_batch_run: batch larger than the tested batch size (8M cells)");
     for (lf_ctx *c : b->ctx) {
         if (c->ch == 0.0f) return fail(-1, "lf_batch_run: Ch not set");
         if (c->kind == LF_TE2D && (!c->arr[LF_CEE] || !c->arr[LF_CEH]))

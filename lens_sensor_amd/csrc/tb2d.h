@@ -94,7 +94,9 @@ __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j
     v.dy = (lossy_y ? a.day : a.cay)[c];
     v.hz = a.Hz0[c];
     if (PX) {
-        const int sx0 = strip_pos(a.s0[0], pc), sx3 = strip_pos(a.s0[3], pc);
+        // psi_Hzx lives on the H nodes 0..Nx-2: plane Nx-1 (past the last H node, masked
+        // below) reads the slab at the last H node, so the load stays inside the slab
+        const int sx0 = strip_pos(a.s0[0], pc < Nx - 1 ? pc : Nx - 2), sx3 = strip_pos(a.s0[3], pc);
         const float *p0 = a.s0[0].psi ? a.s0[0].psi : a.Ex0, *p3 = a.s0[3].psi ? a.s0[3].psi : a.Ex0;
         v.ph = p0[(long long)(sx0 > 0 ? sx0 : 0) * P + jc];
         v.qx = p3[(long long)(sx3 > 0 ? sx3 : 0) * P + jc];
@@ -718,7 +720,8 @@ __device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int j
     v.hy = a.Hy0[c];
     v.hx = a.Hx0[c];
     if (PX) {
-        const int sx1 = strip_pos(a.s0[1], pc), sx2 = strip_pos(a.s0[2], pc);
+        // psi_Hyx lives on the H nodes 0..Nx-2, as psi_Hzx in the TM load
+        const int sx1 = strip_pos(a.s0[1], pc < Nx - 1 ? pc : Nx - 2), sx2 = strip_pos(a.s0[2], pc);
         const float *p1 = a.s0[1].psi ? a.s0[1].psi : a.Ez0, *p2 = a.s0[2].psi ? a.s0[2].psi : a.Ez0;
         v.pyx = p1[(long long)(sx1 > 0 ? sx1 : 0) * P + jc];
         v.pzx = p2[(long long)(sx2 > 0 ? sx2 : 0) * P + jc];

@@ -8,6 +8,14 @@ if REPO not in sys.path:
     sys.path.insert(0, REPO)
 
 
+_EMU_LIB = os.environ.get('LENSFDTD_EMU_LIB')
+if _EMU_LIB:
+    # CPU bounds-checking runs: the GPU tests drive the library built for the CPU under
+    # AddressSanitizer (tests/emu/build.py) instead of liblensfdtd.so
+    from lens_sensor_amd.engine import hip as _hip
+    _hip._LIB_PATH = _EMU_LIB
+
+
 def pytest_configure(config):
     config.addinivalue_line('markers', 'gpu: needs a HIP device and the built liblensfdtd.so')
     config.addinivalue_line('markers', 'slow: longer CPU runs')

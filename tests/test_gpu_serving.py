@@ -1,9 +1,18 @@
-"""Serving loop on the GPU: requests served together (2D through the batch engine, 3D
-overlapped on their own streams) give the same result as each request alone."""
+"""Serving loop on the GPU (needs an MI355X).
+
+* Requests served together (2D through the batch engine, 3D overlapped on their own
+  streams) give the same result as each request served alone with the plain
+  single-instance path (batched=False): bit for bit.
+* The served results agree with the CPU oracle serving each request alone: TE, TM and
+  3D requests, same step count and convergence decision, transmission and phasor norm
+  to 1e-5 relative (the fixture tolerance).  Coarser cells (50 nm in 2D, 60 nm in 3D)
+  keep the oracle's share of the test to seconds.
+"""
 import numpy as np
 import pytest
 
 from lens_sensor_amd import serving
+from oracle.engine import OracleEngine
 
 pytestmark = pytest.mark.gpu
 
@@ -15,14 +24,36 @@ REQS = [dict(id=0, arrival=0.0, kind='2d', dims=(700, 470), pol='s', bead_x_frac
              kappa_aq=0.0),
         dict(id=3, arrival=0.0, kind='3d', dims=(96, 128, 64), bead_x_frac=0.5, bead_r_um=0.3)]
 
+SMALL = [dict(id=0, arrival=0.0, kind='2d', dims=(200, 250), pol='s', bead_x_frac=0.5, bead_r_um=0.4,
+              kappa_aq=0.001),
+         dict(id=1, arrival=0.0, kind='2d', dims=(220, 240), pol='p', bead_x_frac=0.4, bead_r_um=0.5,
+              kappa_aq=0.002),
+         dict(id=2, arrival=0.0, kind='3d', dims=(48, 80, 32), bead_x_frac=0.5, bead_r_um=0.3)]
+
 
 def test_results_identical_alone_and_in_the_mix():
     mix, summ = serving.Server(round_steps=384).run(REQS, realtime=False)
     assert summ['requests'] == len(REQS)
     by_id = {r['id']: r for r in mix}
     for q in REQS:
-        one, _ = serving.Server(round_steps=384).run([q], realtime=False)
+        one, _ = serving.Server(round_steps=384, batched=False).run([q], realtime=False)
         a, b = one[0], by_id[q['id']]
         assert a['steps'] == b['steps'] and a['converged'] == b['converged']
         assert np.array_equal([a['transmission'], a['change'], a['phasor_norm']],
                               [b['transmission'], b['change'], b['phasor_norm']], equal_nan=True)
+
+
+def test_served_results_match_the_oracle(monkeypatch):
+    monkeypatch.setattr(serving.Solve2D, 'dx_nm', 50.0)
+    monkeypatch.setattr(serving.Solve3D, 'dx_nm', 60.0)
+    mix, summ = serving.Server(round_steps=256).run(SMALL, realtime=False)
+    assert summ['requests'] == len(SMALL)
+    by_id = {r['id']: r for r in mix}
+    for q in SMALL:
+        one, _ = serving.Server(round_steps=256, engine_factory=OracleEngine,
+                                batched=False).run([q], realtime=False)
+        a, b = one[0], by_id[q['id']]
+        assert a['steps'] == b['steps'] and a['converged'] == b['converged'], (q, a, b)
+        for k in ('transmission', 'phasor_norm'):
+            assert np.isclose(b[k], a[k], rtol=1e-5, atol=0.0), (q['id'], k, b[k], a[k])
+        assert np.isclose(b['change'], a['change'], rtol=1e-3), (q['id'], b['change'], a['change'])

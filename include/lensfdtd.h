@@ -148,7 +148,7 @@ typedef struct lf_batch lf_batch;
 int lf_batch_create(lf_batch **out);
 /* finalized 2D context, same precision mode for all instances */
 int lf_batch_add(lf_batch *b, lf_ctx *ctx);
-/* depth 1 (off), 2, 4 or 6 (default 6); lx = x planes per wavefront chunk */
+/* depth 1 (off), 2, 4, 6 or 8 (default 6); lx = x planes per wavefront chunk */
 int lf_batch_set_temporal(lf_batch *b, int depth, int lx);
 int lf_batch_set_timing(lf_batch *b, int on);
 /* steps: shared step table; masks: ninst x nsteps monitor masks or NULL (steps[].mask);
@@ -165,6 +165,60 @@ int lf_batch_run_at(lf_batch *b, const int *offs, int nsteps, const unsigned int
 /* last lf_batch_run with timing on: HIP-event time (ms) and blocked passes */
 int lf_batch_stats(lf_batch *b, double *ms, int *passes);
 int lf_batch_destroy(lf_batch *b);
+
+/* ---- Operator level: single CPML updates on caller-owned DEVICE arrays, in the
+ * reference's own C-order shapes (no padding), in place, enqueued on `stream` (a
+ * hipStream_t, NULL = the null stream).  The drop-in for a caller that keeps its own
+ * time loop over device arrays; lf_run is the fast path.  Per element the float32
+ * operations are the reference's, in its order. */
+typedef struct lf_op2d {
+    /* 1-D CPML profiles on the device: x on the Nx nodes (e) and Nx-1 half nodes (h),
+     * y likewise (boundaries.py:104-151) */
+    const float *bex, *cex, *kex, *bhx, *chx, *khx;
+    const float *bey, *cey, *key, *bhy, *chy, *khy;
+    /* the two psi arrays of the update (full-domain, the reference's shapes):
+     * te_update_H psi_Hxy, psi_Hyx; te_update_E psi_Ezx, psi_Ezy;
+     * tm_update_H psi_Hzx, psi_Hzy; tm_update_E psi_Exy, psi_Eyx */
+    float *psi0, *psi1;
+} lf_op2d;
+/* fused2d.te_update_H (planar_beads/fused2d.py:105-108; boundaries.py:186-195):
+ * Ez (Nx,Ny), Hx (Nx,Ny-1), Hy (Nx-1,Ny) */
+int lf_op_te_update_H(const lf_op2d *p, int Nx, int Ny, const float *Ez, float *Hx, float *Hy,
+                      float ch, void *stream);
+/* fused2d.te_update_E (fused2d.py:111-114; boundaries.py:197-205) */
+int lf_op_te_update_E(const lf_op2d *p, int Nx, int Ny, float *Ez, const float *Hx, const float *Hy,
+                      const float *ce_E, const float *ce_H, void *stream);
+/* fused2d.tm_update_H (fused2d.py:117-120; boundaries.py:240-248):
+ * Hz (Nx-1,Ny-1), Ex (Nx-1,Ny), Ey (Nx,Ny-1) */
+int lf_op_tm_update_H(const lf_op2d *p, int Nx, int Ny, const float *Ex, const float *Ey, float *Hz,
+                      float ch, void *stream);
+/* fused2d.tm_update_E (fused2d.py:123-126; boundaries.py:250-258) */
+int lf_op_tm_update_E(const lf_op2d *p, int Nx, int Ny, float *Ex, float *Ey, const float *Hz,
+                      const float *Ca_ex, const float *Ca_ey, void *stream);
+/* CPML_TE.terminate / CPML_TM.terminate (boundaries.py:207-215, :260-268): PEC on the
+ * ends that carry a PML, pec = {x lo, x hi, y lo, y hi} */
+int lf_op_te_terminate(int Nx, int Ny, float *Ez, const int pec[4], void *stream);
+int lf_op_tm_terminate(int Nx, int Ny, float *Ex, float *Ey, const int pec[4], void *stream);
+
+/* CPML_3D (planar_3d/boundaries3d.py:262-362): the 12 psi slabs live in the object
+ * (device memory, the PML slabs only, as _PsiStrip :201-251 keeps them). */
+typedef struct lf_op3d lf_op3d;
+int lf_op3d_create(int Nx, int Ny, int Nz, lf_op3d **out);
+/* axis 0..2: e_* on the N nodes, h_* on the N-1 half nodes, (kappa, b, c) as
+ * _cpml_bc_1d builds them (boundaries3d.py:150-194); PEC on the axis' faces */
+int lf_op3d_set_cpml(lf_op3d *op, int axis, const float *e_kappa, const float *e_b, const float *e_c,
+                     const float *h_kappa, const float *h_b, const float *h_c, int pec_lo, int pec_hi);
+int lf_op3d_finalize(lf_op3d *op);
+/* CPML_3D.step_H (boundaries3d.py:311-330): Ex (Nx-1,Ny,Nz), Ey (Nx,Ny-1,Nz),
+ * Ez (Nx,Ny,Nz-1), Hx (Nx,Ny-1,Nz-1), Hy (Nx-1,Ny,Nz-1), Hz (Nx-1,Ny-1,Nz) */
+int lf_op3d_step_H(lf_op3d *op, const float *Ex, const float *Ey, const float *Ez, float *Hx,
+                   float *Hy, float *Hz, float ch, void *stream);
+/* CPML_3D.step_E (boundaries3d.py:332-362): ce_* on the E grids; da_* the staggered loss
+ * factors (fdtd3d_core.py:606-613) or all three NULL */
+int lf_op3d_step_E(lf_op3d *op, float *Ex, float *Ey, float *Ez, const float *Hx, const float *Hy,
+                   const float *Hz, const float *ce_x, const float *ce_y, const float *ce_z,
+                   const float *da_x, const float *da_y, const float *da_z, void *stream);
+int lf_op3d_destroy(lf_op3d *op);
 
 #ifdef __cplusplus
 }

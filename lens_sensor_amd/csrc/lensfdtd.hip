@@ -42,6 +42,14 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+}  // namespace
+
+namespace lfd {
+void set_error(const char *msg) { g_err = msg; }
+}  // namespace lfd
+
+namespace {
+
 #define LF_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \

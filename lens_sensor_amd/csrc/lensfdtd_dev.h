@@ -7,6 +7,10 @@
 
 namespace lfd {
 
+// sets the message lf_last_error() returns (thread-local), for the translation units
+// whose entry points report their own errors
+void set_error(const char *msg);
+
 struct Strip {            // psi slab of one stretch axis: v < lo -> v, v >= hi -> v-hi+lo
     float *psi;
     int lo, hi, len;      // len = lo + (n - hi); psi == nullptr -> no slab

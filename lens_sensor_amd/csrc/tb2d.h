@@ -491,6 +491,11 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
             // s + 1) or past the grid depend on planes the march does not hold, and no
             // owned output depends on them (the halo cone), so computing them is harmless
             // and the level chain stays free of per-level branches and copies
+            // x profiles of plane q: wave-uniform values, read where every lane is active
+            const bool qh = PX && strip_pos(a.s0[0], q) >= 0, qe = PX && strip_pos(a.s0[3], q) >= 0;
+            float qhb = 0.f, qhc = 0.f, qhk = 1.f, qeb = 0.f, qec = 0.f, qek = 1.f;
+            if (qh) { qhb = plane_val(xhb, q); qhc = plane_val(xhc, q); qhk = plane_val(xhk, q); }
+            if (qe) { qeb = plane_val(xeb, q); qec = plane_val(xec, q); qek = plane_val(xek, q); }
             {
                 const float ex_o = exP[s], ey_o = eyP[s], hz_o = hzP[s];
                 const float ex_jp = lane_next0(ex_o);                // Ex(q, j+1)
@@ -500,9 +505,9 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                     const float dEyx = eyC[s] - ey_o;
                     const float dExy = ex_jp - ex_o;
                     float tx, ty;
-                    if (PX && strip_pos(a.s0[0], q) >= 0) {
-                        psx = plane_val(xhb, q) * psx + plane_val(xhc, q) * dEyx;
-                        tx = div_kappa(dEyx, plane_val(xhk, q)) + psx;
+                    if (qh) {
+                        psx = qhb * psx + qhc * dEyx;
+                        tx = div_kappa(dEyx, qhk) + psx;
                     } else {
                         tx = dEyx;
                     }
@@ -540,9 +545,9 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
                     if (XIN || (q >= 1 && q <= Nx - 2)) {
                         const float d = hz - hzP[s + 1];            // Hz(q) - Hz(q-1)
                         float t;
-                        if (PX && strip_pos(a.s0[3], q) >= 0) {
-                            qx = plane_val(xeb, q) * qx + plane_val(xec, q) * d;
-                            t = div_kappa(d, plane_val(xek, q)) + qx;
+                        if (qe) {
+                            qx = qeb * qx + qec * d;
+                            t = div_kappa(d, qek) + qx;
                         } else {
                             t = d;
                         }
@@ -868,6 +873,11 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
             // s + 1) or past the grid depend on planes the march does not hold, and no
             // owned output depends on them (the halo cone), so computing them is harmless
             // and the level chain stays free of per-level branches and copies
+            // x profiles of plane q: wave-uniform values, read where every lane is active
+            const bool qh = PX && strip_pos(a.s0[1], q) >= 0, qe = PX && strip_pos(a.s0[2], q) >= 0;
+            float qhb = 0.f, qhc = 0.f, qhk = 1.f, qeb = 0.f, qec = 0.f, qek = 1.f;
+            if (qh) { qhb = plane_val(xhb, q); qhc = plane_val(xhc, q); qhk = plane_val(xhk, q); }
+            if (qe) { qeb = plane_val(xeb, q); qec = plane_val(xec, q); qek = plane_val(xek, q); }
             {
                 const float ez_o = ezP[s];
                 const float ez_jp = lane_next0(ez_o);                // Ez(q, j+1)
@@ -887,9 +897,9 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                 if ((XIN || q < Nx - 1) && jin) {                   // Hy(q, j)
                     const float d = ezC[s] - ez_o;
                     float t;
-                    if (PX && strip_pos(a.s0[1], q) >= 0) {
-                        qyx = plane_val(xhb, q) * qyx + plane_val(xhc, q) * d;
-                        t = div_kappa(d, plane_val(xhk, q)) + qyx;
+                    if (qh) {
+                        qyx = qhb * qyx + qhc * d;
+                        t = div_kappa(d, qhk) + qyx;
                     } else {
                         t = d;
                     }
@@ -904,9 +914,9 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
                         const float dHy = hy - hyP[s + 1];          // Hy(q) - Hy(q-1)
                         const float dHx = hx - hx_jm;
                         float t;
-                        if (PX && strip_pos(a.s0[2], q) >= 0) {
-                            qzx = plane_val(xeb, q) * qzx + plane_val(xec, q) * dHy;
-                            t = div_kappa(dHy, plane_val(xek, q)) + qzx;
+                        if (qe) {
+                            qzx = qeb * qzx + qec * dHy;
+                            t = div_kappa(dHy, qek) + qzx;
                         } else {
                             t = dHy;
                         }

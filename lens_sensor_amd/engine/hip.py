@@ -31,7 +31,18 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_set_variant', 'lf_set_tile', 'lf_set_temporal', 'lf_tb_stats',
            'lf_batch_create', 'lf_batch_add', 'lf_batch_set_temporal', 'lf_batch_set_timing',
            'lf_batch_run', 'lf_batch_stats', 'lf_batch_destroy', 'lf_batch_set_table',
-           'lf_batch_run_at', 'lf_autotune', 'lf_set_temporal_mon', 'lf_set_tb_lead')
+           'lf_batch_run_at', 'lf_autotune', 'lf_set_temporal_mon', 'lf_set_tb_lead',
+           'lf_op_te_update_H', 'lf_op_te_update_E', 'lf_op_tm_update_H', 'lf_op_tm_update_E',
+           'lf_op_te_terminate', 'lf_op_tm_terminate', 'lf_op3d_create', 'lf_op3d_set_cpml',
+           'lf_op3d_finalize', 'lf_op3d_step_H', 'lf_op3d_step_E', 'lf_op3d_destroy')
+
+
+class OpProfiles2D(ctypes.Structure):
+    """lf_op2d: device pointers of the 1-D CPML profiles and of the two psi arrays of an
+    operator-level 2D update (include/lensfdtd.h)."""
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ('bex', 'cex', 'kex', 'bhx', 'chx', 'khx', 'bey', 'cey', 'key', 'bhy', 'chy', 'khy',
+                 'psi0', 'psi1')]
 
 
 def library_path():
@@ -89,6 +100,18 @@ def load():
         'lf_batch_destroy': [P],
         'lf_batch_set_table': [P, P, I],
         'lf_batch_run_at': [P, P, I, P, P],
+        'lf_op_te_update_H': [ctypes.POINTER(OpProfiles2D), I, I, P, P, P, F, P],
+        'lf_op_te_update_E': [ctypes.POINTER(OpProfiles2D), I, I, P, P, P, P, P, P],
+        'lf_op_tm_update_H': [ctypes.POINTER(OpProfiles2D), I, I, P, P, P, F, P],
+        'lf_op_tm_update_E': [ctypes.POINTER(OpProfiles2D), I, I, P, P, P, P, P, P],
+        'lf_op_te_terminate': [I, I, P, ip, P],
+        'lf_op_tm_terminate': [I, I, P, P, ip, P],
+        'lf_op3d_create': [I, I, I, ctypes.POINTER(P)],
+        'lf_op3d_set_cpml': [P, I, fp, fp, fp, fp, fp, fp, I, I],
+        'lf_op3d_finalize': [P],
+        'lf_op3d_step_H': [P, P, P, P, P, P, P, F, P],
+        'lf_op3d_step_E': [P, P, P, P, P, P, P, P, P, P, P, P, P, P],
+        'lf_op3d_destroy': [P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -98,6 +121,12 @@ def load():
     lib.lf_last_error.argtypes = []
     _lib = lib
     return lib
+
+
+def op_call(name, *args):
+    """Call an operator-level entry point (lf_op_*, lf_op3d_*) and raise on error."""
+    lib = load()
+    _check(getattr(lib, name)(*args), name)
 
 
 def _check(rc, what):

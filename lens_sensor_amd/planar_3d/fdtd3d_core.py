@@ -635,6 +635,14 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
     Ecw = None
     if nacc_cw > 0:
         Ecw = (2.0 / nacc_cw) * eng.read_monitor(mon_cw)
+    if Ecw is not None and cw_volume and not calibrate_steps:
+        # steady-state volume set of the driven component: Re(E_cw e^{i w t_k}) at the
+        # snapshot count of phases over one period (reference fdtd3d_core.py:1268-1280)
+        nfr_cw = max(1, ksnap if ksnap else n_snapshots)
+        cw_vol = np.empty((nfr_cw,) + Ecw.shape, dtype=vol_dt)
+        for k in range(nfr_cw):
+            tk = (k / nfr_cw) * (1.0 / f0)
+            cw_vol[k] = np.real(Ecw * np.exp(1j * omega * tk)).astype(vol_dt)
     cw_ok = source_type != 'pulse' and book.has('src_up') and book.has('src_dn')
     p_emit = ((book.flux('src_dn') - book.flux('src_up')) * dx * dx) if cw_ok else None
     p_refl = (-book.flux('src_up') * dx * dx) if cw_ok else None
@@ -658,7 +666,10 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
                 n_mu=n_mu, n_co=n_co, lam_nm=lam_nm, source_type=source_type,
                 bead=(bead if bead else {}), det_in_um=det_in_um, det_out_um=det_out_um,
                 x_wg_start_um=x_wg_start_um, end_facet_um=(end_facet_um or 0.0),
-                x_src_um=x_src_um, polarization=pol, cells=cells)
+                x_src_um=x_src_um, polarization=pol, cells=cells,
+                # the snapshot volumes stay in host memory (no temporary stream files);
+                # no CAD geometry on the device path
+                _stream_paths=[], cad=None)
 
 
 # ----------------------------------------------------------------------------- stitch
@@ -863,4 +874,14 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
                 det_out_um=det_out_um, x_wg_start_um=x_wg_start_um, end_facet_um=0.0,
                 x_src_um=x_src_um, polarization=pol, view_primary='cw', cw_only=True,
                 Efull=(Efull if return_efull else None), transmission_intensity=Tr_I,
-                transmission_def=tr_def, n_windows=n_win, cells_per_window=Nx_win * Ny * Nz)
+                transmission_def=tr_def, n_windows=n_win, cells_per_window=Nx_win * Ny * Nz,
+                _stream_paths=[], cad=None)
+
+
+def save_results_3d(result, out_dir='results', prefix='fdtd3d', material_names=None):
+    """PKL summary, z mid-plane frames NPZ and volume NPZ of a 3D result (reference
+    fdtd3d_core.py:2218-2423); keys and file names as the reference writes them
+    (lens_sensor_amd.common.results)."""
+    from lens_sensor_amd.common import results
+    return results.save_results_3d(result, out_dir=out_dir, prefix=prefix,
+                                   material_names=material_names)

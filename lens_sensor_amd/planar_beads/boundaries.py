@@ -6,7 +6,9 @@ Profiles follow planar_beads/boundaries.py of the reference:
                                               kappa = 1 + (kmax-1) d^m, a = a_max (1-d),
                                               b = exp(-(sigma/kappa + a) dt/eps0),
                                               c = sigma (b-1)/(sigma kappa + kappa^2 a)
-  CPML_TE / CPML_TM    boundaries.py:164-274  npml per end, PEC on the PML ends
+  CPML_TE / CPML_TM    boundaries.py:164-274  npml per end, PEC on the PML ends;
+                                              update_H / update_E / terminate on device
+                                              arrays through fused2d (HIP kernels)
   make_boundary        boundaries.py:277-289
 
 Unlike the reference the psi auxiliaries are not host/CuPy arrays here: the step engine
@@ -93,12 +95,107 @@ class CPML2D:
             engine.set_cpml(ax, self.prof[ax][0], self.prof[ax][1], *self.pec[ax])
 
 
+class _CPMLOps(CPML2D):
+    """Operator-level interface of the reference's CPML objects (boundaries.py:164-274) on
+    device arrays.  The 1-D profiles and the full-domain psi arrays are created on the
+    device of the first arrays the object updates (torch tensors, float32, the
+    reference's attribute names and shapes); the updates run the HIP kernels of
+    fused2d.  Host (NumPy) arrays are refused: there is no CPU path."""
+
+    _PSI = ()
+
+    def __init__(self, pol, Nx, Ny, dx, dt, npml=10, npml_x=None, npml_y=None, **kw):
+        super().__init__(pol, Nx, Ny, dx, dt, npml=npml, npml_x=npml_x, npml_y=npml_y, **kw)
+        self._dev = None
+
+    def _device_state(self, like):
+        if self._dev is not None and self._dev == like.device:
+            return
+        import torch
+        dev = like.device
+        names = (('kex', 'bex', 'cex'), ('khx', 'bhx', 'chx'), ('key', 'bey', 'cey'), ('khy', 'bhy', 'chy'))
+        profs = (self.prof[0][0], self.prof[0][1], self.prof[1][0], self.prof[1][1])
+        for nm, pr in zip(names, profs):
+            for n, a in zip(nm, pr):
+                setattr(self, n, torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev))
+        for n, shape in self._PSI:
+            setattr(self, n, torch.zeros(shape, dtype=torch.float32, device=dev))
+        self._dev = dev
+
+    def _op2d(self, like, psi0, psi1):
+        from lens_sensor_amd.engine.hip import OpProfiles2D
+        return OpProfiles2D(*[getattr(self, n).data_ptr() for n in
+                              ('bex', 'cex', 'kex', 'bhx', 'chx', 'khx',
+                               'bey', 'cey', 'key', 'bhy', 'chy', 'khy')],
+                            psi0.data_ptr(), psi1.data_ptr())
+
+    def _need(self, what, *arrays):
+        from lens_sensor_amd.planar_beads import fused2d
+        if not fused2d.available(*arrays):
+            raise TypeError(f'{type(self).__name__}.{what} needs float32, contiguous device '
+                            'tensors on the HIP device (there is no host path)')
+        self._device_state(arrays[0])
+
+
+class CPML_TE(_CPMLOps):
+    """TE CPML (Ez, Hx, Hy), reference boundaries.py:164-221: psi over the full domain,
+    PEC on the E ends that carry a PML."""
+
+    def __init__(self, Nx, Ny, dx, dt, npml=10, npml_x=None, npml_y=None, **kw):
+        self._PSI = (('psi_Hxy', (Nx, Ny - 1)), ('psi_Hyx', (Nx - 1, Ny)),
+                     ('psi_Ezx', (Nx - 2, Ny - 2)), ('psi_Ezy', (Nx - 2, Ny - 2)))
+        super().__init__('s', Nx, Ny, dx, dt, npml=npml, npml_x=npml_x, npml_y=npml_y, **kw)
+
+    def update_H(self, Ez, Hx, Hy, Ch):
+        from lens_sensor_amd.planar_beads import fused2d
+        self._need('update_H', Ez, Hx, Hy)
+        return fused2d.te_update_H(self, Ez, Hx, Hy, Ch)
+
+    def update_E(self, Ez, Hx, Hy, Ce_E, Ce_H):
+        from lens_sensor_amd.planar_beads import fused2d
+        self._need('update_E', Ez, Hx, Hy, Ce_E, Ce_H)
+        return fused2d.te_update_E(self, Ez, Hx, Hy, Ce_E, Ce_H)
+
+    def terminate(self, Ez):
+        from lens_sensor_amd.planar_beads import fused2d
+        self._need('terminate', Ez)
+        return fused2d.te_terminate(self, Ez)
+
+
+class CPML_TM(_CPMLOps):
+    """TM CPML (Hz, Ex, Ey), reference boundaries.py:224-274: PEC sets the tangential E
+    (Ey on the x ends, Ex on the y ends) that carry a PML."""
+
+    def __init__(self, Nx, Ny, dx, dt, npml=10, npml_x=None, npml_y=None, **kw):
+        self._PSI = (('psi_Hzx', (Nx - 1, Ny - 1)), ('psi_Hzy', (Nx - 1, Ny - 1)),
+                     ('psi_Exy', (Nx - 1, Ny - 2)), ('psi_Eyx', (Nx - 2, Ny - 1)))
+        super().__init__('p', Nx, Ny, dx, dt, npml=npml, npml_x=npml_x, npml_y=npml_y, **kw)
+
+    def update_H(self, Ex, Ey, Hz, Ch_tm):
+        from lens_sensor_amd.planar_beads import fused2d
+        self._need('update_H', Ex, Ey, Hz)
+        return fused2d.tm_update_H(self, Ex, Ey, Hz, Ch_tm)
+
+    def update_E(self, Ex, Ey, Hz, Ca_ex, Ca_ey):
+        from lens_sensor_amd.planar_beads import fused2d
+        self._need('update_E', Ex, Ey, Hz, Ca_ex, Ca_ey)
+        return fused2d.tm_update_E(self, Ex, Ey, Hz, Ca_ex, Ca_ey)
+
+    def terminate(self, Ex, Ey):
+        from lens_sensor_amd.planar_beads import fused2d
+        self._need('terminate', Ex, Ey)
+        return fused2d.tm_terminate(self, Ex, Ey)
+
+
 def make_boundary(kind, pol, Nx, Ny, dx, dt, npml=10, npml_x=None, npml_y=None, lam_m=None):
+    """CPML object for kind 'cpml' (reference boundaries.py:277-289): CPML_TE for 's'/'te',
+    CPML_TM for 'p'/'tm'.  It attaches to a step engine (attach) and updates device
+    arrays (update_H / update_E / terminate)."""
     kind = (kind or 'cpml').lower()
     if kind == 'mur1':
         raise NotImplementedError("boundary 'mur1' is not part of the device path; use 'cpml'")
     if kind != 'cpml':
         raise ValueError(f"unknown boundary '{kind}' (cpml)")
     te = str(pol).lower() in ('s', 'te')
-    return CPML2D('s' if te else 'p', Nx, Ny, dx, dt, npml=npml, npml_x=npml_x, npml_y=npml_y,
-                  lam_m=lam_m)
+    return (CPML_TE if te else CPML_TM)(Nx, Ny, dx, dt, npml=npml, npml_x=npml_x, npml_y=npml_y,
+                                         lam_m=lam_m)

@@ -598,6 +598,21 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
         del ez_mm
     Ecw = ((2.0 / nacc_cw) * eng.read_monitor(mon_cw)) if nacc_cw > 0 else None
     Ex_cw = ((2.0 / nacc_cw) * eng.read_monitor(mon_cwex)) if (mon_cwex is not None and nacc_cw) else None
+    # steady-state frame set: Re(E_cw e^{i w t_k}) at n_snapshots phases of one period
+    # (reference fdtd2d_core.py:876-890)
+    cw_path, cw_frames = None, []
+    if save_frames and Ecw is not None:
+        nfr_cw = max(1, n_snapshots)
+        cw_path = os.path.join('results', f'_stream_beads_{_safe(label)}_full_cw.npy')
+        cw_mm = np.lib.format.open_memmap(cw_path, mode='w+', dtype=np.float32,
+                                          shape=(nfr_cw,) + Ecw.shape)
+        for k in range(nfr_cw):
+            tk = (k / nfr_cw) * (1.0 / f0)
+            cw_mm[k] = np.real(Ecw * np.exp(1j * omega * tk)).astype(np.float32)
+            cw_frames.append({'x_start': 0.0, 'x_end': L_um, 'y_start': y0_m * 1e6,
+                              'y_end': (y0_m + H_total) * 1e6, 'slide_i': 0, 'step': k, 'time': tk})
+        cw_mm.flush()
+        del cw_mm
     P_in = float(eng.read_monitor(mon_pin)) * dx * dx
     P_out = float(eng.read_monitor(mon_pout)) * dx * dx
     Iavg = (eng.read_monitor(mon_I) / max(1, nacc_I)) if nacc_I else None
@@ -618,7 +633,8 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
                        frames_memmap=mm_path, t_aq_um=t_aq_um, t_mu_um=t_mucin_um,
                        t_lip_um=t_lipid_um, bead_x_um=bx_um, x_wg_start_um=(input_gap_um or 0.0),
                        x_wg_end_um=L_um, polarization=pol, stack=stack, n_lip=n_lip_l)
-    res.update(Iavg=Iavg, Ecw=Ecw, cw_rel_change=cw_rel_change, transmission_intensity=Tr_I,
+    res.update(Iavg=Iavg, Ecw=Ecw, cw_frames=cw_frames, cw_frames_memmap=cw_path,
+               cw_rel_change=cw_rel_change, transmission_intensity=Tr_I,
                transmission_def=tr_def, steps_total=steps_total, p_in_si=p_si, p_in_unit='W/m',
                view_primary='transient', cells=Nx * Ny, grid=(Nx, Ny))
     if Ex_cw is not None:
@@ -865,6 +881,15 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
         res['Iavg'] = ExI_full
         res['Ex_cw_amp'] = ExA_full
     return res
+
+
+def save_results(result, out_dir='results'):
+    """PKL summary and frames NPZ of a 2D result (reference fdtd2d_core.py:2085-2240);
+    keys and file names as the reference writes them (lens_sensor_amd.common.results)."""
+    from lens_sensor_amd.common import results
+    return results.save_results(result, out_dir=out_dir,
+                                defaults=dict(t_aq=T_AQ, t_mu=T_MU, n_aq=N_AQ, n_mu=N_MUCIN,
+                                              n_lip=N_LIPID))
 
 
 import inspect as _inspect                                                   # noqa: E402

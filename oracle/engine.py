@@ -196,7 +196,7 @@ class OracleEngine:
         shp = [x1 - x0] + list(base[1:])
         if kplane is not None:
             shp = shp[:2]
-        if self.mode == 'f64' or acc == 'c128':
+        if self.wide_acc or acc == 'c128':            # fp64acc / f64: complex128 sums
             acc, dt = 'c128', np.complex128
         elif acc == 'split32':
             dt = np.complex64

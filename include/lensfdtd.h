@@ -93,6 +93,11 @@ int lf_monitor_bytes(lf_ctx *ctx, int id, long long *bytes);
 int lf_monitor_read(lf_ctx *ctx, int id, void *dst, long long bytes);
 int lf_monitor_write(lf_ctx *ctx, int id, const void *src, long long bytes);
 int lf_monitor_zero(lf_ctx *ctx, int id);
+/* Steady-state test on the device (fdtd2d_core.py:859-873, fdtd3d_core.py:1161-1175):
+ * out[0] = sum |A/da - B/db|^2 and out[1] = sum |A/da|^2 in FP64 over the DFT monitors
+ * id_a (A) and id_b (B) of the same shape and format; the reference reads both phasor
+ * windows to compute norm(a - b) / norm(a). */
+int lf_monitor_change(lf_ctx *ctx, int id_a, int id_b, double da, double db, double *out);
 
 /* Advance nsteps steps; steps[s] holds the scalars of step s. */
 int lf_run(lf_ctx *ctx, const lf_step *steps, int nsteps);
@@ -165,6 +170,13 @@ int lf_batch_run_at(lf_batch *b, const int *offs, int nsteps, const unsigned int
 /* last lf_batch_run with timing on: HIP-event time (ms) and blocked passes */
 int lf_batch_stats(lf_batch *b, double *ms, int *passes);
 int lf_batch_destroy(lf_batch *b);
+
+/* The same reduction over caller-owned DEVICE arrays a, b of n complex elements (acc:
+ * LF_ACC_C128 double2, LF_ACC_C64 / LF_ACC_C64P float2); out: 2 doubles in device
+ * memory, written on `stream`.  The partials live in a library buffer per (device,
+ * stream). */
+int lf_norm_diff(const void *a, const void *b, long long n, int acc, double da, double db,
+                 double *out, void *stream);
 
 /* ---- Operator level: single CPML updates on caller-owned DEVICE arrays, in the
  * reference's own C-order shapes (no padding), in place, enqueued on `stream` (a

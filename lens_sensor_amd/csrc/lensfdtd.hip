@@ -1109,7 +1109,6 @@ struct BoxArgs {
 };
 
 __global__ void __launch_bounds__(256) box_kernel(MonArgs m, BoxArgs b) {
-    __shared__ double red[256];
     double s = 0.0;
     for (int cc = 0; cc < b.ncomp; ++cc) {
         const int n0 = b.hi[cc][0] - b.lo[cc][0];
@@ -1133,13 +1132,9 @@ __global__ void __launch_bounds__(256) box_kernel(MonArgs m, BoxArgs b) {
             s += (double)v;
         }
     }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) b.partial[blockIdx.x] += red[0];
+    double unused = 0.0;
+    block_sum2_256(s, unused);                           // wavefront shuffles, fixed order
+    if (threadIdx.x == 0) b.partial[blockIdx.x] += s;
 }
 
 // Monitor launch shape: rows of 256 along the fast axis when the middle extent is 1
@@ -1194,6 +1189,10 @@ struct lf_ctx {
     // Valid only while nothing but passes has been queued since the last pass.
     bool tb_chain = false;
     unsigned tb_chain_sides = 0;         // side streams that recorded ev_join in that pass
+    // partials and the 2-double result of the device steady-state test (lf_monitor_change)
+    double2 *nd_part = nullptr;
+    size_t nd_cap = 0;
+    double *nd_out = nullptr;
     float *arr[LF_NARRAYS] = {};
     float *alt[LF_NARRAYS] = {};         // second buffer of ping-pong fields (fused path)
     int variant = 1;                     // 0: single-pass kernels where available, 1: H then E
@@ -1463,6 +1462,8 @@ int lf_destroy(lf_ctx *c) {
     if (c->tb3_dbox) (void)hipFree(c->tb3_dbox);
     if (c->tb3_ticket) (void)hipFree(c->tb3_ticket);
     if (c->tbm_stage) (void)hipFree(c->tbm_stage);
+    if (c->nd_part) (void)hipFree(c->nd_part);
+    if (c->nd_out) (void)hipFree(c->nd_out);
     for (int m = 0; m < c->nmon; ++m) {
         if (c->mon[m].buf) (void)hipFree(c->mon[m].buf);
         if (c->mon[m].type == 3)
@@ -2756,6 +2757,37 @@ int lf_monitor_read(lf_ctx *c, int id, void *dst, long long bytes) {
         return 0;
     }
     LF_HIP(hipMemcpy(dst, m.buf, m.bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+/* Steady-state test on the device: out[0] = sum |A/da - B/db|^2, out[1] = sum |A/da|^2
+ * over two DFT monitors of the same shape and format (two doubles to the host instead of
+ * two phasor fields). */
+int lf_monitor_change(lf_ctx *c, int id_a, int id_b, double da, double db, double *out) {
+    if (int e = check_ctx(c)) return e;
+    if (id_a < 0 || id_a >= c->nmon || id_b < 0 || id_b >= c->nmon || !out)
+        return fail(-1, "lf_monitor_change: bad arguments");
+    const Monitor &A = c->mon[id_a], &B = c->mon[id_b];
+    if (A.type != 2 || B.type != 2) return fail(-1, "lf_monitor_change: DFT monitors only");
+    if (A.bytes != B.bytes || A.dft.acc != B.dft.acc)
+        return fail(-1, "lf_monitor_change: the monitors differ in shape or format");
+    const size_t el = A.dft.acc == LF_ACC_C128 ? sizeof(double2) : sizeof(float2);
+    const long long n = (long long)(A.bytes / el);
+    const size_t need = (size_t)nd_partial_count(n) * sizeof(double2);
+    if (need > c->nd_cap) {
+        LF_HIP(hipStreamSynchronize(c->stream));
+        if (c->nd_part) LF_HIP(hipFree(c->nd_part));
+        c->nd_part = nullptr;
+        c->nd_cap = 0;
+        LF_HIP(hipMalloc(&c->nd_part, need));
+        c->nd_cap = need;
+    }
+    if (!c->nd_out) LF_HIP(hipMalloc(&c->nd_out, 2 * sizeof(double)));
+    c->tb_chain = false;
+    if (int e = norm_diff_launch(A.buf, B.buf, n, A.dft.acc, da, db, c->nd_part, c->nd_out, c->stream))
+        return e;
+    LF_HIP(hipMemcpyAsync(out, c->nd_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LF_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 

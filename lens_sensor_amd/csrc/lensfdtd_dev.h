@@ -11,6 +11,12 @@ namespace lfd {
 // whose entry points report their own errors
 void set_error(const char *msg);
 
+// steady-state reduction (reduce.hip): partials needed for n elements, and the launch of
+// sum |A/da - B/db|^2, sum |A/da|^2 into out[0..1] (device) on stream st
+int nd_partial_count(long long n);
+int norm_diff_launch(const void *a, const void *b, long long n, int acc, double da, double db,
+                     double2 *part, double *out, hipStream_t st);
+
 struct Strip {            // psi slab of one stretch axis: v < lo -> v, v >= hi -> v-hi+lo
     float *psi;
     int lo, hi, len;      // len = lo + (n - hi); psi == nullptr -> no slab
@@ -111,6 +117,27 @@ __device__ __forceinline__ float lane_next0(float v) {
 }
 __device__ __forceinline__ float lane_prev0(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+// Sum of two doubles over a workgroup of 256 work-items (4 wavefronts): butterfly
+// shuffles within each wavefront, then the 4 wavefront sums in a fixed order.  Every
+// work-item of the workgroup must call it; the result is valid in work-item 0.  The
+// order of the additions depends only on the work-item layout, so sums are reproducible.
+__device__ __forceinline__ void block_sum2_256(double &a, double &b) {
+    __shared__ double part[2][4];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { part[0][w] = a; part[1][w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = ((part[0][0] + part[0][1]) + part[0][2]) + part[0][3];
+        b = ((part[1][0] + part[1][1]) + part[1][2]) + part[1][3];
+    }
+    __syncthreads();
 }
 
 // psi slot of a slab compressed along `axis`, 32-bit element offset (grids < 2^31)

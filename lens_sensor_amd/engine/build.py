@@ -2,6 +2,7 @@
 
 Translation units: csrc/lensfdtd.hip (step kernels, monitors, batch engine, C ABI),
 csrc/ops.hip (operator-level CPML updates on caller-owned device arrays),
+csrc/reduce.hip (device steady-state reductions),
 csrc/tb2d_pass.hip and csrc/tb2d_mon.hip (plain and monitored 2D blocked passes, once per
 polarisation each) and
 csrc/yee_tb3.hip (the temporally blocked 3D pass, once per depth / loss variant); they
@@ -23,6 +24,7 @@ SRC_TB3 = os.path.join(CSRC, 'yee_tb3.hip')
 SRC_TB2M = os.path.join(CSRC, 'tb2d_mon.hip')
 SRC_TB2P = os.path.join(CSRC, 'tb2d_pass.hip')
 SRC_OPS = os.path.join(CSRC, 'ops.hip')
+SRC_RED = os.path.join(CSRC, 'reduce.hip')
 LIB = os.path.join(PKG, 'liblensfdtd.so')
 BUILD_DIR = os.path.join(PKG, '_build')
 ARCH = os.environ.get('LENSFDTD_ARCH', 'gfx950')
@@ -36,6 +38,8 @@ def _jobs():
              os.path.join(BUILD_DIR, 'lensfdtd.o'))]
     obj = os.path.join(BUILD_DIR, 'ops.o')
     jobs.append((['hipcc'] + FLAGS + ['-c', SRC_OPS, '-o', obj], obj))
+    obj = os.path.join(BUILD_DIR, 'reduce.o')
+    jobs.append((['hipcc'] + FLAGS + ['-c', SRC_RED, '-o', obj], obj))
     for tm in (0, 1):
         obj = os.path.join(BUILD_DIR, f'tb2d_mon_{tm}.o')
         jobs.append((['hipcc'] + FLAGS + [f'-DLF_TB2M_TM={tm}', '-c', SRC_TB2M, '-o', obj], obj))

@@ -34,7 +34,8 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_batch_run_at', 'lf_autotune', 'lf_set_temporal_mon', 'lf_set_tb_lead',
            'lf_op_te_update_H', 'lf_op_te_update_E', 'lf_op_tm_update_H', 'lf_op_tm_update_E',
            'lf_op_te_terminate', 'lf_op_tm_terminate', 'lf_op3d_create', 'lf_op3d_set_cpml',
-           'lf_op3d_finalize', 'lf_op3d_step_H', 'lf_op3d_step_E', 'lf_op3d_destroy')
+           'lf_op3d_finalize', 'lf_op3d_step_H', 'lf_op3d_step_E', 'lf_op3d_destroy',
+           'lf_monitor_change', 'lf_norm_diff')
 
 
 class OpProfiles2D(ctypes.Structure):
@@ -112,6 +113,9 @@ def load():
         'lf_op3d_step_H': [P, P, P, P, P, P, P, F, P],
         'lf_op3d_step_E': [P, P, P, P, P, P, P, P, P, P, P, P, P, P],
         'lf_op3d_destroy': [P],
+        'lf_monitor_change': [P, I, I, ctypes.c_double, ctypes.c_double,
+                              ctypes.POINTER(ctypes.c_double)],
+        'lf_norm_diff': [P, P, LL, I, ctypes.c_double, ctypes.c_double, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -351,6 +355,14 @@ class HipEngine:
 
     def zero_monitor(self, mid):
         _check(self._lib.lf_monitor_zero(self.h, mid), 'lf_monitor_zero')
+
+    def monitor_change(self, mid_a, mid_b, da=1.0, db=1.0):
+        """(sum |A/da - B/db|^2, sum |A/da|^2) of two DFT monitors, reduced on the device
+        in FP64 (two doubles cross to the host)."""
+        out = (ctypes.c_double * 2)()
+        _check(self._lib.lf_monitor_change(self.h, mid_a, mid_b, float(da), float(db), out),
+               'lf_monitor_change')
+        return out[0], out[1]
 
     # ---------------------------------------------------------------- stepping
     def run(self, tab):

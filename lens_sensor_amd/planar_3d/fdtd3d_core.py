@@ -592,9 +592,13 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
             snap_times.append(time.time() - t_start)
             ksnap += 1
         if conv and n == steps_total and n_pl_a > 0 and n_pl_b > 0:
-            a = eng.read_monitor(mon_pb) / n_pl_b
-            b = eng.read_monitor(mon_pa) / n_pl_a
-            cw_rel_change = float(np.linalg.norm(a - b) / max(float(np.linalg.norm(a)), 1e-30))
+            if eng.wide_acc:                     # FP64 windows: reduced on the device
+                num, den = eng.monitor_change(mon_pb, mon_pa, n_pl_b, n_pl_a)
+                cw_rel_change = float(np.sqrt(num) / max(float(np.sqrt(den)), 1e-30))
+            else:                                # the reference's arithmetic on the host
+                a = eng.read_monitor(mon_pb) / n_pl_b
+                b = eng.read_monitor(mon_pa) / n_pl_a
+                cw_rel_change = float(np.linalg.norm(a - b) / max(float(np.linalg.norm(a)), 1e-30))
             if cw_rel_change > CW_TOL and steps_total + n_acc_cw <= steps_max:
                 print(f'  [CW] not yet in steady state (change {100 * cw_rel_change:.2f}%) '
                       f'-> +{n_acc_cw} steps')

@@ -576,11 +576,13 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
                            'time': time.time() - t_start})
             k_fr += 1
         if conv and n == steps_total and nacc_prev > 0 and nacc_cw > 0:
-            a = eng.read_monitor(mon_cw) / nacc_cw
-            b = eng.read_monitor(mon_prev) / nacc_prev
-            if not wide:
-                a, b = a.astype(np.complex64), b.astype(np.complex64)
-            cw_rel_change = float(np.linalg.norm(a - b) / max(float(np.linalg.norm(a)), 1e-30))
+            if wide:                             # FP64 windows: reduced on the device
+                num, den = eng.monitor_change(mon_cw, mon_prev, nacc_cw, nacc_prev)
+                cw_rel_change = float(np.sqrt(num) / max(float(np.sqrt(den)), 1e-30))
+            else:                                # the reference's complex64 arithmetic
+                a = (eng.read_monitor(mon_cw) / nacc_cw).astype(np.complex64)
+                b = (eng.read_monitor(mon_prev) / nacc_prev).astype(np.complex64)
+                cw_rel_change = float(np.linalg.norm(a - b) / max(float(np.linalg.norm(a)), 1e-30))
             if cw_rel_change > CW_TOL and steps_total + n_acc_cw <= steps_max:
                 print(f'  [CW] not yet in steady state (change {100 * cw_rel_change:.2f}%) '
                       f'-> +{n_acc_cw} steps')

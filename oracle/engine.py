@@ -233,6 +233,15 @@ class OracleEngine:
                 p[2][...] = 0
                 p[3][...] = 0
 
+    def monitor_change(self, mid_a, mid_b, da=1.0, db=1.0):
+        """(sum |A/da - B/db|^2, sum |A/da|^2) of two DFT monitors in float64 (the engine
+        contract of the device's steady-state reduction)."""
+        a = self.mons[mid_a]['a'].astype(np.complex128) / da
+        b = self.mons[mid_b]['a'].astype(np.complex128) / db
+        d = a - b
+        return (float(np.sum(d.real * d.real + d.imag * d.imag)),
+                float(np.sum(a.real * a.real + a.imag * a.imag)))
+
     def read_monitor(self, mid):
         m = self.mons[mid]
         if m['t'] == 'box':

@@ -134,6 +134,7 @@ def _jobs():
         objs.append(obj)
     tu('lensfdtd.hip', [], 'lensfdtd')
     tu('ops.hip', [], 'ops')
+    tu('reduce.hip', [], 'reduce')
     for tm in (0, 1):
         tu('tb2d_mon.hip', [f'-DLF_TB2M_TM={tm}'], f'tb2d_mon_{tm}')
         tu('tb2d_pass.hip', [f'-DLF_TB2P_TM={tm}'], f'tb2d_pass_{tm}')

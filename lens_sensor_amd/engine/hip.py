@@ -57,6 +57,15 @@ def load():
         return _lib
     if not os.path.exists(_LIB_PATH):
         raise OSError(f'{_LIB_PATH} is not built (run __graft_entry__.build())')
+    # One HIP runtime per process.  torch ships its own libamdhip64 with the same SONAME
+    # (libamdhip64.so.7) as the ROCm one this library links, but requests it by another
+    # file name: loaded after this library it maps a second runtime, which finds no GPU.
+    # Loading torch first binds this library to torch's runtime, so the device pointers
+    # and stream handles of torch tensors (the operator-level API) are valid here too.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(_LIB_PATH)
     P, I, LL, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
     fp = ctypes.POINTER(ctypes.c_float)

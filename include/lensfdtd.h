@@ -178,6 +178,15 @@ int lf_batch_destroy(lf_batch *b);
 int lf_norm_diff(const void *a, const void *b, long long n, int acc, double da, double db,
                  double *out, void *stream);
 
+/* lf_norm_diff that allocates nothing (no hipMalloc / hipFree, no allocating library
+ * call): the partials live in a static 1 MiB device array, one per GPU, held until the
+ * process ends; n is at most 2^28 elements (larger n: error, use lf_norm_diff).  No
+ * counterpart in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Calls that overlap on
+ * one GPU share the array and race. */
+int lf_norm_diff_noalloc(const void *a, const void *b, long long n, int acc, double da, double db,
+                         double *out, void *stream);
+
 /* ---- Operator level: single CPML updates on caller-owned DEVICE arrays, in the
  * reference's own C-order shapes (no padding), in place, enqueued on `stream` (a
  * hipStream_t, NULL = the null stream).  The drop-in for a caller that keeps its own

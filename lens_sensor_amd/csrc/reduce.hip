@@ -61,6 +61,31 @@ std::map<std::pair<int, hipStream_t>, std::pair<void *, size_t>> g_nd_buf;
 
 int nd_fail(const char *m) { set_error(m); return -1; }
 
+// Partials of the allocation-free variant: a file-scope device array (1 MiB), one copy on
+// each GPU the library runs on, held until the process ends.  It bounds n to
+// kNdScratchPairs * kNdChunk = 2^28 elements.  nd_partials writes every pair that
+// nd_final reads, so the array needs no clearing between calls.
+constexpr int kNdScratchPairs = 65536;
+constexpr long long kNdScratchMaxN = (long long)kNdScratchPairs * kNdChunk;
+static __device__ __align__(256) double2 scratch[kNdScratchPairs];
+
+// Device address of `scratch` on each device, looked up once per device with that device
+// current (each GPU holds its own copy at its own address); host code passes these
+// addresses, never the array's name, which is a host-side stand-in.
+constexpr int kNdMaxDevices = 64;
+std::mutex g_sc_mu;
+double2 *g_sc_ptr[kNdMaxDevices] = {};
+
+double2 *scratch_on(int dev) {
+    std::lock_guard<std::mutex> lk(g_sc_mu);
+    if (g_sc_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return nullptr;
+        g_sc_ptr[dev] = (double2 *)p;
+    }
+    return g_sc_ptr[dev];
+}
+
 }  // namespace
 
 namespace lfd {
@@ -113,6 +138,25 @@ int lf_norm_diff(const void *a, const void *b, long long n, int acc, double da, 
         part = (double2 *)slot.first;
     }
     return norm_diff_launch(a, b, n, acc, da, db, part, out, st);
+}
+
+// lf_norm_diff without allocation: the partials go to the device's `scratch` array, so
+// the call makes no hipMalloc / hipFree.  Calls that overlap on one GPU (other streams,
+// other host threads) share that array and race.
+int lf_norm_diff_noalloc(const void *a, const void *b, long long n, int acc, double da, double db,
+                         double *out, void *stream) {
+    if (!a || !b || !out || n <= 0) return nd_fail("lf_norm_diff_noalloc: bad arguments");
+    if (acc != LF_ACC_C128 && acc != LF_ACC_C64 && acc != LF_ACC_C64P)
+        return nd_fail("lf_norm_diff_noalloc: acc must be LF_ACC_C128, LF_ACC_C64 or LF_ACC_C64P");
+    if (n > kNdScratchMaxN)
+        return nd_fail("lf_norm_diff_noalloc: n exceeds the static scratch (at most 2^28 = "
+                       "268435456 elements); use lf_norm_diff");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nd_fail("lf_norm_diff_noalloc: no device");
+    if (dev < 0 || dev >= kNdMaxDevices) return nd_fail("lf_norm_diff_noalloc: device index out of range");
+    double2 *part = scratch_on(dev);
+    if (part == nullptr) return nd_fail("lf_norm_diff_noalloc: hipGetSymbolAddress(scratch) failed");
+    return norm_diff_launch(a, b, n, acc, da, db, part, out, (hipStream_t)stream);
 }
 
 }  // extern "C"

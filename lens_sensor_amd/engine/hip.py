@@ -35,7 +35,7 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_op_te_update_H', 'lf_op_te_update_E', 'lf_op_tm_update_H', 'lf_op_tm_update_E',
            'lf_op_te_terminate', 'lf_op_tm_terminate', 'lf_op3d_create', 'lf_op3d_set_cpml',
            'lf_op3d_finalize', 'lf_op3d_step_H', 'lf_op3d_step_E', 'lf_op3d_destroy',
-           'lf_monitor_change', 'lf_norm_diff')
+           'lf_monitor_change', 'lf_norm_diff', 'lf_norm_diff_noalloc')
 
 
 class OpProfiles2D(ctypes.Structure):
@@ -125,6 +125,7 @@ def load():
         'lf_monitor_change': [P, I, I, ctypes.c_double, ctypes.c_double,
                               ctypes.POINTER(ctypes.c_double)],
         'lf_norm_diff': [P, P, LL, I, ctypes.c_double, ctypes.c_double, P, P],
+        'lf_norm_diff_noalloc': [P, P, LL, I, ctypes.c_double, ctypes.c_double, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

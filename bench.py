@@ -43,7 +43,13 @@ BYTES_YEE3D = 6 * 8 + 3 * 4           # 6 fields r/w + ce_x, ce_y, ce_z = 60 B
 # WRITE_SIZE (KiB per dispatch), scaled by the same counters on a plain 1 GiB copy
 # (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth,
 # x chunk): a pass of another blocking setting has no entry and reports traffic None.
-TRAFFIC_BYTES = {}
+TRAFFIC_BYTES = {
+    # mean over the ten windows: the source window's pass (654.66 MB, four launches) and
+    # nine driven windows' passes (709.42 MB, three launches)
+    ('tm2d_stitch', 8, 128): 703_939_744,
+    # yee_tb_kernel<3, false>: FETCH 1,479,486.5 KiB x 2048 + WRITE 1,043,593.9 KiB x 1024
+    ('yee3d', 3, 96): 3_029_988_357 + 1_068_640_115,
+}
 
 
 def _timed_factory(records, every=1, fixed=None):
@@ -57,9 +63,10 @@ def _timed_factory(records, every=1, fixed=None):
         def __init__(self, kind, dims, mode):
             super().__init__(kind, dims, mode)
             self.set_timing(True)
-            if fixed is not None and kind != 'yee3d':
+            if fixed is not None:
                 self.set_temporal(fixed[0], fixed[1])
-                self.set_tb_lead(fixed[2])
+                if kind != 'yee3d':
+                    self.set_tb_lead(fixed[2])
             self._upd = 0.0
             self._tot = 0.0
             self._n = 0
@@ -198,12 +205,22 @@ def bench_yee3d(args):
     bead = dict(x_um=float(rng.uniform(6.0, 12.0)), d_um=1.0, y_um=-0.6, n=1.59,
                 z_um=float(rng.uniform(-1.0, 1.0)))
     records = []
+    old_at = backend.AUTOTUNE
+    fixed = None
+    if args.temporal:
+        fixed = tuple(int(v) for v in args.temporal.split(','))
+        backend.AUTOTUNE = False
     t0 = time.time()
-    with backend.use_engine(_timed_factory(records)), contextlib.redirect_stdout(io.StringIO()):
-        r = f3d.run_3d('bench3d', 1.49, t_wg_um=t_wg, lam_nm=850.0, lx_um=nx * dx_um, air_um=air,
-                       tear_um=tear, lz_um=nz * dx_um, dx_nm=30.0, vcsel_waist_um=1.5,
-                       vcsel_waist_z_um=1.5, n_snapshots=1, bead=bead, stack=stack, n_co=1.376,
-                       npml=12, engine_mode='fp64acc', steps_override=steps, cw_volume=False)
+    try:
+        with backend.use_engine(_timed_factory(records, fixed=fixed)), \
+                contextlib.redirect_stdout(io.StringIO()):
+            r = f3d.run_3d('bench3d', 1.49, t_wg_um=t_wg, lam_nm=850.0, lx_um=nx * dx_um, air_um=air,
+                           tear_um=tear, lz_um=nz * dx_um, dx_nm=30.0, vcsel_waist_um=1.5,
+                           vcsel_waist_z_um=1.5, n_snapshots=1, bead=bead, stack=stack,
+                           n_co=1.376, npml=12, engine_mode='fp64acc', steps_override=steps,
+                           cw_volume=False)
+    finally:
+        backend.AUTOTUNE = old_at
     wall = time.time() - t0
     e = records[0]
     cells = nx * ny * nz
@@ -344,13 +361,6 @@ def main():
     if args.gpus != 1:
         print('bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not '
               'part of it', file=sys.stderr)
-        sys.exit(2)
-    if args.workload in ('sweep', 'serve'):
-        # the 64-instance batched sweep faulted the device (illegal address, reported at a
-        # batch-pass staging sync) with the current blocked-pass kernels; its cause is not
-        # found yet, so the batched workloads are not run until it is
-        print(f'bench.py: --workload {args.workload} is disabled: its batched passes faulted the '
-              'device at full size and the cause is still open', file=sys.stderr)
         sys.exit(2)
     from lens_sensor_amd.engine import hip
     dev = hip.device_name()

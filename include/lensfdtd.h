@@ -125,8 +125,9 @@ int lf_set_temporal(lf_ctx *ctx, int depth, int lx);
  * flux.py:253-291).  Every accumulator receives the same additions in the same order, so
  * results are bit-identical to depth 1.  2D TE and TM; 3D accepts only 1. */
 int lf_set_temporal_mon(lf_ctx *ctx, int depth);
-/* Blocked passes of the last lf_run with timing on: summed HIP-event time (ms), number of
- * launches, current depth. */
+/* Blocked passes of the last lf_run with timing on: HIP-event time (ms) of the runs of
+ * consecutive plain passes (events at each run's ends, none between its passes), number
+ * of those passes, current depth. */
 int lf_tb_stats(lf_ctx *ctx, double *ms, int *launches, int *depth);
 /* Launch order of the classes of a 2D blocked pass (no reference counterpart: an engine
  * tuning knob): class `cls` (0 slab-free, 1 y-slab rows, 2 x-slab planes, 3 excitation

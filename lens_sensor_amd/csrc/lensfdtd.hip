@@ -2930,8 +2930,19 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         }
         LF_HIP(hipEventRecord(c->ev[2 * nsteps], c->stream));
     }
+    // Timed groups: one per one-step update or monitored pass, and one per run of
+    // consecutive plain blocked passes (events at the run's ends only, so that a pass's
+    // time is the run's start-to-start period, without events between its passes)
     int ngroup = 0;
     std::vector<int> tb_groups;
+    int tb_passes = 0;
+    bool tb_open = false;
+    auto close_run = [&]() {
+        if (!tb_open) return hipSuccess;
+        tb_open = false;
+        tb_groups.push_back(ngroup++);
+        return timing ? hipEventRecord(c->ev[2 * (ngroup - 1) + 1], c->stream) : hipSuccess;
+    };
     for (int s = 0; s < nsteps; ++s) {
         const lf_step &t = steps[s];
         StepArgs st = step_args(t);
@@ -2952,26 +2963,31 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
                 quiet = quiet && (last || (steps[s + u].mask & ~fused) == 0);
             }
             if (quiet) {
-                if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
+                if (!tb_open) {
+                    if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
+                    tb_open = true;
+                }
                 if (c->kind == LF_YEE3D) {
                     if (int e = tb_chunk_3d(c, steps + s, T, fp64)) return e;
                 } else {
                     if (int e = tb_chunk_2d(c, steps + s, T, fp64)) return e;
                 }
-                if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
-                tb_groups.push_back(ngroup);
-                ++ngroup;
+                ++tb_passes;
                 if (c->kind == LF_TM2D) tm = tm_args(c);
                 else if (c->kind == LF_TE2D) te = te_args(c);
                 else yee = yee_args(c);
                 ma = mon_args(c);
                 s += T - 1;
                 const lf_step &tl = steps[s];
-                if ((tl.mask & ~fused) != 0) c->tb_chain = false;
+                if ((tl.mask & ~fused) != 0) {
+                    c->tb_chain = false;
+                    LF_HIP(close_run());
+                }
                 if (int e = launch_monitors(c, ma, tl, step_args(tl), tl.mask & ~fused)) return e;
                 continue;
             }
         }
+        LF_HIP(close_run());
         // 2D steps with monitors: a monitored pass when all of them can be summed inside it
         const int Tm = allow_tb ? c->tb_mon : 1;
         TBMonPlan mp;
@@ -3009,6 +3025,7 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         ++ngroup;
         if (int e = launch_monitors(c, ma, t, st, t.mask)) return e;
     }
+    LF_HIP(close_run());
     LF_HIP(hipGetLastError());
     if (timing) {
         LF_HIP(hipEventRecord(c->ev[2 * nsteps + 1], c->stream));
@@ -3025,7 +3042,7 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
             }
         }
         c->last_tb_ms = tbms;
-        c->last_tb_launches = (int)tb_groups.size();
+        c->last_tb_launches = tb_passes;
         float tot = 0.f;
         LF_HIP(hipEventElapsedTime(&tot, c->ev[2 * nsteps], c->ev[2 * nsteps + 1]));
         c->last_update_ms = upd;
@@ -3176,9 +3193,6 @@ extern "C" int lf_autotune(lf_ctx *c, const lf_step *row, int reps, int *depth, 
 }
 
 // ============================================================================ batch engine (C ABI)
-constexpr long long kBatchMaxInstCells = 1ll << 20;    // cells per instance
-constexpr long long kBatchMaxCells = 8ll << 20;        // cells per batch
-
 struct lf_batch {
     std::vector<lf_ctx *> ctx;
     hipStream_t stream = nullptr, side[2] = {};
@@ -3436,18 +3450,6 @@ int lf_batch_run_at(lf_batch *b, const int *offs, int nsteps, const unsigned *ma
         const int na = nact ? std::min(nact[i], nsteps) : nsteps;
         if (o < 0 || o + na > ntab) return fail(-1, "lf_batch_run_at: steps outside the table");
     }
-    // Size guard: a batched sweep of 64 instances of up to 16384 x 2048 cells ended with an
-    // illegal memory access whose cause is not established; the batch engine runs only
-    // within the instance and batch sizes its device tests cover.
-    long long total = 0;
-    for (lf_ctx *c : b->ctx) {
-        const long long cells = (long long)c->g.nx * c->g.nz;
-        total += cells;
-        if (cells > kBatchMaxInstCells)
-            return fail(-1, "lf_batch_run: instance larger than the tested batch size (1M cells)");
-    }
-    if (total > kBatchMaxCells)
-        return fail(-1, "lf_batch_run: batch larger than the tested batch size (8M cells)");
     for (lf_ctx *c : b->ctx) {
         if (c->ch == 0.0f) return fail(-1, "lf_batch_run: Ch not set");
         if (c->kind == LF_TE2D && (!c->arr[LF_CEE] || !c->arr[LF_CEH]))

@@ -153,7 +153,11 @@ def run_sweep(params, batched=True, mode='fp64acc', steps_cap=None, depth=6, lx=
     instance after another with the one-step-per-launch path.  Returns (results, seconds of
     stepping, seconds of setup)."""
     t0 = time.time()
-    solves = [Solve(p, mode=mode, steps_cap=steps_cap, engine_factory=engine_factory) for p in params]
+    solves = []
+    for k, p in enumerate(params):
+        solves.append(Solve(p, mode=mode, steps_cap=steps_cap, engine_factory=engine_factory))
+        if log:
+            log(f'[sweep] set up instance {k + 1}/{len(params)} ({p["nx"]} x {p["ny"]})')
     t_setup = time.time() - t0
     s0 = solves[0]
     for s in solves:

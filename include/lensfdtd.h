@@ -83,6 +83,20 @@ int lf_set_source(lf_ctx *ctx, int comp, int ix, int plane_count, const float *a
  * (interleaved re,im) of shape (n, plane) stored back to back. */
 int lf_set_drive(lf_ctx *ctx, int ncells, const float *wtap, int ncomp, const int *comps,
                  const float *drives_c64, int math);
+/* Secondary absorbers, selected instead of the CPML on some faces (the run then advances
+ * one step per launch and applies them after the E update and the source):
+ * lf_set_sponge: graded sponge of n cells, factors g[0..n-1] (g[0] outermost) on the faces
+ *   of bitmask faces (bit 2*axis + hi), all six fields, 3D only; replaces _sponge_profile /
+ *   _apply_sponge (fdtd3d_core.py:397-414) as called at fdtd3d_core.py:1133-1135, 1635-1637.
+ * lf_add_mur: first-order Mur on face (axis, hi) of E component comp, coefficient m
+ *   (n == 1 scalar, else one per cell of the boundary plane), faces applied in the order
+ *   added; flags bit 0: F0_old = current value (the 2D inline form), bit 1: zero the
+ *   plane's cells on PEC faces afterwards.  Replaces Mur1_3D.capture/apply
+ *   (boundaries3d.py:99-140, called at fdtd3d_core.py:1112-1132) and the 2D inline Mur of
+ *   fdtd2d_core.py:805-810, 831-833 (stitch :1422-1433, :1458-1469).  Call after
+ *   lf_set_cpml of every axis. */
+int lf_set_sponge(lf_ctx *ctx, int n, const float *g, unsigned faces);
+int lf_add_mur(lf_ctx *ctx, int comp, int axis, int hi, const float *m, long long n, int flags);
 
 /* Monitors; each returns its id (0..31) in *id. */
 int lf_add_box(lf_ctx *ctx, const int *ranges18, int *id);  /* 3 comps x 3 axes x (lo,hi) */

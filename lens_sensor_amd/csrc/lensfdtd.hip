@@ -1163,6 +1163,105 @@ __global__ void gather_kernel(float *dst, const float *src, int nx, int ny, int 
     dst[t] = src[((long long)i * NY + j) * P + k];
 }
 
+// ---------------------------------------------------------------------------- secondary absorbers
+// Graded sponge (3D, fdtd3d_core.py:397-414): the cells within n of an absorbing face are
+// multiplied by g[d] per step, d the distance to that face (g[0] the outermost cell), for
+// all six fields after the E update and the source.  The faces apply one after another,
+// x low, x high, y low, y high, z low, z high, so a cell in several slabs takes the
+// products in that order.  One 64-wide block per (i, j) row of the field: rows inside an
+// x or y slab run their whole k range, the others only the z slabs.
+struct SpongeArgs {
+    float *f;
+    int ex, ey, ez;          // extents of the field
+    Grid g;
+    const float *damp;       // n factors, index 0 = outermost cell
+    int n;
+    unsigned faces;          // bit 2*axis + hi
+};
+
+__device__ __forceinline__ float sponge_cell(const SpongeArgs &a, float v, int i, int j, int k) {
+    const int n = a.n;
+    if ((a.faces & 1u) && i < n) v = v * a.damp[i];
+    if ((a.faces & 2u) && i >= a.ex - n) v = v * a.damp[a.ex - 1 - i];
+    if ((a.faces & 4u) && j < n) v = v * a.damp[j];
+    if ((a.faces & 8u) && j >= a.ey - n) v = v * a.damp[a.ey - 1 - j];
+    if ((a.faces & 16u) && k < n) v = v * a.damp[k];
+    if ((a.faces & 32u) && k >= a.ez - n) v = v * a.damp[a.ez - 1 - k];
+    return v;
+}
+
+__global__ void __launch_bounds__(64) sponge_kernel(SpongeArgs a) {
+    const int j = blockIdx.x, i = blockIdx.y;
+    const int n = a.n;
+    const bool row = ((a.faces & 1u) && i < n) || ((a.faces & 2u) && i >= a.ex - n) ||
+                     ((a.faces & 4u) && j < n) || ((a.faces & 8u) && j >= a.ey - n);
+    float *f = a.f + a.g.at(i, j, 0);
+    if (row) {
+        for (int k = threadIdx.x; k < a.ez; k += 64) f[k] = sponge_cell(a, f[k], i, j, k);
+        return;
+    }
+    // z slabs only; a cell of both z slabs (n > ez/2) is visited once
+    const int lo_end = (a.faces & 16u) ? min(n, a.ez) : 0;
+    const int hi_beg = (a.faces & 32u) ? max(a.ez - n, lo_end) : a.ez;
+    for (int k = threadIdx.x; k < lo_end; k += 64) f[k] = sponge_cell(a, f[k], i, j, k);
+    for (int k = hi_beg + threadIdx.x; k < a.ez; k += 64) f[k] = sponge_cell(a, f[k], i, j, k);
+}
+
+// First-order Mur on one face of one tangential E component (boundaries3d.py:99-140;
+// 2D inline fdtd2d_core.py:805-810, 831-833): the boundary plane F0 takes
+// F1_prev + m (F1_new - F0_old), F1 the plane one cell inside, *_prev / *_old its values
+// before the step.  The 3D form takes F0_old from before the step; the 2D inline form
+// takes the plane's current value (after the TM loss factor D has scaled it).  Storage
+// coordinates: the face axis `ax` (0 x, 1 y, 2 z of the padded grid), the plane spans
+// the other two axes (u slow, v fast).
+struct MurArgs {
+    float *f;
+    Grid g;
+    int ax, i0, i1;          // face axis, boundary and inner indices along it
+    int nu, nv;              // plane extents
+    const float *m;          // per-cell coefficient (nullptr: the scalar ms)
+    float ms;
+    float *prev;             // [F1_prev | F0_prev] planes
+    int f0cur;               // 1: F0_old = current value (2D inline form)
+    // PEC zeroing of cells on PEC faces where the component is tangential (the PEC the
+    // reference applies after this face's update): per storage axis, lo/hi flags
+    int pz[3][2];
+};
+
+__device__ __forceinline__ long long mur_at(const MurArgs &a, int pos, int u, int v) {
+    if (a.ax == 0) return a.g.at(pos, u, v);
+    if (a.ax == 1) return a.g.at(u, pos, v);
+    return a.g.at(u, v, pos);
+}
+
+__global__ void __launch_bounds__(256) mur_capture_kernel(MurArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.nu * a.nv;
+    if (t >= n) return;
+    const int u = t / a.nv, v = t - u * a.nv;
+    a.prev[t] = a.f[mur_at(a, a.i1, u, v)];
+    if (!a.f0cur) a.prev[n + t] = a.f[mur_at(a, a.i0, u, v)];
+}
+
+__global__ void __launch_bounds__(256) mur_apply_kernel(MurArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.nu * a.nv;
+    if (t >= n) return;
+    const int u = t / a.nv, v = t - u * a.nv;
+    const long long c0 = mur_at(a, a.i0, u, v);
+    const float f1 = a.f[mur_at(a, a.i1, u, v)];
+    const float f0 = a.f0cur ? a.f[c0] : a.prev[n + t];
+    const float m = a.m ? a.m[t] : a.ms;
+    float r = a.prev[t] + m * (f1 - f0);
+    // the cell's storage coordinates along the two plane axes
+    const int au = a.ax == 0 ? 1 : 0, av = a.ax == 2 ? 1 : 2;
+    const int nu1 = a.nu - 1, nv1 = a.nv - 1;
+    if ((a.pz[au][0] && u == 0) || (a.pz[au][1] && u == nu1) || (a.pz[av][0] && v == 0) ||
+        (a.pz[av][1] && v == nv1))
+        r = 0.0f;
+    a.f[c0] = r;
+}
+
 }  // namespace
 
 // ============================================================================ host context
@@ -1208,6 +1307,15 @@ struct lf_ctx {
     unsigned *tb3_ticket = nullptr;      // tile ticket counter (device) and the next pass's base
     unsigned tb3_base = 0;
     int tb3_key[4] = {0, 0, 0, 0};       // source plane, drive overlap, rows, monitors of the plan
+    // the plan of each depth (1..3), kept so that passes of different depths can follow one
+    // another without re-planning (each keeps its own device box table); the fields above
+    // are the plan of the depth used last (tb3_dbox aliases that depth's table)
+    struct TB3Saved {
+        std::vector<TB3Box> boxes;
+        TB3Box *dbox = nullptr;
+        int lx = 0, ntile = 0, nwg = 0, key[4] = {0, 0, 0, 0};
+        bool valid = false;
+    } tb3_saved[4];
     int tb3_rows0 = 2;                   // rows per wavefront of the slab-free class
     int ty = 4;                          // owned y rows per wavefront of the 3D single pass
     int tb = 1;                          // temporal-block depth (steps per pass); 1 = off
@@ -1232,6 +1340,13 @@ struct lf_ctx {
     int nk_src = 0;
     Monitor mon[kMaxMon];
     int nmon = 0;
+    // secondary absorbers (selected instead of the CPML on some faces): while either is
+    // set, lf_run advances one step per launch and applies them after every step
+    int sp_n = 0;                        // sponge cells (3D), 0 = none
+    unsigned sp_faces = 0;               // bit 2*axis + hi
+    const float *sp_g = nullptr;         // damping factors, index 0 = outermost cell
+    std::vector<MurArgs> mur;            // Mur faces, in the order they apply
+    std::vector<int> mur_comp;           // their field components (LF_E*)
     bool finalized = false;
     // timing
     bool timing = false;
@@ -1459,7 +1574,8 @@ int lf_destroy(lf_ctx *c) {
             if (c->prof_h[a][q]) (void)hipFree(c->prof_h[a][q]);
         }
     for (auto p : c->ex_bufs) (void)hipFree(p);
-    if (c->tb3_dbox) (void)hipFree(c->tb3_dbox);
+    for (auto &sv : c->tb3_saved)
+        if (sv.dbox) (void)hipFree(sv.dbox);
     if (c->tb3_ticket) (void)hipFree(c->tb3_ticket);
     if (c->tbm_stage) (void)hipFree(c->tbm_stage);
     if (c->nd_part) (void)hipFree(c->nd_part);
@@ -1665,6 +1781,80 @@ int lf_set_drive(lf_ctx *c, int ncells, const float *wtap, int ncomp, const int 
         c->ex.drv_stride[comps[q]] = n;
     }
     LF_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Graded sponge of n cells on the faces in `faces` (bit 2*axis + hi), damping factors
+// g[0..n-1] with g[0] at the outermost cell (fdtd3d_core.py:397-414).  3D only.
+int lf_set_sponge(lf_ctx *c, int n, const float *g, unsigned faces) {
+    if (int r = check_ctx(c)) return r;
+    if (c->kind != LF_YEE3D) return fail(-1, "lf_set_sponge: the sponge is a 3D absorber");
+    if (n <= 0 || g == nullptr) return fail(-1, "lf_set_sponge: n must be > 0 with n factors");
+    if (faces == 0u || faces > 63u) return fail(-1, "lf_set_sponge: faces must be a nonempty subset of 6");
+    if (n > c->nx - 1 || n > c->ny - 1 || n > c->nz - 1)
+        return fail(-1, "lf_set_sponge: sponge thicker than the grid");
+    void *d;
+    if (int r = upload_vec(c, g, (size_t)n * sizeof(float), &d)) return r;
+    LF_HIP(hipStreamSynchronize(c->stream));
+    c->sp_g = (const float *)d;
+    c->sp_n = n;
+    c->sp_faces = faces;
+    return 0;
+}
+
+// First-order Mur on face (axis, hi) of the E component `comp`; faces apply in the order
+// they are added.  m: n == 1 a scalar coefficient, else one per cell of the component's
+// boundary plane (C order of the plane).  flags bit 0: F0_old is the plane's current
+// value (the 2D inline form, fdtd2d_core.py:805-810) instead of its value before the step
+// (boundaries3d.py:125-140); bit 1: cells of the plane on PEC faces where the component is
+// tangential are zeroed after the update (the reference applies that PEC afterwards).
+// Call after lf_set_cpml of every axis (the PEC faces are those of lf_set_cpml).
+int lf_add_mur(lf_ctx *c, int comp, int axis, int hi, const float *m, long long n, int flags) {
+    if (int r = check_ctx(c)) return r;
+    const int ndim = c->kind == LF_YEE3D ? 3 : 2;
+    for (int a = 0; a < ndim; ++a)
+        if (!c->have_prof[a]) return fail(-1, "lf_add_mur: set the profiles of every axis first");
+    if (axis < 0 || axis >= ndim) return fail(-1, "lf_add_mur: bad axis");
+    const bool ok = c->kind == LF_YEE3D ? (comp >= LF_EX && comp <= LF_EZ)
+                  : c->kind == LF_TE2D ? comp == LF_EZ : (comp == LF_EX || comp == LF_EY);
+    if (!ok) return fail(-1, "lf_add_mur: not an E component of this kind");
+    if (m == nullptr || n < 1) return fail(-1, "lf_add_mur: no coefficient");
+    if (c->mur.size() >= 12) return fail(-1, "lf_add_mur: at most 12 faces");
+    int ext[3];
+    comp_extent(c, comp, ext[0], ext[1], ext[2]);
+    const int sa = ndim == 3 ? axis : (axis == 0 ? 0 : 2);     // storage axis
+    if (ext[sa] < 2) return fail(-1, "lf_add_mur: component has fewer than 2 planes");
+    MurArgs a{};
+    a.g = c->g;
+    a.ax = sa;
+    a.i0 = hi ? ext[sa] - 1 : 0;
+    a.i1 = hi ? ext[sa] - 2 : 1;
+    a.nu = sa == 0 ? ext[1] : ext[0];
+    a.nv = sa == 2 ? ext[1] : ext[2];
+    const long long np = (long long)a.nu * a.nv;
+    if (n != 1 && n != np) return fail(-1, "lf_add_mur: coefficient count is neither 1 nor the plane size");
+    a.f0cur = flags & 1;
+    // logical direction of the component (2D TE Ez is normal to the plane of the grid)
+    const int dir = ndim == 2 && comp == LF_EZ ? 2 : comp - LF_EX;
+    for (int la = 0; la < ndim; ++la) {
+        const int s2 = ndim == 3 ? la : (la == 0 ? 0 : 2);
+        for (int lh = 0; lh < 2; ++lh)
+            a.pz[s2][lh] = (flags & 2) && la != dir && c->pec[la][lh] ? 1 : 0;
+    }
+    void *d = nullptr;
+    if (n == 1) {
+        a.ms = m[0];
+    } else {
+        if (int r = upload_vec(c, m, (size_t)np * sizeof(float), &d)) return r;
+        a.m = (const float *)d;
+    }
+    LF_HIP(hipMalloc(&d, (size_t)2 * np * sizeof(float)));
+    c->ex_bufs.push_back(d);
+    LF_HIP(hipMemsetAsync(d, 0, (size_t)2 * np * sizeof(float), c->stream));
+    a.prev = (float *)d;
+    LF_HIP(hipStreamSynchronize(c->stream));
+    c->mur.push_back(a);
+    c->mur_comp.push_back(comp);
     return 0;
 }
 
@@ -2363,9 +2553,22 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
         for (int q = 0; q < 3; ++q) boxx.push_back({b.lo[q][0], b.hi[q][0]});
     }
     const int key_src = c->ex.amp ? c->ex.src_ix : -1, key_drv = c->ex.wtap ? c->ex.drv_n : -1;
+    if (T < 1 || T > 3) return fail(-1, "3D blocked pass: depth must be 1, 2 or 3");
     if (c->tb3_dbox && c->tb3_T == T && c->tb3_lx == lx && c->tb3_key[0] == key_src &&
         c->tb3_key[1] == key_drv && c->tb3_key[2] == c->tb3_rows0 && c->tb3_key[3] == c->nmon)
         return 0;
+    lf_ctx::TB3Saved &sv = c->tb3_saved[T];
+    if (sv.valid && sv.lx == lx && sv.key[0] == key_src && sv.key[1] == key_drv &&
+        sv.key[2] == c->tb3_rows0 && sv.key[3] == c->nmon) {
+        c->tb3_boxes = sv.boxes;
+        c->tb3_dbox = sv.dbox;
+        c->tb3_ntile = sv.ntile;
+        c->tb3_nwg = sv.nwg;
+        c->tb3_T = T;
+        c->tb3_lx = lx;
+        for (int q = 0; q < 4; ++q) c->tb3_key[q] = sv.key[q];
+        return 0;
+    }
     const int N3[3] = {c->g.nz, c->g.ny, c->g.nx};                   // z, y, x
     const int axs[3][4] = {{0, 3, 1, 2}, {1, 4, 0, 5}, {2, 5, 3, 4}};   // sH, sH, sE, sE
     struct Iv { int a, b; bool edge, exc, box; };
@@ -2485,7 +2688,8 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
         first += ntx * b.nty * b.ntz;
     }
     if ((int)boxes.size() > kTB3MaxBox) return fail(-1, "3D blocked pass: too many boxes");
-    if (!c->tb3_dbox) LF_HIP(hipMalloc(&c->tb3_dbox, kTB3MaxBox * sizeof(TB3Box)));
+    if (!sv.dbox) LF_HIP(hipMalloc(&sv.dbox, kTB3MaxBox * sizeof(TB3Box)));
+    c->tb3_dbox = sv.dbox;
     LF_HIP(hipMemcpyAsync(c->tb3_dbox, boxes.data(), boxes.size() * sizeof(TB3Box),
                           hipMemcpyHostToDevice, c->stream));
     LF_HIP(hipStreamSynchronize(c->stream));
@@ -2507,6 +2711,12 @@ int tb3_plan(lf_ctx *c, int T, int lx) {
     c->tb3_lx = lx;
     c->tb3_key[0] = key_src; c->tb3_key[1] = key_drv; c->tb3_key[2] = c->tb3_rows0;
     c->tb3_key[3] = c->nmon;
+    sv.boxes = boxes;
+    sv.lx = lx;
+    sv.ntile = first;
+    sv.nwg = c->tb3_nwg;
+    for (int q = 0; q < 4; ++q) sv.key[q] = c->tb3_key[q];
+    sv.valid = true;
     return 0;
 }
 
@@ -2534,7 +2744,7 @@ int ensure_tb3_buffers(lf_ctx *c) {
 // one temporally blocked 3D pass over T steps: reads arr[] and the psi slabs, writes the
 // other buffers, then swaps
 int tb_chunk_3d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
-    if (T < 2 || T > 3) return fail(-1, "3D temporal block depth must be 2 or 3");
+    if (T < 1 || T > 3) return fail(-1, "3D temporal block depth must be 1, 2 or 3");
     if ((long long)c->g.nx * c->g.ny * c->g.P >= (1LL << 31))
         return fail(-1, "3D blocked pass: grid exceeds 2^31 elements");
     if (int r = ensure_tb3_buffers(c)) return r;
@@ -2590,7 +2800,8 @@ int tb_chunk_3d(lf_ctx *c, const lf_step *steps, int T, int fp64) {
 namespace lfd {
 void launch_yee_tb3(int T, bool loss, const TB3Args &f, const Excite &x, int fp64, unsigned nb,
                     hipStream_t st) {
-    if (T == 2) (loss ? launch_yee_tb3_2_1 : launch_yee_tb3_2_0)(f, x, fp64, nb, st);
+    if (T == 1) (loss ? launch_yee_tb3_1_1 : launch_yee_tb3_1_0)(f, x, fp64, nb, st);
+    else if (T == 2) (loss ? launch_yee_tb3_2_1 : launch_yee_tb3_2_0)(f, x, fp64, nb, st);
     else (loss ? launch_yee_tb3_3_1 : launch_yee_tb3_3_0)(f, x, fp64, nb, st);
 }
 }  // namespace lfd
@@ -2867,6 +3078,40 @@ extern "C" int lf_run(lf_ctx *c, const lf_step *steps, int nsteps) {
 
 namespace {
 
+// Mur: the planes each face needs from before the step
+int absorb_capture(lf_ctx *c) {
+    for (MurArgs &m : c->mur) {
+        m.f = c->arr[c->mur_comp[&m - c->mur.data()]];
+        const int n = m.nu * m.nv;
+        mur_capture_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(m);
+    }
+    return 0;
+}
+
+// sponge on all six fields, then the Mur faces in their order, on the fields of the step
+int absorb_apply(lf_ctx *c) {
+    if (c->sp_n > 0) {
+        const int ids[6] = {LF_EX, LF_EY, LF_EZ, LF_HX, LF_HY, LF_HZ};
+        for (int q = 0; q < 6; ++q) {
+            SpongeArgs a{};
+            a.f = c->arr[ids[q]];
+            comp_extent(c, ids[q], a.ex, a.ey, a.ez);
+            a.g = c->g;
+            a.damp = c->sp_g;
+            a.n = c->sp_n;
+            a.faces = c->sp_faces;
+            sponge_kernel<<<dim3(a.ey, a.ex, 1), 64, 0, c->stream>>>(a);
+        }
+    }
+    for (size_t q = 0; q < c->mur.size(); ++q) {
+        MurArgs &m = c->mur[q];
+        m.f = c->arr[c->mur_comp[q]];
+        const int n = m.nu * m.nv;
+        mur_apply_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(m);
+    }
+    return 0;
+}
+
 // monitors whose bit is set in `mask`, on the fields at the end of step t
 int launch_monitors(lf_ctx *c, const MonArgs &ma, const lf_step &t, const StepArgs &st,
                     unsigned mask) {
@@ -2911,6 +3156,8 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
     if (c->kind == LF_YEE3D && (!c->arr[LF_CEX] || !c->arr[LF_CEY] || !c->arr[LF_CEZ]))
         return fail(-1, "lf_run: ce_x/ce_y/ce_z not uploaded");
     const int fp64 = c->mode == LF_MODE_FP64ACC;
+    const bool absorbers = c->sp_n > 0 || !c->mur.empty();
+    if (absorbers) allow_tb = false;     // sponge / Mur act after every step
     const Grid &g = c->g;
     dim3 blk(64, 4, 1);
     dim3 grd2((g.nz + 63) / 64, (g.nx + 3) / 4, 1);                    // 2D: (fast y, x)
@@ -3005,6 +3252,8 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         }
         c->tb_chain = false;
         if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
+        if (!c->mur.empty())
+            if (int e = absorb_capture(c)) return e;
         if (c->kind == LF_TE2D) {
             te_h_kernel<<<grd2, blk, 0, c->stream>>>(te);
             te_e_kernel<<<grd2, blk, 0, c->stream>>>(te, c->ex, st, fp64);
@@ -3014,6 +3263,22 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         } else if (c->kind == LF_TM2D) {
             tm_h_kernel<<<grd2, blk, 0, c->stream>>>(tm);
             tm_e_kernel<<<grd2, blk, 0, c->stream>>>(tm, c->ex, st, fp64);
+        } else if (allow_tb && c->tb > 1) {
+            // 3D steps that no pass of depth c->tb can take (monitors on a step before a
+            // pass's last, or too few steps left): a pass of depth 1, the same tile code
+            // with one level (one read and one write of every field instead of the two
+            // sweeps of the H then E kernels).  It sums the detector boxes itself; the
+            // other monitors run after it.
+            if (int e = tb3_plan(c, 1, c->tb_lx)) return e;
+            int fid[2];
+            const unsigned fused = c->tb3_ntile <= kTB3BoxSlots ? tb3_fused_boxes(c, fid) : 0u;
+            if (int e = tb_chunk_3d(c, &t, 1, fp64)) return e;
+            yee = yee_args(c);
+            ma = mon_args(c);
+            if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
+            ++ngroup;
+            if (int e = launch_monitors(c, ma, t, st, t.mask & ~fused)) return e;
+            continue;
         } else if (c->variant == 0) {
             if (int e = yee_fused_step(c, st, fp64)) return e;
             ma = mon_args(c);
@@ -3021,6 +3286,8 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
             yee_h_kernel<<<grd3, blk, 0, c->stream>>>(yee);
             yee_e_kernel<<<grd3, blk, 0, c->stream>>>(yee, c->ex, st, fp64, 0);
         }
+        if (absorbers)
+            if (int e = absorb_apply(c)) return e;
         if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup + 1], c->stream));
         ++ngroup;
         if (int e = launch_monitors(c, ma, t, st, t.mask)) return e;
@@ -3399,6 +3666,8 @@ int lf_batch_add(lf_batch *b, lf_ctx *c) {
     if (int e = check_ctx(c)) return e;
     if (!c->finalized) return fail(-1, "lf_batch_add: context not finalized");
     if (c->kind == LF_YEE3D) return fail(-1, "lf_batch_add: 2D contexts only");
+    if (c->sp_n > 0 || !c->mur.empty())
+        return fail(-1, "lf_batch_add: contexts with Mur faces run alone (lf_run)");
     if (c->mode != (b->ctx.empty() ? c->mode : b->ctx[0]->mode))
         return fail(-1, "lf_batch_add: all instances need the same precision mode");
     for (lf_ctx *o : b->ctx) if (o == c) return fail(-1, "lf_batch_add: context already in batch");

@@ -59,9 +59,11 @@ struct TB3Args {
     double *bpart[2];
 };
 
-// launch of one blocked pass: depth T (2 or 3), loss factors present or not
+// launch of one blocked pass: depth T (1, 2 or 3), loss factors present or not
 void launch_yee_tb3(int T, bool loss, const TB3Args &f, const Excite &x, int fp64, unsigned nb,
                     hipStream_t st);
+void launch_yee_tb3_1_0(const TB3Args &, const Excite &, int, unsigned, hipStream_t);
+void launch_yee_tb3_1_1(const TB3Args &, const Excite &, int, unsigned, hipStream_t);
 void launch_yee_tb3_2_0(const TB3Args &, const Excite &, int, unsigned, hipStream_t);
 void launch_yee_tb3_2_1(const TB3Args &, const Excite &, int, unsigned, hipStream_t);
 void launch_yee_tb3_3_0(const TB3Args &, const Excite &, int, unsigned, hipStream_t);

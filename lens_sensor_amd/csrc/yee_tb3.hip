@@ -1,12 +1,12 @@
 // yee_tb3.hip - kernels of the temporally blocked 3D pass.  Built once per (depth,
-// loss) variant with -DLF_TB3_T=<2|3> -DLF_TB3_LOSS=<0|1>, each object defining its
+// loss) variant with -DLF_TB3_T=<1|2|3> -DLF_TB3_LOSS=<0|1>, each object defining its
 // launcher; lensfdtd.hip calls them through launch_yee_tb3.
 #include <hip/hip_runtime.h>
 
 #include "yee_tb3.h"
 
 #ifndef LF_TB3_T
-#error "LF_TB3_T must be defined (2 or 3)"
+#error "LF_TB3_T must be defined (1, 2 or 3)"
 #endif
 #ifndef LF_TB3_LOSS
 #error "LF_TB3_LOSS must be defined (0 or 1)"

@@ -30,7 +30,7 @@ BUILD_DIR = os.path.join(PKG, '_build')
 ARCH = os.environ.get('LENSFDTD_ARCH', 'gfx950')
 FLAGS = [f'--offload-arch={ARCH}', '-O3', '-std=c++17', '-fPIC', '-ffp-contract=off',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
-TB3_VARIANTS = [(2, 0), (2, 1), (3, 0), (3, 1)]       # (depth, loss factors present)
+TB3_VARIANTS = [(1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1)]       # (depth, loss factors present)
 
 
 def _jobs():

@@ -35,7 +35,8 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_op_te_update_H', 'lf_op_te_update_E', 'lf_op_tm_update_H', 'lf_op_tm_update_E',
            'lf_op_te_terminate', 'lf_op_tm_terminate', 'lf_op3d_create', 'lf_op3d_set_cpml',
            'lf_op3d_finalize', 'lf_op3d_step_H', 'lf_op3d_step_E', 'lf_op3d_destroy',
-           'lf_monitor_change', 'lf_norm_diff', 'lf_norm_diff_noalloc')
+           'lf_monitor_change', 'lf_norm_diff', 'lf_norm_diff_noalloc', 'lf_set_sponge',
+           'lf_add_mur')
 
 
 class OpProfiles2D(ctypes.Structure):
@@ -126,6 +127,8 @@ def load():
                               ctypes.POINTER(ctypes.c_double)],
         'lf_norm_diff': [P, P, LL, I, ctypes.c_double, ctypes.c_double, P, P],
         'lf_norm_diff_noalloc': [P, P, LL, I, ctypes.c_double, ctypes.c_double, P, P],
+        'lf_set_sponge': [P, I, fp, ctypes.c_uint],
+        'lf_add_mur': [P, I, I, I, fp, LL, I],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -186,6 +189,7 @@ class HipEngine:
         self.h = h
         self._lib = lib
         self.mon_kind = []
+        self.absorbers = False          # sponge / Mur faces set: one step per launch
 
     @property
     def wide_acc(self):
@@ -247,6 +251,23 @@ class HipEngine:
         w = np.ascontiguousarray(wtap, dtype=np.float32)
         _check(self._lib.lf_set_drive(self.h, int(ncells), _fptr(w), len(comps), cid, _fptr(buf),
                                       0 if math == 'c128' else 1), 'lf_set_drive')
+
+    def set_sponge(self, n, g, faces):
+        """Graded sponge (3D): n cells, factors g (g[0] outermost), faces bitmask
+        (bit 2*axis + hi)."""
+        a = np.ascontiguousarray(g, dtype=np.float32).ravel()
+        if a.size != int(n):
+            raise ValueError('sponge: need n damping factors')
+        _check(self._lib.lf_set_sponge(self.h, int(n), _fptr(a), int(faces)), 'lf_set_sponge')
+        self.absorbers = True
+
+    def add_mur(self, comp, axis, hi, m, flags):
+        """First-order Mur on face (axis, hi) of E component comp; m scalar or one value
+        per cell of the boundary plane; faces apply in the order added."""
+        a = np.ascontiguousarray(np.atleast_1d(m), dtype=np.float32).ravel()
+        _check(self._lib.lf_add_mur(self.h, ARR[comp], int(axis), int(bool(hi)), _fptr(a), a.size,
+                                    int(flags)), 'lf_add_mur')
+        self.absorbers = True
 
     def write_field(self, comp, arr):
         self._upload(comp, arr)

@@ -34,7 +34,8 @@ from lens_sensor_amd.common import vcsel                                      # 
 from lens_sensor_amd.common.layer_stack import (stack_total_um, legacy_stack,  # noqa: E402
                                                 legacy_layers_tuple, describe as describe_stack)
 from lens_sensor_amd.engine.spec import new_steps, fill_phasors, cw_envelope, pulse_envelope  # noqa: E402
-from lens_sensor_amd.planar_3d.boundaries3d import make_boundary_3d, ALL_FACES  # noqa: E402
+from lens_sensor_amd.planar_3d.boundaries3d import (make_boundary_3d, ALL_FACES,  # noqa: E402
+                                                    mur_speeds_3d, Sponge3D)
 
 BEAD_SUBPIXEL = True
 KAPPA = {}                 # extinction per layer key (wg, lip, aq, mu, co, bead)
@@ -327,6 +328,14 @@ def _sponge_cells(n_sponge, sponge_um, dx_um, shape=None, warn=True):
     return n
 
 
+def _abs_cells(bkind, npml, n_sponge, sponge_um, dx_um):
+    """Outer absorber thickness in cells for the stack sizing (fdtd3d_core.py:852-854):
+    the CPML's cells, none for Mur, the sponge's cells."""
+    if bkind == 'cpml':
+        return npml
+    return 0 if bkind == 'mur1' else _sponge_cells(n_sponge, sponge_um, dx_um)
+
+
 def _resolve_tear_um(tear_um, t_lip_um, t_aq_um, t_mu_um, n_abs, dx_um, cornea_um=1.5, stack=None):
     if tear_um is not None:
         return float(tear_um)
@@ -430,8 +439,10 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
     f0 = C0 / LAM
     npml = NPML if npml is None else int(npml)
     bkind = str(boundary or 'cpml').lower()
-    tear_um = _resolve_tear_um(tear_um, t_lip_um, t_aq_um, t_mu_um,
-                               npml if bkind == 'cpml' else _sponge_cells(n_sponge, sponge_um, dx_um),
+    if bkind not in ('cpml', 'sponge', 'mur1'):
+        raise ValueError(f"unknown boundary '{bkind}' (sponge|cpml|mur1)")
+    tear_um = _resolve_tear_um(tear_um, t_lip_um, t_aq_um, t_mu_um, _abs_cells(bkind, npml, n_sponge,
+                                                                             sponge_um, dx_um),
                                dx_um, stack=stack)
     ly_um = t_wg_um + air_um + tear_um
     y0_um = -tear_um
@@ -443,7 +454,8 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
           f'dx={dx_nm}nm, dt={dt * 1e15:.4f} fs, engine {backend.engine_name()}')
     pec_faces = tuple(pec_faces or ())
     absorbing = tuple(f for f in ALL_FACES if f not in pec_faces)
-    bnd = make_boundary_3d(boundary, dx, dt, absorbing, shape=(Nx, Ny, Nz), npml=npml, lam_m=LAM)
+    bnd = make_boundary_3d(boundary, dx, dt, absorbing, shape=(Nx, Ny, Nz), npml=npml, lam_m=LAM,
+                           pec_faces=pec_faces)
     _vd = float(input_gap_um or 0.0)
     x_src_um, x_wg_start_um, _air, _zsrc = _vcsel_geom_3d(n_sp, dx_um, _vd)
     tp = {}
@@ -452,6 +464,10 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
                       n_clad_side=n_clad_side, end_facet_um=end_facet_um,
                       x_wg_start_um=x_wg_start_um, tensor_patch=tp, stack=stack)
     eps, eim = split_eps(eps)
+    if bkind == 'mur1':
+        # Mur with the local speeds (x faces n_eff, y/z faces n of the cells),
+        # fdtd3d_core.py:940-947
+        bnd.set_speeds(mur_speeds_3d(eps, dx, LAM))
     inv, da = inv_loss(eps, eim, dt, dx, LAM)
     del eps, eim
     cex, cey, cez = ce_arrays(inv, tp, dt, dx, dx_um, da=da)
@@ -473,7 +489,13 @@ def run_3d(label, wg_n, t_wg_um=5.0, t_lip_um=0.0, t_aq_um=3.5, t_mu_um=0.5,
                    dax=None if da is None else da[0], day=None if da is None else da[1],
                    daz=None if da is None else da[2])
     del cex, cey, cez
-    bnd.attach(eng)
+    if bkind == 'cpml':
+        bnd.attach(eng)
+    elif bkind == 'mur1':
+        bnd.attach(eng, (Nx, Ny, Nz), pec_faces=pec_faces)
+    else:                                    # graded sponge (fdtd3d_core.py:999, 1133-1135)
+        Sponge3D(n_sp, sponge_alpha, absorbing).attach(eng, (Nx, Ny, Nz), dx, dt,
+                                                      pec_faces=pec_faces)
     eng.finalize()
     eng.set_source(src_comp, src_ix, vs.amp, vs.ph1, vs.ph2)
 
@@ -704,9 +726,11 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
     sigma_t = 4 / (2 * np.pi * f0)
     npml = NPML if npml is None else int(npml)
     bkind = str(boundary or 'cpml').lower()
-    if bkind != 'cpml':
-        raise NotImplementedError(f"boundary '{bkind}' is not part of the device path; use 'cpml'")
-    tear_um = _resolve_tear_um(tear_um, t_lip_um, t_aq_um, t_mu_um, npml, dx_um, stack=stack)
+    if bkind not in ('cpml', 'sponge', 'mur1'):
+        raise ValueError(f"unknown boundary '{bkind}' (sponge|cpml|mur1)")
+    tear_um = _resolve_tear_um(tear_um, t_lip_um, t_aq_um, t_mu_um,
+                               _abs_cells(bkind, npml, n_sponge, sponge_um, dx_um), dx_um,
+                               stack=stack)
     ly_um = t_wg_um + air_um + tear_um
     y0_um = -tear_um
     z0_um = -lz_um / 2.0
@@ -715,7 +739,7 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
     Nx_win = min(int(round(window_w_um / dx_um)), Nx_full)
     S_cells = max(1, min(int(round(slide_um / dx_um)), Nx_win))
     O_cells = Nx_win - S_cells
-    n_sp = _sponge_cells(n_sponge, sponge_um, dx_um, (Nx_win, Ny, Nz), warn=False)
+    n_sp = _sponge_cells(n_sponge, sponge_um, dx_um, (Nx_win, Ny, Nz), warn=bkind == 'sponge')
     _vd = float(input_gap_um or 0.0)
     x_src_um, x_wg_start_um, _air, _zsrc = _vcsel_geom_3d(n_sp, dx_um, _vd)
     EXT = max(n_sp, npml) + 4
@@ -775,20 +799,32 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
                           n_clad_side=n_clad_side, x0_um=x0_um, x_wg_start_um=x_wg_start_um,
                           x_wg_end_um=x_wg_end_um, tensor_patch=tp, stack=stack)
         eps, eim = split_eps(eps)
+        msp = mur_speeds_3d(eps, dx, LAM) if bkind == 'mur1' else None
         inv, da = inv_loss(eps, eim, dt, dx, LAM)
         del eps, eim
         cex, cey, cez = ce_arrays(inv, tp, dt, dx, dx_um, da=da)
         da = stagger_da(da)
         del inv
+        # outer faces (the user's absorber): xmin only in window 0 (the others are driven),
+        # xmax only in the last window (inner seams end in their own CPML)
         aface = tuple(f for f in ALL_FACES if not ((w > 0 and f == 'xmin') or (not last and f == 'xmax')))
-        pml = make_boundary_3d('cpml', dx, dt, aface + (() if last else ('xmax',)),
-                               shape=(Nxw, Ny, Nz), npml=npml, lam_m=LAM)
+        if bkind == 'cpml':                  # one CPML for the outer faces and the seam
+            pml = make_boundary_3d('cpml', dx, dt, aface + (() if last else ('xmax',)),
+                                   shape=(Nxw, Ny, Nz), npml=npml, lam_m=LAM)
+        else:                                # seam CPML on xmax only; sponge / Mur outside
+            pml = make_boundary_3d('cpml', dx, dt, () if last else ('xmax',),
+                                   shape=(Nxw, Ny, Nz), npml=npml, lam_m=LAM)
         eng = backend.make_engine('yee3d', (Nxw, Ny, Nz), mode)
         eng.set_coeffs(Ch, cex=cex, cey=cey, cez=cez,
                        dax=None if da is None else da[0], day=None if da is None else da[1],
                        daz=None if da is None else da[2])
         del cex, cey, cez
         pml.attach(eng)
+        if bkind == 'mur1':                  # fdtd3d_core.py:1611-1612, 1633-1634
+            make_boundary_3d('mur1', dx, dt, aface, speeds=msp).attach_faces(eng, (Nxw, Ny, Nz),
+                                                                             pec_after=False)
+        elif bkind == 'sponge':              # fdtd3d_core.py:1635-1637
+            Sponge3D(n_sp, sponge_alpha, aface).attach_faces(eng)
         eng.finalize()
         if w == 0:
             eng.set_source(src_comp, src_ix, vs.amp, vs.ph1, vs.ph2)

@@ -10,6 +10,13 @@ Profiles follow planar_beads/boundaries.py of the reference:
                                               update_H / update_E / terminate on device
                                               arrays through fused2d (HIP kernels)
   make_boundary        boundaries.py:277-289
+  mur1_coeff           boundaries.py:22-29    m = (c dt - dx)/(c dt + dx), c = c0/n
+  guided_neff          boundaries.py:32-53    n_eff of the fundamental mode of eps(y)
+  mur_speeds / MurSet  boundaries.py:56-93    local speeds of the Mur edges (x ends: n_eff,
+                                              y edges: n of the edge cells)
+  attach_mur_2d                               the inline first-order Mur of
+                                              fdtd2d_core.py:805-810, 831-833 (stitch
+                                              :1422-1433, :1458-1469) as engine faces
 
 Unlike the reference the psi auxiliaries are not host/CuPy arrays here: the step engine
 owns them and stores them only in the PML slabs (c != 0), where psi can become non-zero.
@@ -199,3 +206,88 @@ def make_boundary(kind, pol, Nx, Ny, dx, dt, npml=10, npml_x=None, npml_y=None, 
     te = str(pol).lower() in ('s', 'te')
     return (CPML_TE if te else CPML_TM)(Nx, Ny, dx, dt, npml=npml, npml_x=npml_x, npml_y=npml_y,
                                          lam_m=lam_m)
+
+
+# ----------------------------------------------------------------------------- Mur (first order)
+def mur1_coeff(dx, dt, n=1.0):
+    """Mur-1 coefficient at phase velocity c0/n (n scalar or array), float32 like the
+    reference's (boundaries.py:22-29)."""
+    c = C0 / np.asarray(n, dtype=np.float64)
+    m = (c * dt - dx) / (c * dt + dx)
+    return np.float32(m) if np.ndim(m) == 0 else m.astype(np.float32)
+
+
+def guided_neff(eps_col, dx, lam_m):
+    """n_eff of the fundamental mode of a 1-D profile eps(y): the largest eigenvalue of the
+    finite-difference operator E'' + k0^2 eps E near k0^2 max(eps) (shift-invert Lanczos,
+    scipy), sqrt(max eps) when the profile guides nothing (boundaries.py:32-53)."""
+    e = np.asarray(eps_col, dtype=np.float64)
+    k0 = 2 * np.pi / lam_m
+    try:
+        import scipy.sparse as sps
+        from scipy.sparse.linalg import eigsh
+        N = len(e)
+        diag = -2.0 / dx ** 2 + k0 ** 2 * e
+        off = np.full(N - 1, 1.0 / dx ** 2)
+        A = sps.diags([off, diag, off], [-1, 0, 1], format='csc')
+        beta2 = eigsh(A, k=1, sigma=k0 ** 2 * float(e.max()), which='LM',
+                      return_eigenvectors=False)[0]
+        ne = float(np.sqrt(max(beta2, 0.0)) / k0)
+        if np.isfinite(ne) and 1.0 <= ne <= np.sqrt(e.max()) + 1e-6:
+            return ne
+    except Exception:
+        pass
+    return float(np.sqrt(e.max()))
+
+
+def mur_speeds(eps_r, dx, lam_m):
+    """Local indices of the Mur edges from eps_r (Nx, Ny): the x ends take the guided
+    mode's n_eff of their column, the y edges the n of their cells (boundaries.py:56-65)."""
+    e = np.asarray(eps_r, dtype=np.float64)
+    return dict(xl=guided_neff(e[1, :], dx, lam_m), xr=guided_neff(e[-2, :], dx, lam_m),
+                b=np.sqrt(e[:, 0]), t=np.sqrt(e[:, -1]))
+
+
+class MurSet:
+    """Mur-1 coefficients per edge: scalars on the x ends, vectors along the y edges
+    (b/t on the Nx nodes, bh/th on the Nx-1 half cells of TM Ex), boundaries.py:68-93."""
+
+    def __init__(self, dx, dt, speeds=None):
+        sp = speeds or {}
+        self.xl = mur1_coeff(dx, dt, sp.get('xl', 1.0))
+        self.xr = mur1_coeff(dx, dt, sp.get('xr', 1.0))
+        if 'b' in sp:
+            nb, nt = np.asarray(sp['b']), np.asarray(sp['t'])
+            self.b, self.t = mur1_coeff(dx, dt, nb), mur1_coeff(dx, dt, nt)
+            self.bh = mur1_coeff(dx, dt, 0.5 * (nb[:-1] + nb[1:]))
+            self.th = mur1_coeff(dx, dt, 0.5 * (nt[:-1] + nt[1:]))
+        else:
+            self.b = self.t = self.bh = self.th = mur1_coeff(dx, dt)
+        self.speeds = sp
+
+    def describe(self):
+        sp = self.speeds
+        if not sp:
+            return 'Mur (vacuum c0)'
+        d = 'Mur with local velocity: x ends n_eff=%.4f / %.4f' % (sp.get('xl', 1), sp.get('xr', 1))
+        if 'b' in sp:
+            d += ', y edges n=%.3f..%.3f / %.3f..%.3f' % (np.min(sp['b']), np.max(sp['b']),
+                                                         np.min(sp['t']), np.max(sp['t']))
+        return d
+
+
+def attach_mur_2d(engine, pol, ms, x_lo=True, x_hi=True, y=True):
+    """The reference's inline first-order Mur of a 2D run as engine faces, in its order:
+    the x ends (TE Ez, TM Ey) with F0 = the current boundary value and the PEC of the
+    PML edges after them, then the y edges (TE Ez, TM Ex) (fdtd2d_core.py:805-810,
+    831-833; stitch :1422-1433, :1458-1469).  Call after the CPML profiles are attached."""
+    te = str(pol).lower() in ('s', 'te')
+    cx, cy = ('Ez', 'Ez') if te else ('Ey', 'Ex')
+    yb, yt = (ms.b, ms.t) if te else (ms.bh, ms.th)
+    if x_lo:
+        engine.add_mur(cx, 0, 0, ms.xl, 3)
+    if x_hi:
+        engine.add_mur(cx, 0, 1, ms.xr, 3)
+    if y:
+        engine.add_mur(cy, 1, 0, yb, 1)
+        engine.add_mur(cy, 1, 1, yt, 1)

@@ -31,7 +31,8 @@ from lens_sensor_amd.common import vcsel                                      # 
 from lens_sensor_amd.common.layer_stack import (stack_total_um, legacy_stack,  # noqa: E402
                                                 legacy_layers_tuple, describe as describe_stack)
 from lens_sensor_amd.engine.spec import new_steps, fill_phasors, cw_envelope, pulse_envelope  # noqa: E402
-from lens_sensor_amd.planar_beads.boundaries import make_boundary            # noqa: E402
+from lens_sensor_amd.planar_beads.boundaries import (make_boundary, MurSet, mur_speeds,  # noqa: E402
+                                                    attach_mur_2d)
 
 BEAD_SUBPIXEL = True
 N_POLYSTYR, N_PMMA, N_AQ, N_MUCIN, N_CORNEA, N_LIPID = 1.590, 1.491, 1.336, 1.342, 1.376, 1.480
@@ -445,9 +446,11 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
     dx = dx_nm * 1e-9
     dt = 0.5 * dx / (C0 * np.sqrt(2))
     NP = NPML if npml is None else int(npml)
-    if str(boundary).lower() != 'cpml':
-        raise NotImplementedError(f"boundary '{boundary}' is not part of the device path; use 'cpml'")
-    src_ix = NP + 2
+    bkind = str(boundary).lower()
+    if bkind not in ('cpml', 'mur1'):
+        raise ValueError(f"unknown boundary '{boundary}' (cpml|mur1)")
+    use_cpml = bkind == 'cpml'
+    src_ix = (NP + 2) if use_cpml else 5            # fdtd2d_core.py:628-629
     _vd = float(input_gap_um or 0.0)
     x_facet_um, _air, _zsrc = _vcsel_geom_2d(src_ix, dx, _vd)
     input_gap_um = x_facet_um
@@ -472,6 +475,9 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
     else:
         Ce_E, Ce_H = _coeffs(eps_r, sig, dt, dx)
         eng.set_coeffs(Ch, ceE=Ce_E, ceH=Ce_H)
+    # Mur with the local speeds (x ends: n_eff, y edges: n of the edge cells),
+    # fdtd2d_core.py:675-678
+    ms = None if use_cpml else MurSet(dx, dt, mur_speeds(eps_r, dx, LAM))
     del eps_r, sig
     f0 = C0 / LAM
     sigma_t = 4 / (2 * np.pi * f0)
@@ -481,8 +487,11 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
                                                        else float(n_co_l)), input_gap_um)
     iy_src, vs, vs_ey = _vcsel_source_2d(Ny, dx, dt, T_WG, src_ix, vcsel_offset, vcsel_waist,
                                          lambda_nm, vcsel_tilt, n_src, _zsrc, f0, y0_m=y0_m)
-    bnd = make_boundary('cpml', pol, Nx, Ny, dx, dt, npml=NP, lam_m=LAM)
+    bnd = make_boundary('cpml', pol, Nx, Ny, dx, dt, npml=NP if use_cpml else 0, lam_m=LAM)
     bnd.attach(eng)
+    if ms is not None:                               # inline Mur-1 (fdtd2d_core.py:805-833)
+        attach_mur_2d(eng, pol, ms)
+        print('  ' + ms.describe())
     eng.finalize()
     src = vs_ey if pol == 'p' else vs
     field_comp = 'Ey' if pol == 'p' else 'Ez'
@@ -663,8 +672,10 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
     called after each window."""
     if cad is not None:
         raise NotImplementedError('CAD geometry import is outside the device path')
-    if (boundary or 'cpml').lower() != 'cpml':
-        raise NotImplementedError("only 'cpml' outer boundaries are part of the device path")
+    bkind = (boundary or 'cpml').lower()
+    if bkind not in ('cpml', 'mur1'):
+        raise ValueError(f"stitch: unknown boundary '{bkind}' (cpml|mur1)")
+    ucpml = bkind == 'cpml'                # the user's choice for the outer edges
     global LAM_LOSS
     pol = 'p' if str(polarization).lower() in ('p', 'tm') else 's'
     kind = 'tm2d' if pol == 'p' else 'te2d'
@@ -687,7 +698,7 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
     dt = 0.5 * dx / (C0 * np.sqrt(2))
     NP = NPML if npml is None else int(npml)
     _vd = float(input_gap_um or 0.0)
-    x_facet_um, _air, _zsrc = _vcsel_geom_2d(NP + 2, dx, _vd)
+    x_facet_um, _air, _zsrc = _vcsel_geom_2d((NP + 2) if ucpml else 5, dx, _vd)
     input_gap_um = x_facet_um
     y0_m = -TEAR_BUFFER
     H_total = T_WG + AIR_BUFFER + TEAR_BUFFER
@@ -712,7 +723,7 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
     print(f'\n===== Bead-Demo [stitch]: {label} =====  {n_win} windows of {Nx_win}x{Ny}, overlap '
           f'{O_cells}, steps/window {steps_win}, pol {pol}, engine {backend.engine_name()}')
     Ch = np.float32(dt / (MU0 * dx))
-    src_ix = NP + 2
+    src_ix = (NP + 2) if ucpml else 5      # fdtd2d_core.py:1277
     iy0 = int(round((T_WG / 2 + vcsel_offset * 1e-6 - y0_m) / dx))
     n_src = _n_source_2d(src_ix * dx * 1e6, iy0, y0_m, dx, wg_n, T_WG,
                          n_aq_l if stack is None else (float(complex(stack[0]['n']).real) if stack
@@ -769,12 +780,19 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
             Ce_E, Ce_H = _coeffs(eps_r, sig, dt, dx)
             eng.set_coeffs(Ch, ceE=Ce_E, ceH=Ce_H)
             del Ce_E, Ce_H
+        ms = None if ucpml else MurSet(dx, dt, mur_speeds(eps_r, dx, LAM))
         del eps_r, sig
-        pml_l = w == 0
-        pml_r = True
+        # the CPML carries the seams (inner windows' right end, always) and the outer
+        # edges the user chose it for; the outer edges without it get Mur-1
+        # (fdtd2d_core.py:1390-1398)
+        pml_l = (w == 0) and ucpml
+        pml_r = (not last) or ucpml
         bnd = make_boundary('cpml', pol, Nxw, Ny, dx, dt,
-                            npml_x=(NP if pml_l else 0, NP if pml_r else 0), npml_y=NP, lam_m=LAM)
+                            npml_x=(NP if pml_l else 0, NP if pml_r else 0),
+                            npml_y=NP if ucpml else 0, lam_m=LAM)
         bnd.attach(eng)
+        if ms is not None:
+            attach_mur_2d(eng, pol, ms, x_lo=(w == 0 and not pml_l), x_hi=not pml_r, y=True)
         eng.finalize()
         if w == 0:
             eng.set_source(comp, src.ix, src.amp, src.ph1, src.ph2)

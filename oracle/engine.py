@@ -97,6 +97,8 @@ class OracleEngine:
         self.src = None
         self.drive = None
         self.mons = []
+        self.sponge = None
+        self.mur = []
 
     @property
     def wide_acc(self):
@@ -162,6 +164,20 @@ class OracleEngine:
         self.drive = dict(n=int(ncells), wtap=np.asarray(wtap, dtype=np.float32),
                           D={c: np.asarray(v, dtype=np.complex64) for c, v in drives.items()},
                           math=math)
+
+    def set_sponge(self, n, g, faces):
+        """Graded sponge of n cells on the faces of bitmask `faces` (bit 2*axis + hi),
+        factors g with g[0] at the outermost cell (fdtd3d_core.py:397-414)."""
+        self.sponge = (int(n), np.asarray(g, dtype=np.float32).astype(self.fdt), int(faces))
+
+    def add_mur(self, comp, axis, hi, m, flags):
+        """First-order Mur face (boundaries3d.py:99-140; 2D inline fdtd2d_core.py:805-833);
+        flags bit 0: F0_old is the current value (2D form), bit 1: zero the plane's cells
+        on PEC faces where comp is tangential afterwards."""
+        m = np.asarray(m)
+        m = m.astype(np.float32) if m.dtype != np.float64 or self.fdt != np.float64 else m
+        self.mur.append(dict(comp=comp, axis=int(axis), hi=bool(hi), m=m.astype(self.fdt),
+                             flags=int(flags), prev=None))
 
     def write_field(self, comp, arr):
         self.F[comp][...] = np.asarray(arr, dtype=self.fdt)
@@ -280,6 +296,80 @@ class OracleEngine:
                         self._monitor(m, row)
 
     def _step(self, row):
+        if self.mur:
+            self._mur_capture()
+        self._yee_step(row)
+        if self.sponge is not None:
+            self._apply_sponge()
+        for f in self.mur:
+            self._mur_apply(f)
+
+    # -- secondary absorbers
+    @staticmethod
+    def _plane(F, axis, idx):
+        return F[idx] if axis == 0 else (F[:, idx] if axis == 1 else F[:, :, idx])
+
+    def _mur_capture(self):
+        # the E planes before the step (boundaries3d.py:125-131; fdtd2d_core.py:795, 823-824)
+        for f in self.mur:
+            F = self.F[f['comp']]
+            i0, i1 = (-1, -2) if f['hi'] else (0, 1)
+            f['prev'] = (self._plane(F, f['axis'], i1).copy(),
+                         self._plane(F, f['axis'], i0).copy())
+
+    def _mur_apply(self, f):
+        # F0 <- F1_prev + m (F1_new - F0_old)   (boundaries3d.py:133-140; fdtd2d_core.py:805-810)
+        F = self.F[f['comp']]
+        ax = f['axis']
+        i0, i1 = (-1, -2) if f['hi'] else (0, 1)
+        f1_prev, f0_prev = f['prev']
+        f0_old = self._plane(F, ax, i0).copy() if f['flags'] & 1 else f0_prev
+        val = f1_prev + f['m'] * (self._plane(F, ax, i1) - f0_old)
+        if f['flags'] & 2:
+            # PEC faces (applied after the Mur faces in the reference): the plane's cells
+            # on PEC faces where the component is tangential
+            val = val.copy()
+            ndim = F.ndim
+            cdir = 2 if (ndim == 2 and f['comp'] == 'Ez') else 'xyz'.index(f['comp'][1])
+            others = [b for b in range(ndim) if b != ax]
+            for pos, b in enumerate(others):
+                if b == cdir or b not in self.pec:
+                    continue
+                lo, hi = self.pec[b]
+                sl_lo = [slice(None)] * val.ndim
+                sl_hi = [slice(None)] * val.ndim
+                sl_lo[pos], sl_hi[pos] = 0, -1
+                if lo:
+                    val[tuple(sl_lo)] = 0
+                if hi:
+                    val[tuple(sl_hi)] = 0
+        if ax == 0:
+            F[i0] = val
+        elif ax == 1:
+            F[:, i0] = val
+        else:
+            F[:, :, i0] = val
+
+    def _apply_sponge(self):
+        # all six fields, faces in the order x low/high, y low/high, z low/high
+        # (fdtd3d_core.py:406-414, applied at :1133-1135)
+        n, g, faces = self.sponge
+        for c in E_COMPS[self.kind] + H_COMPS[self.kind]:
+            F = self.F[c]
+            if faces & 1:
+                F[:n] *= g[:, None, None]
+            if faces & 2:
+                F[-n:] *= g[::-1][:, None, None]
+            if faces & 4:
+                F[:, :n] *= g[None, :, None]
+            if faces & 8:
+                F[:, -n:] *= g[::-1][None, :, None]
+            if faces & 16:
+                F[:, :, :n] *= g[None, None, :]
+            if faces & 32:
+                F[:, :, -n:] *= g[None, None, ::-1]
+
+    def _yee_step(self, row):
         if self.kind == 'te2d':
             self._te_H()
             self._te_E()

@@ -7,8 +7,8 @@ tests/emu/hip/hip_runtime.h and linked with tests/emu/emu.cpp into
 tests/emu/_build/liblensfdtd_emu.so.  The same translation units and defines as the
 device build (lens_sensor_amd/engine/build.py) are compiled.
 
-Loaded into a Python process that preloads libasan (tests/test_emu_bounds.py does
-this), the library runs the package's engines on the CPU with every out-of-bounds load
+Loaded into a Python process that preloads libasan (LD_PRELOAD of g++'s libasan.so,
+with LENSFDTD_EMU_LIB naming it), the library runs the package's engines on the CPU with every out-of-bounds load
 or store of a kernel reported by ASan.  Usage:  python tests/emu/build.py
 """
 import os
@@ -21,7 +21,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(REPO, 'lens_sensor_amd', 'csrc')
 INC = os.path.join(REPO, 'include')
-OUT = os.path.join(HERE, '_build')
+OUT = os.environ.get('LENSFDTD_EMU_OUT', os.path.join(HERE, '_build'))   # build folder
+sys.path.insert(0, REPO)
+from lens_sensor_amd.engine.build import TB3_VARIANTS  # noqa: E402  (same variants as the device build)
+
 SRC = os.path.join(OUT, 'src')
 LIB = os.path.join(OUT, 'liblensfdtd_emu.so')
 
@@ -138,7 +141,7 @@ def _jobs():
     for tm in (0, 1):
         tu('tb2d_mon.hip', [f'-DLF_TB2M_TM={tm}'], f'tb2d_mon_{tm}')
         tu('tb2d_pass.hip', [f'-DLF_TB2P_TM={tm}'], f'tb2d_pass_{tm}')
-    for T, L in [(2, 0), (2, 1), (3, 0), (3, 1)]:
+    for T, L in TB3_VARIANTS:
         tu('yee_tb3.hip', [f'-DLF_TB3_T={T}', f'-DLF_TB3_LOSS={L}'], f'yee_tb3_{T}_{L}')
     emu = os.path.join(OUT, 'emu.o')
     jobs.append((['g++', '-std=c++17', '-O1', '-g', '-fPIC', '-fsanitize=address', f'-I{HERE}',

@@ -9,7 +9,7 @@ fdtd3d_core.py:533-537) and is not reproduced by the current reference code; the
 three golden cases are re-run here together with TM, lossy and stitch cases that no
 published value covers.
 
-Usage (writes into this directory):  python tests/golden/make_golden.py
+Usage (writes into this directory):  python tests/golden/make_golden.py [2d_<case> 3d_<case> ...]
 The fixtures are data only (inputs, reduced outputs and small arrays); no reference
 source is stored.
 """
@@ -66,6 +66,10 @@ CASES_2D = {
                             window_w_um=4.0, slide_um=2.5, polarization='p',
                             kappa={'aq': 0.001, 'bead': 0.002}),
 }
+# the same cases with the first-order Mur outer boundary (fdtd2d_core.py:805-833, stitch
+# :1422-1469)
+for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch', 'tm_stitch_lossy'):
+    CASES_2D[_n + '_mur1'] = dict(CASES_2D[_n], boundary='mur1')
 
 CASES_3D = {
     'te_full_char': dict(fn='run_3d', label='golden3d', wg_n=1.491, lx_um=4.0, air_um=1.0,
@@ -86,6 +90,11 @@ CASES_3D = {
                           bead=dict(x_um=3.0, d_um=0.6, n=1.59, z_um=0.1), n_sponge=8,
                           kappa={'aq': 0.001, 'bead': 0.002}),
 }
+# the graded sponge and the first-order Mur as outer absorbers (fdtd3d_core.py:397-414,
+# 1112-1135; boundaries3d.py:99-140), full domain and stitch windows
+for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch_char'):
+    for _b in ('sponge', 'mur1'):
+        CASES_3D[f'{_n}_{_b}'] = dict(CASES_3D[_n], boundary=_b)
 
 
 def run_2d(d2, name, case):
@@ -181,12 +190,21 @@ def setup_vectors(d2, f3d):
     return out
 
 
-def main():
+def main(only=None):
+    """only: names (e.g. 2d_te_full_char_mur1) to write; None writes every fixture."""
     d2, f3d = _import_reference()
     prev = os.getcwd()
     tmp = tempfile.mkdtemp(prefix='lens_fixtures_')
     try:
         os.chdir(tmp)
+        if only:
+            for key in only:
+                dim, name = key.split('_', 1)
+                res = (run_2d(d2, name, CASES_2D[name]) if dim == '2d'
+                       else run_3d(f3d, name, CASES_3D[name]))
+                np.savez_compressed(os.path.join(HERE, f'ref_{key}.npz'), **res)
+                print(key, res['transmission'], res['P_in'], res['P_out'], flush=True)
+            return
         for name, case in CASES_2D.items():
             res = run_2d(d2, name, case)
             np.savez_compressed(os.path.join(HERE, f'ref_2d_{name}.npz'), **res)
@@ -203,4 +221,4 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    main(sys.argv[1:] or None)

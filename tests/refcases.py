@@ -41,6 +41,10 @@ CASES_2D = {
                             window_w_um=4.0, slide_um=2.5, polarization='p',
                             kappa={'aq': 0.001, 'bead': 0.002}),
 }
+# the same cases with the first-order Mur outer boundary (fdtd2d_core.py:805-833, stitch
+# :1422-1469)
+for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch', 'tm_stitch_lossy'):
+    CASES_2D[_n + '_mur1'] = dict(CASES_2D[_n], boundary='mur1')
 
 CASES_3D = {
     'te_full_char': dict(fn='run_3d', label='golden3d', wg_n=1.491, lx_um=4.0, air_um=1.0,
@@ -61,6 +65,11 @@ CASES_3D = {
                           bead=dict(x_um=3.0, d_um=0.6, n=1.59, z_um=0.1), n_sponge=8,
                           kappa={'aq': 0.001, 'bead': 0.002}),
 }
+# the graded sponge and the first-order Mur as outer absorbers (fdtd3d_core.py:397-414,
+# 1112-1135; boundaries3d.py:99-140), full domain and stitch windows
+for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch_char'):
+    for _b in ('sponge', 'mur1'):
+        CASES_3D[f'{_n}_{_b}'] = dict(CASES_3D[_n], boundary=_b)
 
 
 def load_fixture(dim, name):

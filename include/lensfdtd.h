@@ -69,6 +69,15 @@ int lf_device_name(char *buf, int len);
 int lf_upload(lf_ctx *ctx, int array_id, const float *host, long long count);
 int lf_download(lf_ctx *ctx, int array_id, float *host, long long count);
 int lf_set_ch(lf_ctx *ctx, float ch);
+/* 2D coefficient array assembled on the device from its compact form (instead of
+ * uploading the full array): row j of column i is band[i*rows + j - j0] for
+ * j0 <= j < j0 + rows, else table[xclass[i]*ny_c + j]; xclass has the array's x extent,
+ * table ncls rows of its y extent.  Replaces the host-side fill of the per-cell arrays by
+ * build_materials + _coeffs / _tm_coeffs + _apply_bead_tm + _tm_loss (fdtd2d_core.py:122-349,
+ * :445-455) and their transfer; the package computes the compact form with the same
+ * expressions on sub-grids (window_coeffs). */
+int lf_build_coeff(lf_ctx *ctx, int array_id, const unsigned char *xclass, int ncls,
+                   const float *table, int j0, int rows, const float *band);
 /* CPML profiles of one axis (boundaries.py:104-151, boundaries3d.py:150-194):
  * e_* on the n integer nodes, h_* on the n-1 half nodes; PEC on the PML ends. */
 int lf_set_cpml(lf_ctx *ctx, int axis, const float *e_kappa, const float *e_b, const float *e_c,

@@ -1141,6 +1141,19 @@ __global__ void __launch_bounds__(256) box_kernel(MonArgs m, BoxArgs b) {
 // (2D planes), otherwise 64 x 4 tiles (3D volumes).
 static inline dim3 mon_block(int n1) { return n1 == 1 ? dim3(256, 1, 1) : dim3(64, 4, 1); }
 
+// 2D coefficient array from its compact form: row j of column i takes the band value where
+// j0 <= j < j0 + rows, else the value of row j in table row xclass[i]
+__global__ void __launch_bounds__(256) coeff_assemble_kernel(float *dst, const unsigned char *xc,
+                                                             const float *tab, const float *band,
+                                                             int ny, int j0, int rows, int P) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= ny) return;
+    const int r = j - j0;
+    dst[(long long)i * P + j] = (r >= 0 && r < rows) ? band[(long long)i * rows + r]
+                                                      : tab[(long long)xc[i] * ny + j];
+}
+
 __global__ void scatter_kernel(float *dst, const float *src, int nx, int ny, int nz, int NY, int P) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)nx * ny * nz;
@@ -1617,6 +1630,39 @@ int lf_upload(lf_ctx *c, int id, const float *host, long long count) {
     LF_HIP(hipGetLastError());
     LF_HIP(hipStreamSynchronize(c->stream));
     LF_HIP(hipFree(tmp));
+    return 0;
+}
+
+int lf_build_coeff(lf_ctx *c, int id, const unsigned char *xclass, int ncls, const float *table,
+                   int j0, int rows, const float *band) {
+    if (int r = check_ctx(c)) return r;
+    if (c->kind == LF_YEE3D) return fail(-1, "lf_build_coeff: 2D contexts only");
+    if (id < LF_CEE || id >= LF_NARRAYS) return fail(-1, "lf_build_coeff: not a coefficient id");
+    int ex, ey, ez;
+    comp_extent(c, id, ex, ey, ez);
+    if (ncls < 1 || ncls > 255 || !xclass || !table) return fail(-1, "lf_build_coeff: bad class table");
+    if (rows < 0 || j0 < 0 || j0 + rows > ez || (rows > 0 && !band))
+        return fail(-1, "lf_build_coeff: band rows outside the array");
+    for (int i = 0; i < ex; ++i)
+        if (xclass[i] >= ncls) return fail(-1, "lf_build_coeff: class index out of range");
+    if (int r = ensure_array(c, id)) return r;
+    const size_t nt = (size_t)ncls * ez, nb = (size_t)ex * rows;
+    unsigned char *dxc = nullptr;
+    float *dtab = nullptr, *dband = nullptr;
+    LF_HIP(hipMalloc(&dxc, (size_t)ex));
+    LF_HIP(hipMalloc(&dtab, nt * sizeof(float)));
+    if (nb) LF_HIP(hipMalloc(&dband, nb * sizeof(float)));
+    LF_HIP(hipMemcpyAsync(dxc, xclass, (size_t)ex, hipMemcpyHostToDevice, c->stream));
+    LF_HIP(hipMemcpyAsync(dtab, table, nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (nb) LF_HIP(hipMemcpyAsync(dband, band, nb * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    coeff_assemble_kernel<<<dim3((ez + 255) / 256, ex, 1), 256, 0, c->stream>>>(
+        c->arr[id], dxc, dtab, dband, ez, j0, rows, c->g.P);
+    LF_HIP(hipGetLastError());
+    LF_HIP(hipStreamSynchronize(c->stream));
+    LF_HIP(hipFree(dxc));
+    LF_HIP(hipFree(dtab));
+    if (dband) LF_HIP(hipFree(dband));
+    c->tb_chain = false;
     return 0;
 }
 

@@ -36,7 +36,7 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_op_te_terminate', 'lf_op_tm_terminate', 'lf_op3d_create', 'lf_op3d_set_cpml',
            'lf_op3d_finalize', 'lf_op3d_step_H', 'lf_op3d_step_E', 'lf_op3d_destroy',
            'lf_monitor_change', 'lf_norm_diff', 'lf_norm_diff_noalloc', 'lf_set_sponge',
-           'lf_add_mur')
+           'lf_add_mur', 'lf_build_coeff')
 
 
 class OpProfiles2D(ctypes.Structure):
@@ -129,6 +129,7 @@ def load():
         'lf_norm_diff_noalloc': [P, P, LL, I, ctypes.c_double, ctypes.c_double, P, P],
         'lf_set_sponge': [P, I, fp, ctypes.c_uint],
         'lf_add_mur': [P, I, I, I, fp, LL, I],
+        'lf_build_coeff': [P, I, ctypes.POINTER(ctypes.c_ubyte), I, fp, I, I, fp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -226,6 +227,19 @@ class HipEngine:
             if v is not None:
                 self._upload(k, v)
 
+    def build_coeffs(self, ch, packed):
+        """Coefficient arrays assembled on the device from their compact form
+        (fdtd2d_core.window_coeffs: name -> (xclass, table, j0, band))."""
+        _check(self._lib.lf_set_ch(self.h, float(np.float32(ch))), 'lf_set_ch')
+        for name, (xc, table, j0, band) in packed.items():
+            xc = np.ascontiguousarray(xc, dtype=np.uint8)
+            t = np.ascontiguousarray(table, dtype=np.float32)
+            b = np.ascontiguousarray(band, dtype=np.float32)
+            _check(self._lib.lf_build_coeff(self.h, ARR[name],
+                                            xc.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                            t.shape[0], _fptr(t), int(j0), b.shape[1], _fptr(b)),
+                   f'lf_build_coeff({name})')
+
     def set_cpml(self, axis, e_prof, h_prof, pec_lo, pec_hi):
         ek, eb, ec = (np.ascontiguousarray(x, dtype=np.float32) for x in e_prof)
         hk, hb, hc = (np.ascontiguousarray(x, dtype=np.float32) for x in h_prof)
@@ -271,6 +285,14 @@ class HipEngine:
 
     def write_field(self, comp, arr):
         self._upload(comp, arr)
+
+    def read_coeff(self, name):
+        """A per-cell coefficient array (reference shape) as the engine holds it."""
+        shp = comp_shape(self.kind, self.dims, {'ceE': 'node', 'ceH': 'node', 'cax': 'Ex',
+                                                 'cay': 'Ey', 'dax': 'Ex', 'day': 'Ey'}[name])
+        out = np.empty(shp, dtype=np.float32)
+        _check(self._lib.lf_download(self.h, ARR[name], _fptr(out), out.size), f'lf_download({name})')
+        return out
 
     def read_field(self, comp):
         shp = comp_shape(self.kind, self.dims, comp)

@@ -62,15 +62,14 @@ def _bead_list(place_bead, bead_x_um, bead_d_um, bead_y_um, bead_n, beads):
 
 
 # ----------------------------------------------------------------------------- materials
-def build_materials(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_start_um=0.0,
+def _material_parts(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_start_um=0.0,
                     place_bead=True, n_aq=N_AQ, n_mucin=N_MUCIN, n_cornea=N_CORNEA, t_aq_um=None,
                     t_mu_um=None, t_lip_um=0.0, n_lip=N_LIPID, input_gap_um=0.0, length_um=None,
                     bead_y_um=None, curved=False, r_bend_m=8.3e-3, tm_patch=None, stack=None,
                     beads=None):
-    """eps_r (Nx,Ny) float32 and sigma (Nx,Ny) float32 of the planar stack with the core on
-    [input_gap, L] x [0, t_wg] and disc beads blended by area fraction.
-    Reference fdtd2d_core.py:122-259.  ``beads`` (list of {x_um, d_um, y_um, n}) replaces
-    the single bead for several beads per window."""
+    """What build_materials fills the grid from: the two layer columns (outside / inside
+    the core's x range), the core's x range, the beads' boxes, the curvature metric.
+    Reference fdtd2d_core.py:122-259."""
     t_aq = (t_aq_um * 1e-6) if t_aq_um is not None else T_AQ
     t_mu = (t_mu_um * 1e-6) if t_mu_um is not None else T_MU
     t_lip = (t_lip_um or 0.0) * 1e-6
@@ -92,7 +91,6 @@ def build_materials(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_st
     else:
         k_co = 0.0
     edt = np.complex128 if lossy else np.float64
-    eps = np.full((Nx, Ny), N_AIR ** 2, dtype=np.complex128 if lossy else np.float32)
     ys = -TEAR_BUFFER + np.arange(Ny) * dx
     y_lip_bot = -t_lip
     y_aq_bot = y_lip_bot - t_aq
@@ -129,20 +127,21 @@ def build_materials(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_st
 
     col = cell_average_column(ys, dx, lambda y: eps_y(y, False))
     col_core = cell_average_column(ys, dx, lambda y: eps_y(y, True))
-    eps[:, :] = col[None, :]
     xs_abs = x_start_um * 1e-6 + np.arange(Nx) * dx
     in_wg_x = (xs_abs >= x_wg_start) & (xs_abs <= x_wg_end)
-    if in_wg_x.any():
-        eps[in_wg_x, :] = col_core[None, :]
-    if tm_patch is not None and not curved:
-        tm_patch['layer'] = dict(
+    layer = None
+    if not curved:
+        layer = dict(
             in_wg_x=in_wg_x,
             h_out=cell_average_column(ys[:-1], dx, lambda y: eps_y(y, False).real,
                                       harmonic=True, shift=0.5 * dx),
             h_core=cell_average_column(ys[:-1], dx, lambda y: eps_y(y, True).real,
                                        harmonic=True, shift=0.5 * dx))
+        if tm_patch is not None:
+            tm_patch['layer'] = layer
     x0 = x_start_um * 1e-6
     xs = x0 + np.arange(Nx) * dx
+    boxes = []
     for bd in blist:
         r = bd['d_um'] * 1e-6 / 2.0
         xc = bd['x_um'] * 1e-6
@@ -153,33 +152,74 @@ def build_materials(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_st
         yc = (bd['y_um'] * 1e-6) if bd.get('y_um') is not None else -r
         ix0 = max(0, int((xc - r - x0) / dx) - 2)
         ix1 = min(Nx, int((xc + r - x0) / dx) + 2)
+        jy0 = max(0, int((yc - r - ys[0]) / dx) - 2)
+        jy1 = min(Ny, int((yc + r - ys[0]) / dx) + 3)
+        boxes.append(dict(r=r, xc=xc, yc=yc, bn=bn, bn_c=bn_c, ix0=ix0, ix1=ix1, jy0=jy0, jy1=jy1))
+    metric = None
+    if curved:
+        u = ys - wg_top / 2.0
+        metric = (1.0 + u / r_bend_m).astype(np.float32) ** 2
+    return dict(Nx=Nx, Ny=Ny, dx=dx, lossy=lossy, col=col, col_core=col_core, in_wg_x=in_wg_x,
+                xs=xs, ys=ys, boxes=boxes, metric=metric, curved=curved, edt=edt, layer=layer)
+
+
+def _material_rows(mp, j0, j1, tm_patch=None):
+    """eps_r, sigma (float32) of the grid rows [j0, j1), all x: the layer columns, then
+    the beads blended by area fraction (every bead's box lies inside the rows), the
+    curvature metric, then the complex index split (fdtd2d_core.py:219-259)."""
+    Nx = mp['Nx']
+    col, col_core = mp['col'], mp['col_core']
+    eps = np.full((Nx, j1 - j0), N_AIR ** 2, dtype=np.complex128 if mp['lossy'] else np.float32)
+    eps[:, :] = col[None, j0:j1]
+    if mp['in_wg_x'].any():
+        eps[mp['in_wg_x'], :] = col_core[None, j0:j1]
+    xs, ys, dx = mp['xs'], mp['ys'], mp['dx']
+    for b in mp['boxes']:
+        ix0, ix1, jy0, jy1 = b['ix0'], b['ix1'], b['jy0'], b['jy1']
         if BEAD_SUBPIXEL:
-            jy0 = max(0, int((yc - r - ys[0]) / dx) - 2)
-            jy1 = min(Ny, int((yc + r - ys[0]) / dx) + 3)
             if ix1 > ix0 and jy1 > jy0:
-                if tm_patch is not None and not curved:
+                if not (j0 <= jy0 and jy1 <= j1):
+                    raise ValueError('bead box outside the material rows')
+                blk = eps[ix0:ix1, jy0 - j0:jy1 - j0]
+                if tm_patch is not None and not mp['curved']:
                     tm_patch.setdefault('beads', []).append(dict(
-                        eps_bg=eps[ix0:ix1, jy0:jy1].real.astype(np.float64), ix0=ix0, jy0=jy0,
-                        xs=xs[ix0:ix1], ys=ys[jy0:jy1], xc=xc, yc=yc, r=r, n=bn))
-                blend(eps[ix0:ix1, jy0:jy1],
-                      circle_fraction(xs[ix0:ix1], ys[jy0:jy1], xc, yc, r, dx), bn_c)
+                        eps_bg=blk.real.astype(np.float64), ix0=ix0, jy0=jy0,
+                        xs=xs[ix0:ix1], ys=ys[jy0:jy1], xc=b['xc'], yc=b['yc'], r=b['r'], n=b['bn']))
+                blend(blk, circle_fraction(xs[ix0:ix1], ys[jy0:jy1], b['xc'], b['yc'], b['r'], dx),
+                      b['bn_c'])
         else:
+            r, xc, yc = b['r'], b['xc'], b['yc']
+            yr = ys[j0:j1]
             for i in range(ix0, ix1):
                 ddx = xs[i] - xc
                 if abs(ddx) > r:
                     continue
                 ddy = np.sqrt(max(0.0, r * r - ddx * ddx))
-                eps[i, (ys >= yc - ddy) & (ys <= yc + ddy)] = bn_c ** 2
-    if curved:
-        u = ys - wg_top / 2.0
-        metric = (1.0 + u / r_bend_m).astype(np.float32) ** 2
-        eps = eps * metric[None, :]
-    if lossy:
+                eps[i, (yr >= yc - ddy) & (yr <= yc + ddy)] = b['bn_c'] ** 2
+    if mp['metric'] is not None:
+        eps = eps * mp['metric'][None, j0:j1]
+    if mp['lossy']:
         sig = loss_sigma(eps.imag, LAM_LOSS).astype(np.float32)
         eps = eps.real.astype(np.float32)
     else:
-        sig = np.zeros((Nx, Ny), dtype=np.float32)
+        sig = np.zeros((Nx, j1 - j0), dtype=np.float32)
     return eps, sig
+
+
+def build_materials(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_start_um=0.0,
+                    place_bead=True, n_aq=N_AQ, n_mucin=N_MUCIN, n_cornea=N_CORNEA, t_aq_um=None,
+                    t_mu_um=None, t_lip_um=0.0, n_lip=N_LIPID, input_gap_um=0.0, length_um=None,
+                    bead_y_um=None, curved=False, r_bend_m=8.3e-3, tm_patch=None, stack=None,
+                    beads=None):
+    """eps_r (Nx,Ny) float32 and sigma (Nx,Ny) float32 of the planar stack with the core on
+    [input_gap, L] x [0, t_wg] and disc beads blended by area fraction.
+    Reference fdtd2d_core.py:122-259.  ``beads`` (list of {x_um, d_um, y_um, n}) replaces
+    the single bead for several beads per window."""
+    mp = _material_parts(Nx, Ny, dx, wg_n, bead_n, wg_top, bead_x_um, bead_d_um, x_start_um,
+                         place_bead, n_aq, n_mucin, n_cornea, t_aq_um, t_mu_um, t_lip_um, n_lip,
+                         input_gap_um, length_um, bead_y_um, curved, r_bend_m, tm_patch, stack,
+                         beads)
+    return _material_rows(mp, 0, Ny, tm_patch)
 
 
 def _coeffs(eps_r, sig, dt, dx):
@@ -202,10 +242,13 @@ def _tm_coeffs(eps_r, dt, dx):
     return Ca_ex, Ca_ey, np.float32(dt / (MU0 * dx))
 
 
-def _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt):
+def _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt, lossy=None):
     """Semi-implicit TM loss: Ca /= (1+a) in place, returns D = (1-a)/(1+a) per component
-    or (None, None) (fdtd2d_core.py:283-294)."""
-    if sig is None or not bool(np.any(sig > 0)):
+    or (None, None) (fdtd2d_core.py:283-294).  ``lossy`` overrides the test of the given
+    sigma (a part of a grid takes the whole grid's decision)."""
+    if lossy is None:
+        lossy = sig is not None and bool(np.any(sig > 0))
+    if not lossy:
         return None, None
     s_ex = 0.5 * (sig[:-1, :] + sig[1:, :])
     e_ex = 0.5 * (eps_r[:-1, :] + eps_r[1:, :])
@@ -218,19 +261,21 @@ def _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt):
     return ((1 - a_ex) / (1 + a_ex)).astype(np.float32), ((1 - a_ey) / (1 + a_ey)).astype(np.float32)
 
 
-def _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx):
+def _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx, row0=0):
     """Layer-harmonic Ey coefficient and the anisotropic (Farjadpour) bead-edge tensor for
-    Ex/Ey: (1/eps)_ii = n_i^2 <1/eps> + (1 - n_i^2)/<eps>  (fdtd2d_core.py:297-349)."""
+    Ex/Ey: (1/eps)_ii = n_i^2 <1/eps> + (1 - n_i^2)/<eps>  (fdtd2d_core.py:297-349).
+    ``row0``: the arrays hold the grid rows from row0 on (a band of rows)."""
     if not bp:
         return
     c0 = dt / (EPS0 * dx)
     lp = bp.get('layer')
     h = None
     if lp is not None:
-        h = np.where(lp['in_wg_x'][:, None], lp['h_core'][None, :], lp['h_out'][None, :])
+        h = np.where(lp['in_wg_x'][:, None], lp['h_core'][None, row0:row0 + Ca_ey.shape[1]],
+                     lp['h_out'][None, row0:row0 + Ca_ey.shape[1]])
         Ca_ey[...] = (c0 / h).astype(np.float32)
     for bd in bp.get('beads', []):
-        e_bg, ix0, jy0 = bd['eps_bg'], bd['ix0'], bd['jy0']
+        e_bg, ix0, jy0 = bd['eps_bg'], bd['ix0'], bd['jy0'] - row0
         xs, ys, xc, yc, r, eb = bd['xs'], bd['ys'], bd['xc'], bd['yc'], bd['r'], bd['n'] ** 2
         c = dt / (EPS0 * dx)
 
@@ -261,6 +306,85 @@ def _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx):
         m = f > 0
         blk[m] = c * inv[m]
         Ca_ey[sl] = blk.astype(np.float32)
+
+
+def window_coeffs(pol, mp, dt, dx):
+    """Per-cell update coefficients of a 2D grid in compact form, for the engine to
+    assemble the full arrays on the device.  Outside the rows the beads touch, a cell's
+    coefficient depends only on its row and on which layer columns (outside / inside the
+    core's x range) its x neighbours hold, so each array is: a class per x (the columns
+    it reads), one row of values per class, and the explicit values of the rows the beads
+    touch (one more row each side for the staggered neighbours).  Every value comes from
+    the same NumPy expressions as the full arrays of build_materials + _coeffs (TE) or
+    _tm_coeffs + _apply_bead_tm + _tm_loss (TM), applied to sub-grids; those are
+    element-wise in float32, so expanding the compact form gives the full arrays bit for
+    bit.  Returns {name: (xclass uint8, table float32 (classes, ny_c), j0, band (nx_c, rows))}
+    and the TM Ch."""
+    Nx, Ny = mp['Nx'], mp['Ny']
+    core = mp['in_wg_x'].astype(np.uint8)
+    boxes = [b for b in mp['boxes'] if b['ix1'] > b['ix0'] and b['jy1'] > b['jy0']]
+    if boxes:
+        b0 = max(0, min(b['jy0'] for b in boxes) - 1)
+        b1 = min(Ny, max(b['jy1'] for b in boxes) + 1)
+    else:
+        b0 = b1 = 0
+    bp = {'layer': mp['layer']} if mp['layer'] is not None else {}
+    eps_b, sig_b = _material_rows(mp, b0, b1, tm_patch=bp)
+    # the two layer columns (outside / inside the core), full height
+    eps_c, sig_c = _material_rows(dict(mp, Nx=2, in_wg_x=np.array([False, True]), boxes=[]), 0, Ny)
+    used = sorted(set(core.tolist()))
+    out_rows = np.ones(Ny, bool)
+    out_rows[b0:b1] = False
+    lossy = bool(np.any(sig_b > 0)) or any(bool(np.any(sig_c[c][out_rows] > 0)) for c in used)
+    res = {}
+    if str(pol).lower() in ('s', 'te'):
+        tE, tH = _coeffs(eps_c, sig_c, dt, dx)
+        bE, bH = _coeffs(eps_b, sig_b, dt, dx)
+        res['ceE'] = (core, tE, b0, bE)
+        res['ceH'] = (core, tH, b0, bH)
+        return res, None
+    # TM.  Ex between columns i and i+1: four classes; Ey inside column i: two
+    cls_x = (2 * core[:-1] + core[1:]).astype(np.uint8)
+    tab = {k: [] for k in ('cax', 'dax', 'cay', 'day')}
+    for a, b in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        e2, s2 = eps_c[[a, b]], sig_c[[a, b]]
+        cax, cay, ch = _tm_coeffs(e2, dt, dx)
+        dax, _ = _tm_loss(e2, s2, cax, cay, dt, lossy=lossy)
+        tab['cax'].append(cax[0])
+        if dax is not None:
+            tab['dax'].append(dax[0])
+    lp = bp.get('layer')
+    for c in (0, 1):
+        e1, s1 = eps_c[[c]], sig_c[[c]]
+        cax, cay, _ = _tm_coeffs(e1, dt, dx)
+        if lp is not None:
+            _apply_bead_tm({'layer': dict(lp, in_wg_x=np.array([c == 1]))}, cax, cay, dt, dx)
+        _, day = _tm_loss(e1, s1, cax, cay, dt, lossy=lossy)
+        tab['cay'].append(cay[0])
+        if day is not None:
+            tab['day'].append(day[0])
+    cax_b, cay_b, ch = _tm_coeffs(eps_b, dt, dx) if b1 > b0 else (None, None, ch)
+    if b1 > b0:
+        _apply_bead_tm(bp, cax_b, cay_b, dt, dx, row0=b0)
+        dax_b, day_b = _tm_loss(eps_b, sig_b, cax_b, cay_b, dt, lossy=lossy)
+    else:
+        cax_b = np.zeros((Nx - 1, 0), np.float32)
+        cay_b = np.zeros((Nx, 0), np.float32)
+        dax_b, day_b = (cax_b, cay_b) if lossy else (None, None)
+    res['cax'] = (cls_x, np.array(tab['cax'], np.float32), b0, cax_b)
+    res['cay'] = (core, np.array(tab['cay'], np.float32), b0, cay_b)
+    if lossy:
+        res['dax'] = (cls_x, np.array(tab['dax'], np.float32), b0, dax_b)
+        res['day'] = (core, np.array(tab['day'], np.float32), b0, day_b)
+    return res, ch
+
+
+def expand_coeffs(packed):
+    """The full array of one compact coefficient (window_coeffs)."""
+    xc, table, j0, band = packed
+    full = table[xc.astype(np.intp)]
+    full[:, j0:j0 + band.shape[1]] = band
+    return full
 
 
 def _tm_node_field(Ey):
@@ -459,26 +583,21 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
     Nx, Ny = int(round(L / dx)), int(round(H_total / dx))
     print(f'\n===== Bead-Demo [full]: {label} =====  grid {Nx}x{Ny} dx={dx_nm}nm  pol {pol}  '
           f'engine {backend.engine_name()}')
-    bp = {}
-    eps_r, sig = build_materials(Nx, Ny, dx, wg_n, bead_n, T_WG, bx_um, bead_d_um, 0.0, place_bead,
-                                 n_aq=n_aq_l, n_mucin=n_mu_l, n_cornea=n_co_l, t_aq_um=t_aq_um,
-                                 t_mu_um=t_mucin_um, t_lip_um=t_lipid_um, n_lip=n_lip_l,
-                                 input_gap_um=input_gap_um, length_um=length_um, curved=curved,
-                                 tm_patch=bp, bead_y_um=bead_y_um, stack=stack, beads=beads)
+    # materials and coefficients (fdtd2d_core.py:664-672): computed in compact form on
+    # the host and assembled on the device (window_coeffs, lf_build_coeff)
+    mp = _material_parts(Nx, Ny, dx, wg_n, bead_n, T_WG, bx_um, bead_d_um, 0.0, place_bead,
+                         n_aq=n_aq_l, n_mucin=n_mu_l, n_cornea=n_co_l, t_aq_um=t_aq_um,
+                         t_mu_um=t_mucin_um, t_lip_um=t_lipid_um, n_lip=n_lip_l,
+                         input_gap_um=input_gap_um, length_um=length_um, curved=curved,
+                         bead_y_um=bead_y_um, stack=stack, beads=beads)
     Ch = np.float32(dt / (MU0 * dx))
     eng = backend.make_engine(kind, (Nx, Ny), _engine_mode(engine_mode))
-    if pol == 'p':
-        Ca_ex, Ca_ey, Ch_tm = _tm_coeffs(eps_r, dt, dx)
-        _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx)
-        Da_ex, Da_ey = _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt)
-        eng.set_coeffs(Ch_tm, cax=Ca_ex, cay=Ca_ey, dax=Da_ex, day=Da_ey)
-    else:
-        Ce_E, Ce_H = _coeffs(eps_r, sig, dt, dx)
-        eng.set_coeffs(Ch, ceE=Ce_E, ceH=Ce_H)
+    packed, ch_tm = window_coeffs(pol, mp, dt, dx)
+    eng.build_coeffs(ch_tm if pol == 'p' else Ch, packed)
+    del packed
     # Mur with the local speeds (x ends: n_eff, y edges: n of the edge cells),
     # fdtd2d_core.py:675-678
-    ms = None if use_cpml else MurSet(dx, dt, mur_speeds(eps_r, dx, LAM))
-    del eps_r, sig
+    ms = None if use_cpml else MurSet(dx, dt, mur_speeds(_material_rows(mp, 0, Ny)[0], dx, LAM))
     f0 = C0 / LAM
     sigma_t = 4 / (2 * np.pi * f0)
     iy0 = int(round((T_WG / 2 + vcsel_offset * 1e-6 - y0_m) / dx))
@@ -762,26 +881,18 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
         if w == 0:
             fpl.update(src_up=vs.plane_up, src_dn=vs.plane_dn)
         Nxw = Nx_win + (0 if last else EXT)
-        bp = {}
-        eps_r, sig = build_materials(Nxw, Ny, dx, wg_n, bead_n, T_WG, bx_um, bead_d_um, x_start_um,
-                                     place_bead, n_aq=n_aq_l, n_mucin=n_mu_l, n_cornea=n_co_l,
-                                     t_aq_um=t_aq_um, t_mu_um=t_mucin_um, t_lip_um=t_lipid_um,
-                                     n_lip=n_lip_l, input_gap_um=input_gap_um, length_um=len_mat,
-                                     curved=curved, tm_patch=bp, bead_y_um=bead_y_um, stack=stack,
-                                     beads=beads)
+        # the window's materials and coefficients (fdtd2d_core.py:1350-1379), computed in
+        # compact form on the host and assembled on the device (window_coeffs)
+        mp = _material_parts(Nxw, Ny, dx, wg_n, bead_n, T_WG, bx_um, bead_d_um, x_start_um,
+                             place_bead, n_aq=n_aq_l, n_mucin=n_mu_l, n_cornea=n_co_l,
+                             t_aq_um=t_aq_um, t_mu_um=t_mucin_um, t_lip_um=t_lipid_um,
+                             n_lip=n_lip_l, input_gap_um=input_gap_um, length_um=len_mat,
+                             curved=curved, bead_y_um=bead_y_um, stack=stack, beads=beads)
         eng = backend.make_engine(kind, (Nxw, Ny), mode)
-        if pol == 'p':
-            Ca_ex, Ca_ey, Ch_tm = _tm_coeffs(eps_r, dt, dx)
-            _apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx)
-            Da_ex, Da_ey = _tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt)
-            eng.set_coeffs(Ch_tm, cax=Ca_ex, cay=Ca_ey, dax=Da_ex, day=Da_ey)
-            del Ca_ex, Ca_ey, Da_ex, Da_ey
-        else:
-            Ce_E, Ce_H = _coeffs(eps_r, sig, dt, dx)
-            eng.set_coeffs(Ch, ceE=Ce_E, ceH=Ce_H)
-            del Ce_E, Ce_H
-        ms = None if ucpml else MurSet(dx, dt, mur_speeds(eps_r, dx, LAM))
-        del eps_r, sig
+        packed, ch_tm = window_coeffs(pol, mp, dt, dx)
+        eng.build_coeffs(ch_tm if pol == 'p' else Ch, packed)
+        del packed
+        ms = None if ucpml else MurSet(dx, dt, mur_speeds(_material_rows(mp, 0, Ny)[0], dx, LAM))
         # the CPML carries the seams (inner windows' right end, always) and the outer
         # edges the user chose it for; the outer edges without it get Mur-1
         # (fdtd2d_core.py:1390-1398)

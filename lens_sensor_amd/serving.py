@@ -148,22 +148,16 @@ class Solve2D(_CWSolve):
         try:
             src_ix = npml + 2
             x_facet_um, _air, zsrc = d2._vcsel_geom_2d(src_ix, dx, 0.0)
-            bp = {}
-            eps_r, sig = d2.build_materials(
+            mp = d2._material_parts(
                 Nx, Ny, dx, wg_n, 1.59, T_WG, beads[0]['x_um'], 2 * r, 0.0, False, n_aq=1.333,
                 n_mucin=1.34, n_cornea=1.376, t_aq_um=3.0, t_mu_um=0.5, t_lip_um=0.0,
                 n_lip=n_at('lipid', LAMBDA_NM), input_gap_um=x_facet_um, length_um=L_um,
-                curved=False, tm_patch=bp, bead_y_um=None, stack=stack, beads=beads)
+                curved=False, bead_y_um=None, stack=stack, beads=beads)
             self.eng = eng = (engine_factory or backend.make_engine)(self.kind, (Nx, Ny), mode)
-            if self.pol == 'p':
-                Ca_ex, Ca_ey, Ch_tm = d2._tm_coeffs(eps_r, dt, dx)
-                d2._apply_bead_tm(bp, Ca_ex, Ca_ey, dt, dx)
-                Da_ex, Da_ey = d2._tm_loss(eps_r, sig, Ca_ex, Ca_ey, dt)
-                eng.set_coeffs(Ch_tm, cax=Ca_ex, cay=Ca_ey, dax=Da_ex, day=Da_ey)
-            else:
-                Ce_E, Ce_H = d2._coeffs(eps_r, sig, dt, dx)
-                eng.set_coeffs(np.float32(dt / (MU0 * dx)), ceE=Ce_E, ceH=Ce_H)
-            del eps_r, sig
+            # coefficients in compact form, assembled by the engine (fdtd2d_core.window_coeffs)
+            packed, ch_tm = d2.window_coeffs(self.pol, mp, dt, dx)
+            eng.build_coeffs(ch_tm if self.pol == 'p' else np.float32(dt / (MU0 * dx)), packed)
+            del packed
             f0 = C0 / LAM
             y0_m = -tear
             n_src = d2._n_source_2d(src_ix * dx * 1e6, int(round((T_WG / 2 - y0_m) / dx)), y0_m, dx,

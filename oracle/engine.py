@@ -113,6 +113,15 @@ class OracleEngine:
         for k, v in arrays.items():
             self.coef[k] = None if v is None else np.asarray(v).astype(self.fdt)
 
+    def build_coeffs(self, ch, packed):
+        """Compact coefficients (name -> (xclass, table, j0, band)) expanded on the host."""
+        arrays = {}
+        for name, (xc, table, j0, band) in packed.items():
+            full = np.asarray(table, dtype=np.float32)[np.asarray(xc).astype(np.intp)]
+            full[:, j0:j0 + band.shape[1]] = band
+            arrays[name] = full
+        self.set_coeffs(ch, **arrays)
+
     def set_cpml(self, axis, e_prof, h_prof, pec_lo, pec_hi):
         self.prof[axis] = (tuple(np.asarray(a).astype(self.fdt) for a in e_prof),
                            tuple(np.asarray(a).astype(self.fdt) for a in h_prof))

@@ -60,6 +60,23 @@ def legacy_stack(t_lip_um, n_lip, t_aq_um, n_aq, t_mu_um, n_mu, names=None):
     return out
 
 
+def stack_at_lambda(stack, lam_nm):
+    """The stack at another wavelength (reference planar_beads/fdtd2d_core.py:83-95,
+    planar_3d/fdtd3d_core.py:92-104): a layer that names a material takes that material's
+    real index at ``lam_nm`` and keeps its absorption; layers given by a number stay."""
+    if stack is None:
+        return None
+    from lens_sensor_amd.common.physics import n_at
+    out = []
+    for layer in stack:
+        layer = dict(layer)
+        if layer.get('mat'):
+            n = complex(n_at(layer['mat'], lam_nm), complex(layer['n']).imag)
+            layer['n'], layer['kappa'] = n, n.imag
+        out.append(layer)
+    return out
+
+
 def stack_total_um(stack):
     return float(sum(L['t_um'] for L in stack or []))
 

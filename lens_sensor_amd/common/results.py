@@ -3,8 +3,8 @@
 The file names and the NPZ / PKL keys are the reference's (save_results,
 planar_beads/fdtd2d_core.py:2055-2240; save_results_3d, planar_3d/fdtd3d_core.py:2218-2423),
 so that its analyzer and viewers (fdtd_analyzer.py:420-586) read what the drivers of
-this package write.  CAD geometry and pulse spectra are not produced by the device path,
-so their optional keys are never written.
+this package write.  CAD geometry is not produced by the device path, so its optional
+keys are never written; the Fourier pulse stitch adds its spectrum keys.
 
 2D:  <out>/beads_<scenario>_<method>.pkl         every scalar / small entry of the result
      <out>/beads_<scenario>_<method>_frames.npz   Ez frames (x stride keeps <= 5000 columns,
@@ -49,6 +49,23 @@ def _view_flags(result, cw_only, has_cw):
     return dict(has_transient=_arr1(0 if cw_only else 1), has_cw=_arr1(1 if has_cw else 0),
                 cw_only=_arr1(1 if cw_only else 0),
                 view_primary=_arr1(str(result.get('view_primary', 'transient'))))
+
+
+def _spectrum_npz(result, with_tmax):
+    """Fourier pulse stitch: the spectrum T(lambda_k), the comb weights and the pulse
+    parameters (reference fdtd2d_core.py:2229-2236, fdtd3d_core.py:2301-2308)."""
+    sp = result.get('spectrum')
+    if not sp:
+        return {}
+    pu = result.get('pulse') or {}
+    out = dict(spec_lam_nm=np.asarray(sp['lam_nm'], dtype=np.float64),
+               spec_T=np.asarray(sp['T'], dtype=np.float64),
+               spec_weight=np.asarray(sp['weight'], dtype=np.float64),
+               pulse_cycles=_arr1(pu.get('cycles', 0.0)))
+    if with_tmax:
+        out['pulse_tmax_fs'] = _arr1(pu.get('tmax_fs', 0.0))
+    out['pulse_dispersion'] = _arr1(int(bool(pu.get('dispersion', True))))
+    return out
 
 
 def _first_stride(n, limit, choices):
@@ -162,6 +179,7 @@ def save_results(result, out_dir='results', defaults=None):
               lam_nm=_arr1(result.get('lambda_nm', 850.0)))
     if result.get('bead_y_um') is not None:
         kw['bead_y_um'] = _arr1(result['bead_y_um'])
+    kw.update(_spectrum_npz(result, with_tmax=True))
     npz = f'{out_dir}/beads_{tag}_frames.npz'
     np.savez_compressed(npz, **kw, **cw_kw, **_view_flags(result, cw_only, cw_only or bool(cw_kw)),
                         **extra)
@@ -219,6 +237,7 @@ def save_results_3d(result, out_dir='results', prefix='fdtd3d', material_names=N
                                                     for k in range(ncw)], dtype=np.float64))
         cw_v = dict(Ez_cw=cw_vol)
     flags = _view_flags(result, cw_only, cw_only or cw_vol is not None)
+    flags.update(_spectrum_npz(result, with_tmax=False))      # into both NPZ files
     bead = result.get('bead') or {}
     bd = bead.get('d_um', 0.0)
     bead_kw = dict(bead_x_um=_arr1(bead.get('x_um', 0.0)), bead_diameter_um=_arr1(bd),

@@ -15,6 +15,8 @@ Grid (C order, x slowest, z contiguous):
   Ex (Nx-1,Ny,Nz)  Ey (Nx,Ny-1,Nz)  Ez (Nx,Ny,Nz-1)
   Hx (Nx,Ny-1,Nz-1) Hy (Nx-1,Ny,Nz-1) Hz (Nx-1,Ny-1,Nz)
 """
+import contextlib
+import io
 import os
 import sys
 import time
@@ -32,7 +34,8 @@ from lens_sensor_amd.common.subpixel import sphere_fraction, blend, cell_average
 from lens_sensor_amd.common.flux import PlaneFluxBook, FluxAccumulator, guard_in_plane  # noqa: E402
 from lens_sensor_amd.common import vcsel                                      # noqa: E402
 from lens_sensor_amd.common.layer_stack import (stack_total_um, legacy_stack,  # noqa: E402
-                                                legacy_layers_tuple, describe as describe_stack)
+                                                legacy_layers_tuple, stack_at_lambda,
+                                                describe as describe_stack)
 from lens_sensor_amd.engine.spec import new_steps, fill_phasors, cw_envelope, pulse_envelope  # noqa: E402
 from lens_sensor_amd.planar_3d.boundaries3d import (make_boundary_3d, ALL_FACES,  # noqa: E402
                                                     mur_speeds_3d, Sponge3D)
@@ -916,6 +919,124 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
                 Efull=(Efull if return_efull else None), transmission_intensity=Tr_I,
                 transmission_def=tr_def, n_windows=n_win, cells_per_window=Nx_win * Ny * Nz,
                 _stream_paths=[], cad=None)
+
+
+def _index_at(spec, lam_nm):
+    """One entry of a dispersion map at lam_nm: a material name, 'cornea_layered' (the
+    default cornea) or (material, offset)."""
+    from lens_sensor_amd.common.physics import n_at, cornea_default
+    if spec == 'cornea_layered':
+        return cornea_default(lam_nm)
+    if isinstance(spec, (tuple, list)):
+        return n_at(spec[0], lam_nm) + float(spec[1])
+    return n_at(spec, lam_nm)
+
+
+def run_3d_stitched_pulse(label, wg_n, window_w_um=20.0, slide_um=12.0, pulse_cycles=30.0,
+                          pulse_tmax_fs=None, pulse_nfreq=None, pulse_dispersion=True,
+                          disp_map=None, t_eval_s=None, cad=None, stack=None, **kw):
+    """Transient 3D stitch by Fourier synthesis (reference fdtd3d_core.py:2087-2215).
+
+    One CW run_3d_stitched per line of the pulse comb
+    (lens_sensor_amd.common.fourier_pulse.PulseComb) on the driven component; the frames
+    E(t_j) = (1/pi) Re sum_k i E_k S_k e^{i w_k t_j} dw are summed frequency by frequency
+    (complex64, x blocks of 64 planes) into a memory-mapped volume array.  ``disp_map``
+    {'wg_n' | 'bead' | run_3d_stitched keyword: material, 'cornea_layered' or
+    (material, offset)} sets those indices from n(lambda_k) when ``pulse_dispersion`` is
+    on; material layers of ``stack`` follow n(lambda_k) as well.  The result also holds
+    the CW volume at the centre line (8 phases), the time-averaged |E|^2 of the frames
+    and the spectrum T(lambda_k); ``transmission`` is the |S|^2-weighted mean."""
+    from lens_sensor_amd.common.fourier_pulse import PulseComb, add_frequency
+    if cad is not None:
+        raise NotImplementedError('CAD geometry import is outside the device path')
+    t_total = time.time()
+    lam_nm = kw.pop('lam_nm', 850.0)
+    lx = kw.get('lx_um', 40.0)
+    comb = PulseComb(lam_nm, pulse_cycles, lx * 1e-6 * wg_n * 1.15 / C0,
+                     tmax_fs=pulse_tmax_fs, nfreq=pulse_nfreq)
+    if comb.aliased:
+        print(f'  [WARN] {pulse_nfreq} frequencies -> repetition period '
+              f'{2 * np.pi / comb.dw * 1e15:.0f} fs < t_max {comb.t_max * 1e15:.0f} fs: '
+              'temporal aliasing.')
+    n_snap = int(kw.pop('n_snapshots', 8))
+    t_j = (np.asarray(t_eval_s, dtype=np.float64) if t_eval_s is not None
+           else np.linspace(0.0, comb.t_max, max(n_snap, 8)))
+    print(f'\n===== 3D-Fourier-Stitch (transient): {label} =====')
+    print(f'Pulse {pulse_cycles:g} periods (s={comb.sp * 1e15:.1f} fs), band '
+          f'{comb.lam_nm.min():.1f}..{comb.lam_nm.max():.1f} nm, {len(comb)} frequencies, '
+          f't_max={comb.t_max * 1e15:.0f} fs, {len(t_j)} frames, '
+          f'dispersion {"on" if pulse_dispersion else "off"}')
+    if stack is not None:
+        print(f'Layers: {describe_stack(stack)}'
+              + ('  (material layers follow n(lambda_k))' if pulse_dispersion else ''))
+    disp = dict(disp_map or {}) if pulse_dispersion else {}
+    fr = fr_path = None
+    spec_T = []
+    r_c = E_c = None
+    amp = np.abs(comb.S).max()
+    os.makedirs('results', exist_ok=True)
+    safe = ''.join(c if (c.isalnum() or c in '-_.') else '_' for c in str(label))
+    for k, (wk, lk, Sk) in enumerate(zip(comb.w, comb.lam_nm, comb.S)):
+        t1 = time.time()
+        args, wg_k = dict(kw), wg_n
+        for par, mat in disp.items():
+            if par == 'wg_n':
+                wg_k = _index_at(mat, lk)
+            elif par == 'bead':
+                if args.get('bead'):
+                    args['bead'] = dict(args['bead'], n=_index_at(mat, lk))
+            else:
+                args[par] = _index_at(mat, lk)
+        stk = stack_at_lambda(stack, float(lk)) if (stack is not None and pulse_dispersion) else stack
+        with contextlib.redirect_stdout(io.StringIO()):
+            r = run_3d_stitched(f'{label}_f{k}', wg_k, window_w_um=window_w_um,
+                                slide_um=slide_um, lam_nm=float(lk), n_snapshots=1,
+                                return_efull=True, stack=stk, **args)
+        E = r.pop('Efull')
+        if fr is None:
+            fr_path = os.path.join('results', f'_stream3d_{safe}_fourier_ez.npy')
+            fr = np.lib.format.open_memmap(fr_path, mode='w+', dtype=np.float32,
+                                           shape=(len(t_j),) + E.shape)
+            fr[:] = 0.0
+        add_frequency(fr, E, Sk * comb.dw / np.pi, wk, t_j, wide=False)
+        spec_T.append(float(r['transmission']))
+        r['vols_array'] = r['Iavg'] = None
+        if k == comb.centre:
+            r_c, E_c = r, E
+        del E
+        print(f'  [{k + 1}/{len(comb)}] lambda={lk:.2f} nm  |S|={abs(Sk) / amp:.3f}  '
+              f'T={r["transmission"]:.4f}  ({time.time() - t1:.0f}s)', flush=True)
+    fr.flush()
+    # the steady state at the centre line (8 phases) and the frames' mean |E|^2
+    nfr_cw = 8
+    cw_path = os.path.join('results', f'_stream3d_{safe}_fourier_cwvol.npy')
+    cw_vol = np.lib.format.open_memmap(cw_path, mode='w+', dtype=np.float32,
+                                       shape=(nfr_cw,) + E_c.shape)
+    for q in range(nfr_cw):
+        cw_vol[q] = np.real(E_c * np.complex64(np.exp(2j * np.pi * q / nfr_cw))).astype(np.float32)
+    cw_vol.flush()
+    ia_path = os.path.join('results', f'_stream3d_{safe}_fourier_Iavg.npy')
+    Iavg = np.lib.format.open_memmap(ia_path, mode='w+', dtype=np.float32, shape=E_c.shape)
+    for x0 in range(0, E_c.shape[0], 64):
+        Iavg[x0:x0 + 64] = np.mean(np.asarray(fr[:, x0:x0 + 64], dtype=np.float32) ** 2, axis=0)
+    Iavg.flush()
+    res = dict(r_c)
+    dt_c = r_c['dt']
+    res.update(scenario=label, vols_array=fr, vols_ex=None, vols_ey=None, vector=False,
+               _stream_paths=[fr_path, cw_path, ia_path],
+               snap_steps=[int(round(t / dt_c)) for t in t_j], snap_times=[float(t) for t in t_j],
+               Iavg=Iavg, cw_vol=cw_vol, view_primary='transient', cw_only=False,
+               source_type='pulse', lam_nm=lam_nm, total_time=time.time() - t_total,
+               transmission=comb.weighted(spec_T),
+               spectrum=dict(lam_nm=comb.lam_nm, T=np.array(spec_T), weight=np.abs(comb.S) / amp),
+               pulse=dict(cycles=pulse_cycles, tmax_fs=comb.t_max * 1e15, nfreq=len(comb),
+                          dispersion=bool(pulse_dispersion)))
+    if stack is not None:                     # the stack at lambda0
+        res.update(stack=[dict(layer) for layer in stack], stack_custom=True,
+                   layers=legacy_layers_tuple(stack))
+    print(f'\n[Result] {label} [3D-Fourier-Stitch]: {len(comb)} frequencies in '
+          f'{res["total_time"]:.0f}s  T(weighted)={res["transmission"]:.4f}')
+    return res
 
 
 def save_results_3d(result, out_dir='results', prefix='fdtd3d', material_names=None):

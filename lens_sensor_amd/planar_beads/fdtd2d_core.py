@@ -12,6 +12,8 @@ TE = polarisation 's' (Ez, Hx, Hy); TM = 'p' (Hz, Ex, Ey).
 The time loop is executed by the step engine (HIP library); per step the host only
 fills table entries (envelope, phasors) with the reference's expressions.
 """
+import contextlib
+import io
 import os
 import sys
 import time
@@ -29,7 +31,8 @@ from lens_sensor_amd.common.subpixel import circle_fraction, blend, cell_average
 from lens_sensor_amd.common.flux import PlaneFluxBook, FluxAccumulator, guard_in_plane  # noqa: E402
 from lens_sensor_amd.common import vcsel                                      # noqa: E402
 from lens_sensor_amd.common.layer_stack import (stack_total_um, legacy_stack,  # noqa: E402
-                                                legacy_layers_tuple, describe as describe_stack)
+                                                legacy_layers_tuple, stack_at_lambda,
+                                                describe as describe_stack)
 from lens_sensor_amd.engine.spec import new_steps, fill_phasors, cw_envelope, pulse_envelope  # noqa: E402
 from lens_sensor_amd.planar_beads.boundaries import (make_boundary, MurSet, mur_speeds,  # noqa: E402
                                                     attach_mur_2d)
@@ -505,8 +508,9 @@ def run_beads(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n_snaps
         raise NotImplementedError('CAD geometry import is outside the device path')
     if method == 'sliding':
         raise NotImplementedError("method 'sliding' is not part of the device path")
-    if str(source_type).lower() == 'pulse' and method == 'stitch':
-        raise NotImplementedError('pulse stitching is not part of the device path')
+    if str(source_type).lower() == 'pulse' and method == 'stitch' and pulse_mode == 'td':
+        raise NotImplementedError("pulse_mode 'td' (time-domain TF/SF stitch) is not part of "
+                                  "the device path; pulse_mode 'fourier' is")
     npml_v = NPML if npml is None else int(npml)
     if stack is not None and tear_buffer_um is None:
         need = stack_total_um(stack) + 1.5 + npml_v * dx_nm * 1e-3
@@ -539,6 +543,18 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
                     tear_buffer_um=None, stack=None, npml=None, engine_mode=None, beads=None,
                     steps_override=None):
     global LAM_LOSS
+    if method == 'stitch' and str(source_type).lower() == 'pulse':
+        return run_beads_stitched_pulse(
+            wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots, wg_thickness_um,
+            lambda_nm, vcsel_waist, vcsel_tilt, vcsel_offset, window_w_um, slide_um, place_bead,
+            t_aqueous_um=t_aqueous_um, t_mucin_um=t_mucin_um, t_lipid_um=t_lipid_um,
+            input_gap_um=input_gap_um, length_um=length_um, bead_x_um=bead_x_um,
+            bead_y_um=bead_y_um, wg_n=wg_n, bead_n=bead_n, n_aqueous=n_aqueous, n_mucin=n_mucin,
+            n_cornea=n_cornea, n_lipid=n_lipid, polarization=polarization, curved=curved,
+            boundary=boundary, cad=cad, pulse_cycles=pulse_cycles, pulse_tmax_fs=pulse_tmax_fs,
+            pulse_nfreq=pulse_nfreq, pulse_dispersion=pulse_dispersion,
+            n_aq_tear_offset=n_aq_tear_offset, stack=stack, npml=npml, engine_mode=engine_mode,
+            beads=beads)
     if method == 'stitch':
         return run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots,
                                   wg_thickness_um, lambda_nm, vcsel_waist, vcsel_tilt,
@@ -1011,6 +1027,141 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
     if ExI_full is not None:
         res['Iavg'] = ExI_full
         res['Ex_cw_amp'] = ExA_full
+    return res
+
+
+# ----------------------------------------------------------------------------- pulse stitch
+_FIXED_INDEX = (('wg_n', None), ('bead_n', None), ('n_aqueous', 'aqueous'),
+                ('n_mucin', 'mucin'), ('n_cornea', None), ('n_lipid', 'lipid'))
+
+
+def run_beads_stitched_pulse(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots,
+                             wg_thickness_um, lambda_nm, vcsel_waist, vcsel_tilt, vcsel_offset,
+                             window_w_um, slide_um, place_bead, pulse_cycles=30.0,
+                             pulse_tmax_fs=None, pulse_nfreq=None, pulse_dispersion=True,
+                             cad=None, t_eval_s=None, n_aq_tear_offset=None, stack=None, **kw):
+    """Transient stitch by Fourier synthesis (reference fdtd2d_core.py:1866-2017).
+
+    The Gaussian source pulse (the full-domain pulse source) is split into the comb of
+    lens_sensor_amd.common.fourier_pulse.PulseComb; every comb line is one CW stitch
+    (run_beads_stitched) at lambda_k, with the materials at lambda_k unless
+    ``pulse_dispersion`` is off (then every index stays at lambda0).  The transient
+    frames E(t_j) = (1/pi) Re sum_k i E_k S_k e^{i w_k t_j} dw are summed frequency by
+    frequency into a memory-mapped array; the CW frames at the centre line and the
+    spectrum T(lambda_k) come with the result, and ``transmission`` is the |S|^2-weighted
+    mean.  Keywords run_beads_stitched takes (polarization, boundary, npml, engine_mode,
+    beads, ...) pass through."""
+    from lens_sensor_amd.common.fourier_pulse import PulseComb, add_frequency
+    if cad is not None:
+        raise NotImplementedError('CAD geometry import is outside the device path')
+    t_total = time.time()
+    pol = 'p' if str(kw.get('polarization', 's')).lower() in ('p', 'tm') else 's'
+    L_um = kw.get('length_um') or L_DEMO * 1e6
+    n_core = kw.get('wg_n') or n_at(wg_mat, lambda_nm)
+    y0_um, y1_um = -TEAR_BUFFER * 1e6, (wg_thickness_um * 1e-6 + AIR_BUFFER) * 1e6
+    # the pulse crosses the structure at a group index about 15 % above n_core
+    comb = PulseComb(lambda_nm, pulse_cycles, L_um * 1e-6 * n_core * 1.15 / C0,
+                     tmax_fs=pulse_tmax_fs, nfreq=pulse_nfreq)
+    if comb.aliased:
+        print(f'  [WARN] {pulse_nfreq} frequencies -> repetition period '
+              f'{2 * np.pi / comb.dw * 1e15:.0f} fs < t_max {comb.t_max * 1e15:.0f} fs: '
+              'temporal aliasing.')
+    fixed = {}
+    if not pulse_dispersion:
+        for key, mat in _FIXED_INDEX:
+            if kw.get(key):
+                fixed[key] = kw[key]
+            elif key == 'wg_n':
+                fixed[key] = n_at(wg_mat, lambda_nm)
+            elif key == 'bead_n':
+                fixed[key] = n_at(bead_mat, lambda_nm)
+            elif key == 'n_cornea':
+                fixed[key] = cornea_default(lambda_nm)
+            else:
+                fixed[key] = n_at(mat, lambda_nm)
+    base = {k: v for k, v in kw.items() if k not in fixed}
+    tear_follows = pulse_dispersion and n_aq_tear_offset is not None
+    if t_eval_s is not None:
+        t_j = np.asarray(t_eval_s, dtype=np.float64)
+    else:
+        t_j = np.linspace(0.0, comb.t_max, max(int(n_snapshots), 24))
+    label = f'WG-{wg_mat}_Bead-{bead_mat}_d{bead_d_um:g}_L{lambda_nm:g}' + ('' if place_bead else '_ref')
+    print(f'\n===== Fourier-Stitch (transient): {label} =====')
+    print(f'Pulse: {pulse_cycles:g} periods (s={comb.sp * 1e15:.1f} fs), band '
+          f'{comb.lam_nm.min():.1f}..{comb.lam_nm.max():.1f} nm, {len(comb)} frequencies, '
+          f'dw/2pi={comb.dw / 2 / np.pi / 1e12:.2f} THz, t_max={comb.t_max * 1e15:.0f} fs, '
+          f'{len(t_j)} time frames, dispersion {"on" if pulse_dispersion else "off (n at lambda0)"}')
+    if stack is not None:
+        print(f'Layers: {describe_stack(stack)}'
+              + ('  (material layers follow n(lambda_k))' if pulse_dispersion else ''))
+    frames_t = mm_path = None
+    spec_T, spec_P = [], []
+    r_c = E_c = None
+    amp = np.abs(comb.S).max()
+    for k, (wk, lk, Sk) in enumerate(zip(comb.w, comb.lam_nm, comb.S)):
+        t1 = time.time()
+        args = dict(base)
+        if tear_follows:
+            args['n_aqueous'] = n_at('aqueous', float(lk)) + float(n_aq_tear_offset)
+        stk = stack_at_lambda(stack, float(lk)) if (stack is not None and pulse_dispersion) else stack
+        with contextlib.redirect_stdout(io.StringIO()):
+            r = run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, False, 4,
+                                   wg_thickness_um, float(lk), vcsel_waist, vcsel_tilt,
+                                   vcsel_offset, 'cw', window_w_um, slide_um, place_bead,
+                                   return_efull=True, stack=stk, **fixed, **args)
+        E = np.asarray(r.pop('Efull'))
+        if pol == 'p':                       # Ey (Nx, Ny-1) onto the node grid (Nx, Ny)
+            En = np.empty((E.shape[0], E.shape[1] + 1), dtype=np.complex64)
+            En[:, 1:-1] = 0.5 * (E[:, :-1] + E[:, 1:])
+            En[:, 0], En[:, -1] = E[:, 0], E[:, -1]
+            E = En
+        if frames_t is None:
+            os.makedirs('results', exist_ok=True)
+            mm_path = os.path.join('results', f'_stream_beads_{_safe(label)}_pulse_{os.getpid()}.npy')
+            frames_t = np.lib.format.open_memmap(mm_path, mode='w+', dtype=np.float32,
+                                                 shape=(len(t_j),) + E.shape)
+            frames_t[:] = 0.0
+        add_frequency(frames_t, E, Sk * comb.dw / np.pi, wk, t_j, wide=True)
+        spec_T.append(float(r['transmission']))
+        spec_P.append(float(r['P_out']))
+        if k == comb.centre:
+            r_c, E_c = r, E
+        print(f'  [{k + 1}/{len(comb)}] lambda={lk:.2f} nm  |S|={abs(Sk) / amp:.3f}  '
+              f'T={r["transmission"]:.4f}  ({time.time() - t1:.0f}s)', flush=True)
+    frames_t.flush()
+    del frames_t
+    frames = [{'x_start': 0.0, 'x_end': L_um, 'y_start': y0_um, 'y_end': y1_um, 'slide_i': 0,
+               'step': j, 'time': float(t)} for j, t in enumerate(t_j)]
+    # the steady state at the centre line, the analyzer's second view
+    wc = comb.w[comb.centre]
+    nfr_cw = max(1, n_snapshots)
+    cw_path = os.path.join('results', f'_stream_beads_{_safe(label)}_pulse_cw_{os.getpid()}.npy')
+    cw_mm = np.lib.format.open_memmap(cw_path, mode='w+', dtype=np.float32,
+                                      shape=(nfr_cw,) + E_c.shape)
+    cw_frames = []
+    for q in range(nfr_cw):
+        tq = (q / nfr_cw) * (2 * np.pi / wc)
+        cw_mm[q] = np.real(E_c * np.exp(1j * wc * tq)).astype(np.float32)
+        cw_frames.append({'x_start': 0.0, 'x_end': L_um, 'y_start': y0_um, 'y_end': y1_um,
+                          'slide_i': 0, 'step': q, 'time': tq})
+    cw_mm.flush()
+    del cw_mm
+    res = dict(r_c)
+    res.update(scenario=label, method='stitch-pulse', frames=frames, frames_memmap=mm_path,
+               cw_frames=cw_frames, cw_frames_memmap=cw_path, view_primary='transient',
+               cw_only=False, Iavg=None, source_type='pulse', lambda_nm=lambda_nm,
+               total_time=time.time() - t_total,
+               spectrum=dict(lam_nm=comb.lam_nm, T=np.array(spec_T), P_out=np.array(spec_P),
+                             weight=np.abs(comb.S) / amp),
+               pulse=dict(cycles=pulse_cycles, s_fs=comb.sp * 1e15, tmax_fs=comb.t_max * 1e15,
+                          nfreq=len(comb), dispersion=bool(pulse_dispersion)))
+    if stack is not None:                    # the stack at lambda0
+        res['stack'] = [dict(layer) for layer in stack]
+        res['stack_custom'] = True
+        res['layers'] = legacy_layers_tuple(stack)
+    res['transmission'] = comb.weighted(spec_T)
+    print(f'\n[Result] {label} [Fourier-Stitch]: {len(comb)} frequencies in '
+          f'{res["total_time"]:.0f}s  T(weighted)={res["transmission"]:.4f}')
     return res
 
 

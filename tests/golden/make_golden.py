@@ -70,6 +70,14 @@ CASES_2D = {
 # :1422-1469)
 for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch', 'tm_stitch_lossy'):
     CASES_2D[_n + '_mur1'] = dict(CASES_2D[_n], boundary='mur1')
+# the Fourier pulse stitch (fdtd2d_core.py:1866-2017): three comb lines of a 10-period
+# pulse; TE with every index fixed at lambda0, lossy TM with a layer stack whose material
+# layers follow n(lambda_k)
+PULSE = dict(source_type='pulse', pulse_cycles=10.0, pulse_nfreq=3)
+STACK_PULSE = [dict(name='Aqueous', t_um=1.0, n=complex(1.329, 0.001), kappa=0.001, mat='aqueous'),
+               dict(name='Mucin', t_um=0.5, n=complex(1.34, 0.0), kappa=0.0, mat=None)]
+CASES_2D['te_stitch_pulse'] = dict(CASES_2D['te_stitch'], pulse_dispersion=False, **PULSE)
+CASES_2D['tm_stitch_pulse_stack'] = dict(CASES_2D['tm_stitch_lossy'], stack=STACK_PULSE, **PULSE)
 
 CASES_3D = {
     'te_full_char': dict(fn='run_3d', label='golden3d', wg_n=1.491, lx_um=4.0, air_um=1.0,
@@ -95,6 +103,11 @@ CASES_3D = {
 for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch_char'):
     for _b in ('sponge', 'mur1'):
         CASES_3D[f'{_n}_{_b}'] = dict(CASES_3D[_n], boundary=_b)
+# the 3D Fourier pulse stitch (fdtd3d_core.py:2087-2215): three comb lines, the core,
+# bead, aqueous and cornea indices following n(lambda_k)
+CASES_3D['te_stitch_char_pulse'] = dict(
+    CASES_3D['te_stitch_char'], fn='run_3d_stitched_pulse', pulse_cycles=10.0, pulse_nfreq=3,
+    disp_map={'wg_n': 'pmma', 'bead': 'polystyrol', 'n_aq': 'aqueous', 'n_co': 'cornea_layered'})
 
 
 def run_2d(d2, name, case):
@@ -114,6 +127,12 @@ def run_2d(d2, name, case):
         out['Iavg'] = np.asarray(r['Iavg'], dtype=np.float32)
     if r.get('Ex_cw_amp') is not None:
         out['Ex_cw_amp'] = np.asarray(r['Ex_cw_amp'], dtype=np.complex64)
+    if r.get('spectrum') is not None:
+        out['spectrum_T'] = np.asarray(r['spectrum']['T'], dtype=np.float64)
+        out['spectrum_lam'] = np.asarray(r['spectrum']['lam_nm'], dtype=np.float64)
+    cwm = r.get('cw_frames_memmap')
+    if cwm and os.path.exists(cwm):
+        out['cw_frames'] = np.array(np.load(cwm, mmap_mode='r'), dtype=np.float32)
     for k in ('cw_rel_change', 'steps_total', 'p_in_si', 'coupling_efficiency',
               'facet_reflection', 'transmission_intensity'):
         if r.get(k) is not None:
@@ -133,6 +152,11 @@ def run_3d(f3d, name, case):
     out = dict(transmission=r['transmission'], P_in=r['P_in'], P_out=r['P_out'],
                Iavg=np.asarray(r['Iavg'], dtype=np.float32),
                vols=np.asarray(r['vols_array'], dtype=np.float32))
+    if r.get('spectrum') is not None:
+        out['spectrum_T'] = np.asarray(r['spectrum']['T'], dtype=np.float64)
+        out['spectrum_lam'] = np.asarray(r['spectrum']['lam_nm'], dtype=np.float64)
+    if r.get('cw_vol') is not None:
+        out['cw_vol'] = np.asarray(r['cw_vol'], dtype=np.float32)
     for k in ('cw_rel_change', 'steps_total', 'p_in_si', 'coupling_efficiency',
               'facet_reflection', 'transmission_intensity'):
         if r.get(k) is not None:

@@ -40,6 +40,9 @@ KW_2D = dict(wg_mat='pmma', bead_mat='polystyrol', bead_d_um=0.5, dx_nm=100.0, s
 CASES_2D = {
     '2d_te_full': dict(KW_2D),
     '2d_tm_stitch': dict(KW_2D, method='stitch', polarization='p', length_um=9.0, bead_x_um=5.5),
+    # the Fourier pulse stitch (run_beads -> run_beads_stitched_pulse)
+    '2d_te_stitch_pulse': dict(KW_2D, method='stitch', source_type='pulse', length_um=9.0,
+                               bead_x_um=5.0, pulse_cycles=10.0, pulse_nfreq=3),
 }
 # the keyword set of run_simulation._run_3d for run_3d (run_simulation.py:360-429)
 KW_3D = dict(label='Bead-polystyrol_d0.5_full', wg_n=1.491, t_wg_um=1.0, t_lip_um=0.0,
@@ -51,7 +54,19 @@ KW_3D = dict(label='Bead-polystyrol_d0.5_full', wg_n=1.491, t_wg_um=1.0, t_lip_u
              input_gap_um=0.0, polarization='s', stack=None, steps_factor=1.0, calibrate_steps=0,
              save_vector=False, check_resources=False, allow_large=False, boundary='cpml',
              pec_faces=[], end_facet_um=0.0, cad=None, pulse_cycles=30.0)
-CASES_3D = {'3d_te_full': dict(KW_3D)}
+# run_simulation._run_3d's call of run_3d_stitched_pulse (run_simulation.py:365-413): the
+# keywords common to the 3D drivers without source_type, plus the pulse and window ones
+_COMMON_3D = ('label', 'wg_n', 't_wg_um', 't_lip_um', 't_aq_um', 't_mu_um', 'n_lip', 'n_aq',
+              'n_mu', 'n_co', 'lam_nm', 'lx_um', 'air_um', 'tear_um', 'lz_um', 'dx_nm',
+              'vcsel_waist_um', 'vcsel_waist_z_um', 'n_snapshots', 'bead', 'vol_dtype',
+              'wg_width_um', 'n_clad_side', 'vcsel_tilt_deg', 'vcsel_offset_y_um',
+              'vcsel_offset_z_um', 'input_gap_um', 'polarization', 'stack')
+KW_3D_PULSE = dict({k: KW_3D[k] for k in _COMMON_3D}, label='Bead-polystyrol_d0.5_stitch',
+                   window_w_um=2.0, slide_um=1.2, pulse_cycles=10.0, pulse_tmax_fs=None,
+                   pulse_nfreq=3, pulse_dispersion=True, cad=None, boundary='cpml',
+                   disp_map={'wg_n': 'pmma', 'n_mu': 'mucin', 'n_co': 'cornea_layered',
+                             'n_lip': 'lipid', 'bead': 'polystyrol', 'n_aq': 'aqueous'})
+CASES_3D = {'3d_te_full': dict(KW_3D), '3d_te_stitch_pulse': dict(KW_3D_PULSE)}
 
 
 def _import_reference():
@@ -92,6 +107,9 @@ def main():
                     r['layer_names'] = NAMES
                     r['n_lip'] = 1.48
                     d2.save_results(r, out_dir=res_dir)
+                elif 'pulse_nfreq' in kw:
+                    r = f3d.run_3d_stitched_pulse(**kw)
+                    f3d.save_results_3d(r, out_dir=res_dir, prefix='planar3d', material_names=NAMES)
                 else:
                     k = dict(kw)
                     k['pec_faces'] = tuple(k['pec_faces'])

@@ -45,6 +45,14 @@ CASES_2D = {
 # :1422-1469)
 for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch', 'tm_stitch_lossy'):
     CASES_2D[_n + '_mur1'] = dict(CASES_2D[_n], boundary='mur1')
+# the Fourier pulse stitch (fdtd2d_core.py:1866-2017): three comb lines of a 10-period
+# pulse; TE with every index fixed at lambda0, lossy TM with a layer stack whose material
+# layers follow n(lambda_k)
+PULSE = dict(source_type='pulse', pulse_cycles=10.0, pulse_nfreq=3)
+STACK_PULSE = [dict(name='Aqueous', t_um=1.0, n=complex(1.329, 0.001), kappa=0.001, mat='aqueous'),
+               dict(name='Mucin', t_um=0.5, n=complex(1.34, 0.0), kappa=0.0, mat=None)]
+CASES_2D['te_stitch_pulse'] = dict(CASES_2D['te_stitch'], pulse_dispersion=False, **PULSE)
+CASES_2D['tm_stitch_pulse_stack'] = dict(CASES_2D['tm_stitch_lossy'], stack=STACK_PULSE, **PULSE)
 
 CASES_3D = {
     'te_full_char': dict(fn='run_3d', label='golden3d', wg_n=1.491, lx_um=4.0, air_um=1.0,
@@ -70,6 +78,11 @@ CASES_3D = {
 for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch_char'):
     for _b in ('sponge', 'mur1'):
         CASES_3D[f'{_n}_{_b}'] = dict(CASES_3D[_n], boundary=_b)
+# the 3D Fourier pulse stitch (fdtd3d_core.py:2087-2215): three comb lines, the core,
+# bead, aqueous and cornea indices following n(lambda_k)
+CASES_3D['te_stitch_char_pulse'] = dict(
+    CASES_3D['te_stitch_char'], fn='run_3d_stitched_pulse', pulse_cycles=10.0, pulse_nfreq=3,
+    disp_map={'wg_n': 'pmma', 'bead': 'polystyrol', 'n_aq': 'aqueous', 'n_co': 'cornea_layered'})
 
 
 def load_fixture(dim, name):
@@ -109,9 +122,19 @@ def run_case(dim, name, engine_factory, quiet=True, **extra):
             res['Iavg'] = np.asarray(r['Iavg'], dtype=np.float32)
         if r.get('Ex_cw_amp') is not None:
             res['Ex_cw_amp'] = np.asarray(r['Ex_cw_amp'], dtype=np.complex64)
+        if r.get('spectrum') is not None:
+            res['spectrum_T'] = np.asarray(r['spectrum']['T'], dtype=np.float64)
+            res['spectrum_lam'] = np.asarray(r['spectrum']['lam_nm'], dtype=np.float64)
+        if r.get('cw_frames_memmap'):
+            res['cw_frames'] = np.array(np.load(r['cw_frames_memmap']), dtype=np.float32)
     else:
         res['Iavg'] = np.asarray(r['Iavg'], dtype=np.float32)
         res['vols'] = np.asarray(r['vols_array'], dtype=np.float32)
+        if r.get('spectrum') is not None:
+            res['spectrum_T'] = np.asarray(r['spectrum']['T'], dtype=np.float64)
+            res['spectrum_lam'] = np.asarray(r['spectrum']['lam_nm'], dtype=np.float64)
+        if r.get('cw_vol') is not None:
+            res['cw_vol'] = np.asarray(r['cw_vol'], dtype=np.float32)
     for k in ('steps_total', 'cw_rel_change'):
         if r.get(k) is not None:
             res[k] = float(r[k])

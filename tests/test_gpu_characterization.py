@@ -1,8 +1,9 @@
 """The solver drivers on the HIP engine (MI355X) reproduce the reference's outputs.
 
-Same seven cases and fixtures as tests/test_characterization.py (written by the
-reference itself, tests/golden/make_golden.py), now with every time step executed by the
-gfx950 kernels in the reference-faithful 'ref32' mode.  Tolerance: the device computes
+Same cases and fixtures as tests/test_characterization.py (written by the reference
+itself, tests/golden/make_golden.py: full domain, stitch, absorbers and the Fourier pulse
+stitch), now with every time step executed by the gfx950 kernels in the
+reference-faithful 'ref32' mode.  Tolerance: the device computes
 the float32 source phase with sinf instead of NumPy's float32 sin (<= 2 ulp), every
 other float32 operation is identical; over a few hundred steps this stays below
 1e-5 relative on scalars and relative L2 on arrays (the reference's own tolerance is

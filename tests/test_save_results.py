@@ -46,6 +46,9 @@ def test_result_files_have_the_reference_keys(case, in_tmp):
             r['layer_names'] = GOLD['names']
             r['n_lip'] = 1.48
             d2.save_results(r, out_dir=out_dir)
+        elif 'pulse_nfreq' in kw:                     # the 3D Fourier pulse stitch
+            r = f3d.run_3d_stitched_pulse(**kw)
+            f3d.save_results_3d(r, out_dir=out_dir, prefix='planar3d', material_names=GOLD['names'])
         else:
             kw['pec_faces'] = tuple(kw['pec_faces'])
             r = f3d.run_3d(**kw)

@@ -13,6 +13,11 @@ all faces, 4-layer stack + lipid, one bead, Gaussian CW source, 8000 steps, FP64
 sums on one monitor plane; the detector boxes of every step are summed inside the
 temporally blocked passes (depth 3), the steps of the final CW window run one per launch.
 
+--workload pulse2d: the two pulse modes of the 2D stitch on a lossy TM geometry (40 um
+long, dx = 20 nm, 30 um windows every 18 um, 30-period pulse): the Fourier stitch (one CW
+stitch per comb line) and the time-domain TF/SF stitch; value = cell updates / second of
+the two whole solves (set-up included).
+
 Prints ONE JSON line: metric, value, roofline of the dominant kernel (the blocked pass;
 frac from the counter-measured HBM bytes of the pass), CPU baseline of the NumPy oracle
 on a bounded sample of the same workload.  The default run adds, in the same line:
@@ -265,6 +270,38 @@ def bench_serve(args):
     return recs, summ, wall, cells
 
 
+PULSE_CASE = dict(wg_mat='pmma', bead_mat='polystyrol', bead_d_um=1.0, dx_nm=20.0,
+                  save_frames=False, n_snapshots=24, wg_thickness_um=5.0, lambda_nm=850.0,
+                  method='stitch', source_type='pulse', pulse_cycles=30.0, length_um=40.0,
+                  bead_x_um=20.0, window_w_um=30.0, slide_um=18.0, polarization='p',
+                  engine_mode='fp64acc')
+
+
+def bench_pulse(args):
+    """The two pulse modes of the 2D stitch on one lossy TM geometry (40 um long, dx 20 nm,
+    30 um windows every 18 um, 30-period pulse at 850 nm): the Fourier stitch (one CW
+    stitch per comb line) and the time-domain TF/SF stitch.  Wall time of each whole
+    solve, set-up included."""
+    from lens_sensor_amd.planar_beads import fdtd2d_core as d2
+    out = {}
+    d2.KAPPA = {'aq': 0.001, 'bead': 0.002}
+    try:
+        for mode in ('fourier', 'td'):
+            print(f'[pulse] {mode}', file=sys.stderr, flush=True)
+            t0 = time.time()
+            r = d2.run_beads(pulse_mode=mode, **PULSE_CASE)
+            wall = time.time() - t0
+            out[mode] = dict(wall_s=wall, cell_updates=int(r['cell_updates']),
+                             mcups=r['cell_updates'] / wall / 1e6,
+                             transmission=float(r['transmission']),
+                             lines=(r['pulse'].get('nfreq') if mode == 'fourier' else None),
+                             windows=[dict(steps=w['steps'], cells=w['cells'], s=round(w['seconds'], 2))
+                                      for w in r.get('windows') or []] if mode == 'td' else None)
+    finally:
+        d2.KAPPA = {}
+    return out
+
+
 def cpu_baseline(workload, seconds=12.0):
     """NumPy oracle (one core) on a bounded sample: the same grid and coefficients, as many
     steps as fit in ~``seconds``; Mcell-updates/s."""
@@ -339,7 +376,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=16,
                     help='tm2d_stitch: steps of one untimed warm-up window (0: none)')
     ap.add_argument('--workload', default='tm2d_stitch',
-                    choices=('tm2d_stitch', 'yee3d', 'sweep', 'serve'))
+                    choices=('tm2d_stitch', 'yee3d', 'sweep', 'serve', 'pulse2d'))
     ap.add_argument('--requests', type=int, default=24, help='serve: requests in the stream')
     ap.add_argument('--windows', type=int, default=None, help='tm2d_stitch: windows to run')
     ap.add_argument('--steps-override', type=int, default=None,
@@ -377,6 +414,19 @@ def main():
                    per_request=[dict(id=r['id'], kind=r['kind'], dims=list(r['dims']), steps=r['steps'],
                                      latency_s=round(r['latency'], 3), converged=bool(r['converged']))
                                 for r in recs])
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == 'pulse2d':
+        r = bench_pulse(args)
+        tot = sum(v['cell_updates'] for v in r.values())
+        wall = sum(v['wall_s'] for v in r.values())
+        out = dict(metric='Mcell-updates/s', value=tot / wall / 1e6, unit='Mcell-updates/s',
+                   n_gpus=1, steps=None, warmup=0, ms_per_step=None, higher_is_better=True,
+                   scaling='weak', vs_baseline=None, dtype='fp32', data='synthetic',
+                   config=dict(workload='pulse2d_tm_40um_fourier_and_td', dx_nm=20.0,
+                               window_um=30.0, slide_um=18.0, pulse_cycles=30.0,
+                               fields='fp32', phasors='fp64'),
+                   device=dev, fourier=r['fourier'], td=r['td'])
         print(json.dumps(out), flush=True)
         return
     if args.workload == 'sweep':

@@ -39,7 +39,7 @@ typedef struct lf_step {
     float qr32, qi32;            /* float32(cos w t), -float32(sin w t)           */
     float dr32, di32;            /* complex64(exp(+i w t))                        */
     unsigned int mask;           /* bit m set: monitor m accumulates this step    */
-    unsigned int pad;
+    unsigned int row;            /* row of the plane recorders and TF/SF series   */
 } lf_step;
 
 enum { LF_TE2D = 0, LF_TM2D = 1, LF_YEE3D = 2 };
@@ -121,6 +121,34 @@ int lf_monitor_zero(lf_ctx *ctx, int id);
  * id_a (A) and id_b (B) of the same shape and format; the reference reads both phasor
  * windows to compute norm(a - b) / norm(a). */
 int lf_monitor_change(lf_ctx *ctx, int id_a, int id_b, double da, double db, double *out);
+
+/* ---- Time-domain TF/SF stitch (run_beads_stitched_td, fdtd2d_core.py:1586-1863;
+ * run_3d_stitched_td, fdtd3d_core.py:1785-2084).  A context with TF/SF corrections or
+ * plane recorders advances one step per launch.
+ * lf_add_box_form: lf_add_box with a 2D form, 0 = the node-grid I_E of lf_add_box, 1 (2D
+ *   TM) = the node Ey^2 alone; replaces the detector sums of fdtd2d_core.py:1811-1815.
+ * lf_add_sx: FP64 sum over the masked steps of the instantaneous net flux in +x through x
+ *   plane il (H averaged onto the E plane); replaces _sx_plane_2d (fdtd2d_core.py:489-497)
+ *   and _sx_plane_3d (fdtd3d_core.py:762-767) as summed at fdtd2d_core.py:1804-1818 and
+ *   fdtd3d_core.py:1990-1999.
+ * lf_add_rec: x plane il of component comp copied into row lf_step.row of a rows x plane
+ *   buffer on the steps of its mask bit, phase 0 at the start of the step (E(n)), phase 1
+ *   after the H update and its TF/SF corrections (H(n+1/2)); replaces the rec_E / rec_H
+ *   writes of fdtd2d_core.py:1767-1776 and fdtd3d_core.py:1968-1977.
+ * lf_add_tfsf / lf_load_tfsf: F = F +- c inc on x plane il over [j0,j1) x [k0,k1), c = Ch
+ *   (coef = -1, H components, after the H update) or coefficient array coef at the cell (E
+ *   components, after the E update and source), inc = row lf_step.row of the series
+ *   loaded with lf_load_tfsf (nrows x plane floats from row 0); replaces the TF/SF lines
+ *   fdtd2d_core.py:1771, 1779, 1784, 1790 and fdtd3d_core.py:1974-1975, 1985-1986.
+ * lf_field_maxabs: max |F| of a component; replaces xp.max(xp.abs(...)) of the stop test
+ *   (fdtd2d_core.py:1829-1835, fdtd3d_core.py:2000-2004). */
+int lf_add_box_form(lf_ctx *ctx, const int *ranges18, int form, int *id);
+int lf_add_sx(lf_ctx *ctx, int il, int *id);
+int lf_add_rec(lf_ctx *ctx, int comp, int il, int phase, int rows, int *id);
+int lf_add_tfsf(lf_ctx *ctx, int comp, int il, int sign, int coef, int j0, int j1, int k0, int k1,
+                int rows, int *id);
+int lf_load_tfsf(lf_ctx *ctx, int id, const float *rows, int nrows);
+int lf_field_maxabs(lf_ctx *ctx, int comp, float *out);
 
 /* Advance nsteps steps; steps[s] holds the scalars of step s. */
 int lf_run(lf_ctx *ctx, const lf_step *steps, int nsteps);

@@ -1123,7 +1123,10 @@ __global__ void __launch_bounds__(256) box_kernel(MonArgs m, BoxArgs b) {
             const int i = (int)(rest / n1) + b.lo[cc][0];
             const int k = q + b.lo[cc][2];
             float v;
-            if (b.comp[cc] < 0) {
+            if (b.comp[cc] == -2) {                        // 2D TM: node Ey^2 only
+                const float e = node_ey(m, i, k);
+                v = e * e;
+            } else if (b.comp[cc] < 0) {
                 v = intensity2d(m, i, k);                  // 2D: I_E on the node grid
             } else {
                 const float e = comp_ptr(m, b.comp[cc])[m.g.at(i, j, k)];
@@ -1135,6 +1138,108 @@ __global__ void __launch_bounds__(256) box_kernel(MonArgs m, BoxArgs b) {
     double unused = 0.0;
     block_sum2_256(s, unused);                           // wavefront shuffles, fixed order
     if (threadIdx.x == 0) b.partial[blockIdx.x] += s;
+}
+
+// ---------------------------------------------------------------- time-domain stitch
+// (fdtd2d_core.py:1586-1863, fdtd3d_core.py:1785-2084): plane records, TF/SF corrections,
+// instantaneous Poynting sums, max |F|
+
+constexpr int kSxBlocks = 64;       // per-block FP64 partials of a Poynting-sum monitor
+constexpr int kMaxAbsBlocks = 512;  // per-block maxima of lf_field_maxabs
+
+// dst[j*n2 + k] = F[il, j, k]: one x plane of a component into one row of a recorder
+__global__ void __launch_bounds__(256) rec_plane_kernel(const float *__restrict__ f, Grid g, int il,
+                                                        int n1, int n2, float *__restrict__ dst) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n1 * n2) return;
+    const int j = q / n2, k = q - (q / n2) * n2;
+    dst[q] = f[g.at(il, j, k)];
+}
+
+// TF/SF correction on x plane il over [j0,j1) x [k0,k1): F = F + c inc or F = F - c inc,
+// c the scalar cs or the coefficient array carr at the cell; inc one row (n1 x n2) of the
+// incident series, laid out like the plane
+struct TfsfArgs {
+    int comp, il, sign, j0, j1, k0, k1, n1, n2, rows;
+    int coef;                // LF_* coefficient array id, or -1 for the context's Ch
+    float *series;           // rows x n1 x n2
+};
+
+__global__ void __launch_bounds__(256) tfsf_kernel(float *__restrict__ f, Grid g, TfsfArgs a,
+                                                   const float *__restrict__ carr, float cs,
+                                                   int row) {
+    const int w = a.k1 - a.k0;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= (a.j1 - a.j0) * w) return;
+    const int j = a.j0 + q / w, k = a.k0 + (q - (q / w) * w);
+    const long long o = g.at(a.il, j, k);
+    const float c = carr ? carr[o] : cs;
+    const float p = c * a.series[((long long)row * a.n1 + j) * a.n2 + k];
+    f[o] = a.sign > 0 ? f[o] + p : f[o] - p;
+}
+
+// Instantaneous net flux in +x through plane il, H averaged onto the E plane
+// (fdtd2d_core.py:489-497, fdtd3d_core.py:762-767): float32 products, FP64 sums, one
+// partial per block (fixed order), accumulated over the steps of the mask
+struct SxArgs {
+    int il;
+    double *partial;         // kSxBlocks
+};
+
+__global__ void __launch_bounds__(256) sx_kernel(MonArgs m, SxArgs a) {
+    const Grid &g = m.g;
+    double s = 0.0;
+    const int stride = gridDim.x * 256;
+    const int t0 = blockIdx.x * 256 + threadIdx.x;
+    if (m.kind == LF_TE2D) {                    // S_x = -Ez hy
+        for (int k = t0; k < g.nz; k += stride) {
+            const float hy = 0.5f * (m.Hy[g.at(a.il - 1, 0, k)] + m.Hy[g.at(a.il, 0, k)]);
+            s -= (double)(m.Ez[g.at(a.il, 0, k)] * hy);
+        }
+    } else if (m.kind == LF_TM2D) {             // S_x = Ey hz
+        for (int k = t0; k < g.nz - 1; k += stride) {
+            const float hz = 0.5f * (m.Hz[g.at(a.il - 1, 0, k)] + m.Hz[g.at(a.il, 0, k)]);
+            s += (double)(m.Ey[g.at(a.il, 0, k)] * hz);
+        }
+    } else {                                    // S_x = Ey hz - Ez hy
+        const int ny = g.ny, nz = g.nz;
+        double s2 = 0.0;
+        for (int q = t0; q < (ny - 1) * nz; q += stride) {
+            const int j = q / nz, k = q - (q / nz) * nz;
+            const float hz = 0.5f * (m.Hz[g.at(a.il - 1, j, k)] + m.Hz[g.at(a.il, j, k)]);
+            s += (double)(m.Ey[g.at(a.il, j, k)] * hz);
+        }
+        for (int q = t0; q < ny * (nz - 1); q += stride) {
+            const int j = q / (nz - 1), k = q - (q / (nz - 1)) * (nz - 1);
+            const float hy = 0.5f * (m.Hy[g.at(a.il - 1, j, k)] + m.Hy[g.at(a.il, j, k)]);
+            s2 += (double)(m.Ez[g.at(a.il, j, k)] * hy);
+        }
+        s -= s2;
+    }
+    double unused = 0.0;
+    block_sum2_256(s, unused);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] += s;
+}
+
+// per-block max |F| over a component's logical extent
+__global__ void __launch_bounds__(256) maxabs_kernel(const float *__restrict__ f, Grid g, int ex,
+                                                     int ey, int ez, float *__restrict__ part) {
+    __shared__ float wmax[4];
+    float v = 0.0f;
+    const long long tot = (long long)ex * ey * ez;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < tot;
+         t += (long long)gridDim.x * 256) {
+        const int k = (int)(t % ez);
+        const long long r = t / ez;
+        const int j = (int)(r % ey), i = (int)(r / ey);
+        v = fmaxf(v, fabsf(f[g.at(i, j, k)]));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
 }
 
 // Monitor launch shape: rows of 256 along the fast axis when the middle extent is 1
@@ -1279,13 +1384,15 @@ __global__ void __launch_bounds__(256) mur_apply_kernel(MurArgs a) {
 
 // ============================================================================ host context
 struct Monitor {
-    int type = -1;           // 0 box, 1 sq, 2 dft, 3 flux
+    int type = -1;           // 0 box, 1 sq, 2 dft, 3 flux, 4 Poynting sum, 5 plane record
     void *buf = nullptr;
     size_t bytes = 0;
     int form = 0, nxa = 0, n0 = 0, n1 = 0, n2 = 0;
     DftArgs dft{};
     FluxArgs flux{};
     BoxArgs box{};
+    SxArgs sx{};
+    int rcomp = 0, ril = 0, rphase = 0;  // plane record: component, x plane, phase
 };
 
 struct lf_ctx {
@@ -1360,6 +1467,11 @@ struct lf_ctx {
     const float *sp_g = nullptr;         // damping factors, index 0 = outermost cell
     std::vector<MurArgs> mur;            // Mur faces, in the order they apply
     std::vector<int> mur_comp;           // their field components (LF_E*)
+    // time-domain stitch: TF/SF corrections (each with its incident series on the device)
+    // and plane recorders; while either exists lf_run advances one step per launch
+    std::vector<TfsfArgs> tfsf;
+    int nrec = 0;
+    float *mx_part = nullptr;            // per-block maxima of lf_field_maxabs
     bool finalized = false;
     // timing
     bool timing = false;
@@ -1593,6 +1705,8 @@ int lf_destroy(lf_ctx *c) {
     if (c->tbm_stage) (void)hipFree(c->tbm_stage);
     if (c->nd_part) (void)hipFree(c->nd_part);
     if (c->nd_out) (void)hipFree(c->nd_out);
+    for (auto &t : c->tfsf) if (t.series) (void)hipFree(t.series);
+    if (c->mx_part) (void)hipFree(c->mx_part);
     for (int m = 0; m < c->nmon; ++m) {
         if (c->mon[m].buf) (void)hipFree(c->mon[m].buf);
         if (c->mon[m].type == 3)
@@ -2912,6 +3026,122 @@ int lf_add_box(lf_ctx *c, const int *r, int *id) {
     return 0;
 }
 
+/* lf_add_box with a 2D form: 0 the node-grid I_E of lf_add_box, 1 (2D TM) the node Ey^2
+ * alone (the time-domain stitch detectors, fdtd2d_core.py:1811-1815). */
+int lf_add_box_form(lf_ctx *c, const int *r, int form, int *id) {
+    if (int e = check_ctx(c)) return e;
+    if (form != 0 && (form != 1 || c->kind != LF_TM2D))
+        return fail(-1, "lf_add_box_form: form 1 is 2D TM only");
+    if (int e = lf_add_box(c, r, id)) return e;
+    if (form == 1) c->mon[*id].box.comp[0] = -2;
+    return 0;
+}
+
+/* Sum over the masked steps of the instantaneous net flux in +x through x plane il
+ * (fdtd2d_core.py:489-497, fdtd3d_core.py:762-767), FP64. */
+int lf_add_sx(lf_ctx *c, int il, int *id) {
+    if (int e = check_ctx(c)) return e;
+    if (il < 1 || il > c->nx - 2) return fail(-1, "lf_add_sx: plane index out of range");
+    Monitor *m;
+    if (int e = new_monitor(c, id, &m)) return e;
+    m->type = 4;
+    m->sx.il = il;
+    m->bytes = kSxBlocks * sizeof(double);
+    if (int e = alloc_zero(c, &m->buf, m->bytes)) return e;
+    m->sx.partial = (double *)m->buf;
+    return 0;
+}
+
+/* Plane recorder: x plane il of component comp into row `lf_step.row` of a rows x plane
+ * buffer on the steps of its mask bit; phase 0 records at the start of the step (E(n)),
+ * phase 1 after the H update and its TF/SF corrections (H(n+1/2)). */
+int lf_add_rec(lf_ctx *c, int comp, int il, int phase, int rows, int *id) {
+    if (int e = check_ctx(c)) return e;
+    if (comp < LF_EX || comp > LF_HZ) return fail(-1, "lf_add_rec: field component expected");
+    if (phase != 0 && phase != 1) return fail(-1, "lf_add_rec: phase 0 or 1");
+    if (rows < 1) return fail(-1, "lf_add_rec: rows >= 1");
+    int ex, ey, ez;
+    comp_extent(c, comp, ex, ey, ez);
+    if (il < 0 || il >= ex) return fail(-1, "lf_add_rec: plane index out of range");
+    Monitor *m;
+    if (int e = new_monitor(c, id, &m)) return e;
+    m->type = 5;
+    m->rcomp = comp; m->ril = il; m->rphase = phase;
+    m->n0 = rows; m->n1 = ey; m->n2 = ez;
+    m->bytes = (size_t)rows * ey * ez * sizeof(float);
+    ++c->nrec;
+    return alloc_zero(c, &m->buf, m->bytes);
+}
+
+/* TF/SF correction of x plane il of component comp (fdtd2d_core.py:1771-1789,
+ * fdtd3d_core.py:1973-1986): F = F + c inc (sign > 0) or F - c inc over [j0,j1) x [k0,k1)
+ * of the plane, c = the context's Ch (coef = -1) or a coefficient array (LF_CEH, LF_CAY,
+ * LF_CEY, LF_CEZ, ...) at the cell, inc = row `lf_step.row` of a rows x plane series
+ * loaded with lf_load_tfsf.  H components act after the H update, E components after the
+ * E update (and source) of every step.  Returns the correction's index in *id. */
+int lf_add_tfsf(lf_ctx *c, int comp, int il, int sign, int coef, int j0, int j1, int k0, int k1,
+                int rows, int *id) {
+    if (int e = check_ctx(c)) return e;
+    if (comp < LF_EX || comp > LF_HZ) return fail(-1, "lf_add_tfsf: field component expected");
+    if (rows < 1) return fail(-1, "lf_add_tfsf: rows >= 1");
+    int ex, ey, ez;
+    comp_extent(c, comp, ex, ey, ez);
+    if (il < 0 || il >= ex) return fail(-1, "lf_add_tfsf: plane index out of range");
+    if (j0 < 0 || j1 > ey || j0 >= j1 || k0 < 0 || k1 > ez || k0 >= k1)
+        return fail(-1, "lf_add_tfsf: range outside the plane");
+    const bool h = comp >= LF_HX;
+    if (coef >= 0) {
+        if (h || coef < LF_CEE || coef >= LF_NARRAYS || !c->arr[coef])
+            return fail(-1, "lf_add_tfsf: coefficient array not uploaded (E components only)");
+        int cx, cy, cz;
+        comp_extent(c, coef, cx, cy, cz);
+        if (il >= cx || j1 > cy || k1 > cz) return fail(-1, "lf_add_tfsf: coefficient array too small");
+    } else if (!h) {
+        return fail(-1, "lf_add_tfsf: E components take a coefficient array");
+    }
+    TfsfArgs a{};
+    a.comp = comp; a.il = il; a.sign = sign > 0 ? 1 : -1; a.coef = coef;
+    a.j0 = j0; a.j1 = j1; a.k0 = k0; a.k1 = k1; a.n1 = ey; a.n2 = ez; a.rows = rows;
+    LF_HIP(hipMalloc(&a.series, (size_t)rows * ey * ez * sizeof(float)));
+    LF_HIP(hipMemsetAsync(a.series, 0, (size_t)rows * ey * ez * sizeof(float), c->stream));
+    *id = (int)c->tfsf.size();
+    c->tfsf.push_back(a);
+    return 0;
+}
+
+/* Rows [0, nrows) of a TF/SF correction's incident series (nrows x plane floats). */
+int lf_load_tfsf(lf_ctx *c, int id, const float *rows, int nrows) {
+    if (int e = check_ctx(c)) return e;
+    if (id < 0 || id >= (int)c->tfsf.size()) return fail(-1, "lf_load_tfsf: bad id");
+    TfsfArgs &a = c->tfsf[id];
+    if (nrows < 0 || nrows > a.rows || (nrows > 0 && !rows)) return fail(-1, "lf_load_tfsf: bad row count");
+    if (nrows == 0) return 0;
+    LF_HIP(hipMemcpyAsync(a.series, rows, (size_t)nrows * a.n1 * a.n2 * sizeof(float),
+                          hipMemcpyHostToDevice, c->stream));
+    LF_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+/* max |F| over a field component (fdtd2d_core.py:1829-1835, fdtd3d_core.py:2000-2004). */
+int lf_field_maxabs(lf_ctx *c, int comp, float *out) {
+    if (int e = check_ctx(c)) return e;
+    if (comp < LF_EX || comp > LF_HZ || !c->arr[comp] || !out)
+        return fail(-1, "lf_field_maxabs: bad component");
+    if (!c->mx_part) LF_HIP(hipMalloc(&c->mx_part, kMaxAbsBlocks * sizeof(float)));
+    int ex, ey, ez;
+    comp_extent(c, comp, ex, ey, ez);
+    c->tb_chain = false;
+    maxabs_kernel<<<kMaxAbsBlocks, 256, 0, c->stream>>>(c->arr[comp], c->g, ex, ey, ez, c->mx_part);
+    LF_HIP(hipGetLastError());
+    float part[kMaxAbsBlocks];
+    LF_HIP(hipMemcpyAsync(part, c->mx_part, sizeof(part), hipMemcpyDeviceToHost, c->stream));
+    LF_HIP(hipStreamSynchronize(c->stream));
+    float v = 0.0f;
+    for (float p : part) v = std::max(v, p);
+    *out = v;
+    return 0;
+}
+
 int lf_add_sq(lf_ctx *c, int form, int nx_acc, int *id) {
     if (int e = check_ctx(c)) return e;
     Monitor *m;
@@ -3064,7 +3294,7 @@ int lf_monitor_write(lf_ctx *c, int id, const void *src, long long bytes) {
         }
         return 0;
     }
-    if (m.type == 0) {                          // box: the value goes to partial[0]
+    if (m.type == 0 || m.type == 4) {           // box / Poynting sum: the value goes to partial[0]
         std::vector<double> part(m.bytes / sizeof(double), 0.0);
         part[0] = *(const double *)src;
         LF_HIP(hipMemcpy(m.buf, part.data(), m.bytes, hipMemcpyHostToDevice));
@@ -3177,7 +3407,10 @@ int launch_monitors(lf_ctx *c, const MonArgs &ma, const lf_step &t, const StepAr
             const float fr = d.phasor == LF_PH_Q ? t.qr32 : t.pr32;
             const float fi = d.phasor == LF_PH_Q ? t.qi32 : t.pi32;
             dft_kernel<<<gq, mb, 0, c->stream>>>(ma, d, t.pr, t.pi, fr, fi);
-        } else {
+        } else if (m.type == 4) {
+            SxArgs a = m.sx;
+            sx_kernel<<<kSxBlocks, 256, 0, c->stream>>>(ma, a);
+        } else if (m.type == 3) {
             const FluxArgs &f = m.flux;
             const int n1 = f.n1[0] > (f.npair > 1 ? f.n1[1] : 0) ? f.n1[0] : f.n1[1];
             const int n2 = f.n2[0] > (f.npair > 1 ? f.n2[1] : 0) ? f.n2[0] : f.n2[1];
@@ -3186,6 +3419,41 @@ int launch_monitors(lf_ctx *c, const MonArgs &ma, const lf_step &t, const StepAr
                                                    t.pi32, t.hr32, t.hi32);
         }
     }
+    return 0;
+}
+
+// time-domain stitch: the plane recorders of one phase whose bit is set in the step's
+// mask, into row t.row
+int td_record(lf_ctx *c, const lf_step &t, int phase) {
+    for (int id = 0; id < c->nmon; ++id) {
+        const Monitor &m = c->mon[id];
+        if (m.type != 5 || m.rphase != phase || !((t.mask >> id) & 1u)) continue;
+        const int n = m.n1 * m.n2;
+        float *dst = (float *)m.buf + (size_t)t.row * n;
+        rec_plane_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(c->arr[m.rcomp], c->g, m.ril,
+                                                                m.n1, m.n2, dst);
+    }
+    return 0;
+}
+
+// the TF/SF corrections of the H (h = true) or the E components, row `row` of their series
+int td_tfsf(lf_ctx *c, int row, bool h) {
+    for (const TfsfArgs &a : c->tfsf) {
+        if ((a.comp >= LF_HX) != h) continue;
+        const int n = (a.j1 - a.j0) * (a.k1 - a.k0);
+        tfsf_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(
+            c->arr[a.comp], c->g, a, a.coef >= 0 ? c->arr[a.coef] : nullptr, c->ch, row);
+    }
+    return 0;
+}
+
+// rows the step's recorders and TF/SF series are addressed with must exist
+int td_check_row(const lf_ctx *c, const lf_step &t) {
+    for (const TfsfArgs &a : c->tfsf)
+        if (t.row >= (unsigned)a.rows) return fail(-1, "lf_run: step row outside a TF/SF series");
+    for (int id = 0; id < c->nmon; ++id)
+        if (c->mon[id].type == 5 && ((t.mask >> id) & 1u) && t.row >= (unsigned)c->mon[id].n0)
+            return fail(-1, "lf_run: step row outside a plane recorder");
     return 0;
 }
 
@@ -3203,7 +3471,11 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         return fail(-1, "lf_run: ce_x/ce_y/ce_z not uploaded");
     const int fp64 = c->mode == LF_MODE_FP64ACC;
     const bool absorbers = c->sp_n > 0 || !c->mur.empty();
-    if (absorbers) allow_tb = false;     // sponge / Mur act after every step
+    const bool td = !c->tfsf.empty() || c->nrec > 0;
+    if (absorbers || td) allow_tb = false;   // sponge / Mur / TF/SF / records act inside every step
+    if (td)
+        for (int s = 0; s < nsteps; ++s)
+            if (int e = td_check_row(c, steps[s])) return e;
     const Grid &g = c->g;
     dim3 blk(64, 4, 1);
     dim3 grd2((g.nz + 63) / 64, (g.nx + 3) / 4, 1);                    // 2D: (fast y, x)
@@ -3298,9 +3570,22 @@ int run_impl(lf_ctx *c, const lf_step *steps, int nsteps, bool allow_tb) {
         }
         c->tb_chain = false;
         if (timing) LF_HIP(hipEventRecord(c->ev[2 * ngroup], c->stream));
+        if (td) td_record(c, t, 0);
         if (!c->mur.empty())
             if (int e = absorb_capture(c)) return e;
-        if (c->kind == LF_TE2D) {
+        if (td) {
+            // H update, TF/SF on H, records of H(n+1/2), E update with source and PEC, TF/SF
+            // on E (fdtd2d_core.py:1768-1794, fdtd3d_core.py:1966-1990)
+            if (c->kind == LF_TE2D) te_h_kernel<<<grd2, blk, 0, c->stream>>>(te);
+            else if (c->kind == LF_TM2D) tm_h_kernel<<<grd2, blk, 0, c->stream>>>(tm);
+            else yee_h_kernel<<<grd3, blk, 0, c->stream>>>(yee);
+            td_tfsf(c, (int)t.row, true);
+            td_record(c, t, 1);
+            if (c->kind == LF_TE2D) te_e_kernel<<<grd2, blk, 0, c->stream>>>(te, c->ex, st, fp64);
+            else if (c->kind == LF_TM2D) tm_e_kernel<<<grd2, blk, 0, c->stream>>>(tm, c->ex, st, fp64);
+            else yee_e_kernel<<<grd3, blk, 0, c->stream>>>(yee, c->ex, st, fp64, 0);
+            td_tfsf(c, (int)t.row, false);
+        } else if (c->kind == LF_TE2D) {
             te_h_kernel<<<grd2, blk, 0, c->stream>>>(te);
             te_e_kernel<<<grd2, blk, 0, c->stream>>>(te, c->ex, st, fp64);
         } else if (c->kind == LF_TM2D && c->variant == 0) {
@@ -3714,6 +3999,8 @@ int lf_batch_add(lf_batch *b, lf_ctx *c) {
     if (c->kind == LF_YEE3D) return fail(-1, "lf_batch_add: 2D contexts only");
     if (c->sp_n > 0 || !c->mur.empty())
         return fail(-1, "lf_batch_add: contexts with Mur faces run alone (lf_run)");
+    if (!c->tfsf.empty() || c->nrec > 0)
+        return fail(-1, "lf_batch_add: contexts with TF/SF corrections or plane records run alone");
     if (c->mode != (b->ctx.empty() ? c->mode : b->ctx[0]->mode))
         return fail(-1, "lf_batch_add: all instances need the same precision mode");
     for (lf_ctx *o : b->ctx) if (o == c) return fail(-1, "lf_batch_add: context already in batch");

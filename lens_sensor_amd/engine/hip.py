@@ -36,7 +36,8 @@ EXPORTS = ('lf_create', 'lf_destroy', 'lf_last_error', 'lf_device_count', 'lf_de
            'lf_op_te_terminate', 'lf_op_tm_terminate', 'lf_op3d_create', 'lf_op3d_set_cpml',
            'lf_op3d_finalize', 'lf_op3d_step_H', 'lf_op3d_step_E', 'lf_op3d_destroy',
            'lf_monitor_change', 'lf_norm_diff', 'lf_norm_diff_noalloc', 'lf_set_sponge',
-           'lf_add_mur', 'lf_build_coeff')
+           'lf_add_mur', 'lf_build_coeff', 'lf_add_box_form', 'lf_add_sx', 'lf_add_rec',
+           'lf_add_tfsf', 'lf_load_tfsf', 'lf_field_maxabs')
 
 
 class OpProfiles2D(ctypes.Structure):
@@ -130,6 +131,12 @@ def load():
         'lf_set_sponge': [P, I, fp, ctypes.c_uint],
         'lf_add_mur': [P, I, I, I, fp, LL, I],
         'lf_build_coeff': [P, I, ctypes.POINTER(ctypes.c_ubyte), I, fp, I, I, fp],
+        'lf_add_box_form': [P, ip, I, ip],
+        'lf_add_sx': [P, I, ip],
+        'lf_add_rec': [P, I, I, I, I, ip],
+        'lf_add_tfsf': [P, I, I, I, I, I, I, I, I, I, ip],
+        'lf_load_tfsf': [P, I, fp, I],
+        'lf_field_maxabs': [P, I, fp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -306,7 +313,8 @@ class HipEngine:
         self.mon_kind.append(info)
         return mid.value
 
-    def add_box(self, box):
+    def add_box(self, box, form=0):
+        """form 1 (2D TM): the node Ey^2 alone instead of the node-grid I_E."""
         mid = ctypes.c_int()
         if self.kind == 'yee3d':
             r = []
@@ -317,8 +325,51 @@ class HipEngine:
             (a0, a1), (b0, b1) = resolve_box(self.dims, box)
             r = [a0, a1, b0, b1] + [0] * 14
         arr = (ctypes.c_int * 18)(*r)
-        return self._new_id(self._lib.lf_add_box(self.h, arr, ctypes.byref(mid)), mid,
+        rc = (self._lib.lf_add_box_form(self.h, arr, int(form), ctypes.byref(mid)) if form
+              else self._lib.lf_add_box(self.h, arr, ctypes.byref(mid)))
+        return self._new_id(rc, mid, dict(t='box'))
+
+    # -- time-domain stitch
+    def add_sx(self, il):
+        """FP64 sum over the masked steps of the instantaneous flux in +x through plane il."""
+        mid = ctypes.c_int()
+        return self._new_id(self._lib.lf_add_sx(self.h, int(il), ctypes.byref(mid)), mid,
                             dict(t='box'))
+
+    def add_rec(self, comp, il, phase, rows):
+        """Plane il of comp into row step['row'] of a (rows,) + plane buffer on the masked
+        steps; phase 0 at the start of the step, 1 after the H update and its TF/SF
+        corrections."""
+        mid = ctypes.c_int()
+        shp = (int(rows),) + comp_shape(self.kind, self.dims, comp)[1:]
+        rc = self._lib.lf_add_rec(self.h, ARR[comp], int(il), int(phase), int(rows),
+                                  ctypes.byref(mid))
+        return self._new_id(rc, mid, dict(t='rec', shape=shp))
+
+    def add_tfsf(self, comp, il, sign, coef, rng, rows):
+        """TF/SF correction F[il][rng] +-= c inc[row][rng] (c = Ch for coef None, else the
+        coefficient array coef at the cell); rng: one (lo, hi) per plane axis.  Returns its
+        index for load_tfsf."""
+        if self.kind == 'yee3d':
+            (j0, j1), (k0, k1) = rng
+        else:
+            (j0, j1), (k0, k1) = (0, 1), rng[0]
+        idx = ctypes.c_int()
+        _check(self._lib.lf_add_tfsf(self.h, ARR[comp], int(il), int(sign),
+                                     -1 if coef is None else ARR[coef], j0, j1, k0, k1, int(rows),
+                                     ctypes.byref(idx)), 'lf_add_tfsf')
+        self.absorbers = True
+        return idx.value
+
+    def load_tfsf(self, idx, rows):
+        """Rows 0..len(rows)-1 of a TF/SF correction's incident series."""
+        a = np.ascontiguousarray(rows, dtype=np.float32)
+        _check(self._lib.lf_load_tfsf(self.h, int(idx), _fptr(a), len(a)), 'lf_load_tfsf')
+
+    def field_maxabs(self, comp):
+        out = ctypes.c_float()
+        _check(self._lib.lf_field_maxabs(self.h, ARR[comp], ctypes.byref(out)), 'lf_field_maxabs')
+        return float(out.value)
 
     def add_sq(self, form, nx_acc=None):
         mid = ctypes.c_int()
@@ -370,7 +421,7 @@ class HipEngine:
         _check(self._lib.lf_monitor_read(self.h, mid, raw.ctypes.data, nb), 'lf_monitor_read')
         if info['t'] == 'box':
             return float(np.sum(raw.view(np.float64)))
-        if info['t'] == 'sq':
+        if info['t'] in ('sq', 'rec'):
             return raw.view(np.float32).reshape(info['shape']).copy()
         if info['t'] == 'dft':
             return raw.view(info['dtype']).reshape(info['shape']).copy()
@@ -399,7 +450,7 @@ class HipEngine:
                 parts.append(np.ascontiguousarray(aE, dtype=info['dtype']).view(np.uint8).ravel())
                 parts.append(np.ascontiguousarray(aH, dtype=info['dtype']).view(np.uint8).ravel())
             raw = np.concatenate(parts)
-        elif info['t'] == 'sq':
+        elif info['t'] in ('sq', 'rec'):
             raw = np.ascontiguousarray(value, dtype=np.float32).view(np.uint8).ravel()
         else:
             raw = np.ascontiguousarray(value, dtype=info['dtype']).view(np.uint8).ravel()

@@ -37,7 +37,7 @@ STEP_DTYPE = np.dtype([
     ('hr32', '<f4'), ('hi32', '<f4'),  # complex64(exp(-i w (t - dt/2)))
     ('qr32', '<f4'), ('qi32', '<f4'),  # (float32(cos w t), -float32(sin w t))
     ('dr32', '<f4'), ('di32', '<f4'),  # complex64(exp(+i w t))
-    ('mask', '<u4'), ('pad', '<u4'),
+    ('mask', '<u4'), ('row', '<u4'),   # monitor bits; row of the plane records / TF/SF series
 ], align=False)
 assert STEP_DTYPE.itemsize == 120
 
@@ -90,27 +90,22 @@ def fill_phasors(tab, omega, dt, t_phys, env, src_omega=None):
     tab['wt32'] = wt.astype(np.float32)                   # vcsel.py:155 np.float32(wt)
     tab['ssin'] = env * np.sin(wt)
     tab['scos'] = env * np.cos(wt)
-    for i, t in enumerate(t_phys.tolist()):
-        p = np.exp(-1j * omega * t)                       # fdtd2d_core.py:839
-        tab['pr'][i] = p.real
-        tab['pi'][i] = p.imag
-        p32 = np.complex64(p)                             # flux.py:32, stitch :1438
-        tab['pr32'][i] = p32.real
-        tab['pi32'][i] = p32.imag
-        h = np.exp(-1j * omega * (t - 0.5 * dt))          # flux.py:33
-        tab['hr'][i] = h.real
-        tab['hi'][i] = h.imag
-        h32 = np.complex64(h)
-        tab['hr32'][i] = h32.real
-        tab['hi32'][i] = h32.imag
-        d = np.exp(1j * omega * t)                        # stitch drive, fdtd2d_core.py:1426
-        tab['dr'][i] = d.real
-        tab['di'][i] = d.imag
-        d32 = np.complex64(d)                             # fdtd3d_core.py:1627
-        tab['dr32'][i] = d32.real
-        tab['di32'][i] = d32.imag
-        tab['qr32'][i] = np.float32(np.cos(omega * t))    # fdtd3d_core.py:1638
-        tab['qi32'][i] = -np.float32(np.sin(omega * t))   # fdtd3d_core.py:1639 (acc_im -= ...)
+    # whole-table NumPy expressions; element for element the same values as the
+    # reference's per-step scalar expressions (tests/test_step_table.py compares them)
+    p = np.exp(-1j * omega * t_phys)                      # fdtd2d_core.py:839
+    tab['pr'], tab['pi'] = p.real, p.imag
+    p32 = p.astype(np.complex64)                          # flux.py:32, stitch :1438
+    tab['pr32'], tab['pi32'] = p32.real, p32.imag
+    h = np.exp(-1j * omega * (t_phys - 0.5 * dt))         # flux.py:33
+    tab['hr'], tab['hi'] = h.real, h.imag
+    h32 = h.astype(np.complex64)
+    tab['hr32'], tab['hi32'] = h32.real, h32.imag
+    d = np.exp(1j * omega * t_phys)                       # stitch drive, fdtd2d_core.py:1426
+    tab['dr'], tab['di'] = d.real, d.imag
+    d32 = d.astype(np.complex64)                          # fdtd3d_core.py:1627
+    tab['dr32'], tab['di32'] = d32.real, d32.imag
+    tab['qr32'] = np.cos(omega * t_phys).astype(np.float32)       # fdtd3d_core.py:1638
+    tab['qi32'] = -np.sin(omega * t_phys).astype(np.float32)      # fdtd3d_core.py:1639 (acc_im -= ...)
     return tab
 
 

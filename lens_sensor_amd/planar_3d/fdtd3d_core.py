@@ -396,10 +396,14 @@ def _n_source_fn_3d(x_src_um, y0_um, dx_um, wg_n, t_wg_um, n_below, x_wg_start_u
     return fn
 
 
-def _vcsel_res(p_emit, p_refl, p_in, distance_um, air_um, x_facet_um, waist_um, waist_z_um):
+def _vcsel_res(p_emit, p_refl, p_in, distance_um, air_um, x_facet_um, waist_um, waist_z_um,
+               energy=False):
+    """VCSEL figures (reference fdtd3d_core.py:547-561); energy=True: pulse energies (J),
+    no source power stored."""
     r = dict(laser_power_mw=LASER_POWER_MW, vcsel_distance_um=float(distance_um or 0.0),
              vcsel_air_um=float(air_um), x_facet_um=float(x_facet_um),
-             vcsel_waist_um=float(waist_um), vcsel_waist_z_um=float(waist_z_um), p_src_si=p_emit)
+             vcsel_waist_um=float(waist_um), vcsel_waist_z_um=float(waist_z_um),
+             p_src_si=None if energy else p_emit)
     if p_emit and p_emit > 0:
         eta = (p_in / p_emit) if p_in is not None else float('nan')
         refl = (p_refl / p_emit) if p_refl is not None else float('nan')
@@ -919,6 +923,268 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
                 Efull=(Efull if return_efull else None), transmission_intensity=Tr_I,
                 transmission_def=tr_def, n_windows=n_win, cells_per_window=Nx_win * Ny * Nz,
                 _stream_paths=[], cad=None)
+
+
+def run_3d_stitched_td(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t_lip_um=0.0,
+                       t_aq_um=3.5, t_mu_um=0.5, n_lip=1.480, n_aq=1.336, n_mu=1.342,
+                       n_co=1.376, lam_nm=850.0, lx_um=40.0, air_um=3.0, tear_um=None,
+                       lz_um=8.0, dx_nm=50.0, vcsel_waist_um=3.0, vcsel_waist_z_um=3.0,
+                       source_type='pulse', n_snapshots=8, bead=None, n_sponge=None,
+                       sponge_um=1.2, sponge_alpha=0.3, det_in_um=None, det_out_um=None,
+                       vol_dtype='float32', wg_width_um=None, n_clad_side=N_AIR,
+                       vcsel_tilt_deg=0.0, vcsel_offset_y_um=0.0, vcsel_offset_z_um=0.0,
+                       input_gap_um=0.0, polarization='s', boundary='cpml', cad=None,
+                       pulse_cycles=30.0, pulse_tmax_fs=None, t_eval_s=None, stack=None,
+                       npml=None, engine_mode=None):
+    """Transient 3D stitch in the time domain with Huygens (TF/SF) planes (reference
+    fdtd3d_core.py:1785-2084), the 3D form of run_beads_stitched_td.
+
+    Window w records Ey, Ez(n) at the start of each step and Hy, Hz(n+1/2) after the H
+    update at the column of window w+1's TF/SF plane i0 = n_sp + 2 (device plane
+    recorders into memory-mapped series); window w+1 replays them: Hy[i0-1] -= Ch Ez_inc,
+    Hz[i0-1] += Ch Ey_inc after the H update, Ey[i0] += ce_y Hz_inc, Ez[i0] -= ce_z Hy_inc
+    after the E update.  CPML on all faces (boundary 'cpml', when the TF/SF plane lies
+    behind it), else the graded sponge on all faces.  Output: transient volume frames of
+    the driven component, <|E|^2> over the whole time axis, time-integrated core
+    intensity and Poynting flux at the detector planes, pulse energy around the source.
+    No dispersion."""
+    from lens_sensor_amd.common.td_stitch import TD_CHECK, TD_ROWS, td_next_event, signal_span
+    if cad is not None:
+        raise NotImplementedError('CAD geometry import is outside the device path')
+    t_total = time.time()
+    dx = dx_nm * 1e-9
+    dx_um = dx_nm / 1000.0
+    dt = 0.5 * dx / (C0 * np.sqrt(3))
+    global LAM_LOSS
+    LAM = lam_nm * 1e-9
+    LAM_LOSS = LAM
+    f0 = C0 / LAM
+    omega = 2 * np.pi * f0
+    npml = NPML if npml is None else int(npml)
+    bkind = str(boundary or 'cpml').lower()
+    tear_um = _resolve_tear_um(tear_um, t_lip_um, t_aq_um, t_mu_um,
+                               _abs_cells(bkind, npml, n_sponge, sponge_um, dx_um), dx_um,
+                               stack=stack)
+    ly_um = t_wg_um + air_um + tear_um
+    y0_um = -tear_um
+    z0_um = -lz_um / 2.0
+    Ny, Nz = int(round(ly_um / dx_um)), int(round(lz_um / dx_um))
+    Nx_full = int(round(lx_um / dx_um))
+    Nx_win = min(int(round(window_w_um / dx_um)), Nx_full)
+    S_cells = max(1, min(int(round(slide_um / dx_um)), Nx_win))
+    O_cells = Nx_win - S_cells
+    n_sp = _sponge_cells(n_sponge, sponge_um, dx_um, (Nx_win, Ny, Nz), warn=bkind == 'sponge')
+    i0 = n_sp + 2                                # TF/SF plane of the driven windows
+    if O_cells < i0 + 4:
+        raise ValueError(f'Overlap {O_cells} cells < {i0 + 4}: choose window_w >> slide.')
+    _vd = float(input_gap_um or 0.0)
+    x_src_um, x_wg_start_um, _air, _zsrc = _vcsel_geom_3d(n_sp, dx_um, _vd)
+    EXT = n_sp + 4
+    x_wg_end_um = lx_um + EXT * dx_um
+    n_win = (int(np.ceil((Nx_full - Nx_win) / S_cells)) + 1) if Nx_full > Nx_win else 1
+
+    def gx0(w):
+        return min(w * S_cells, max(Nx_full - Nx_win, 0))
+    sp = pulse_cycles / f0
+    tc = 3 * sp
+    t_max = (pulse_tmax_fs * 1e-15) if pulse_tmax_fs else (tc + 3 * sp + lx_um * 1e-6 * wg_n * 1.15 / C0)
+    N = int(np.ceil(t_max / dt)) + 1
+    t_j = (np.asarray(t_eval_s, dtype=np.float64) if t_eval_s is not None
+           else np.linspace(0.0, t_max, max(int(n_snapshots), 8)))
+    n_j = np.clip(np.round(t_j / dt).astype(np.int64), 0, N - 1)
+    frame_steps = np.unique(n_j)
+    pol, Nj, Nk = _resolve_pol(polarization, Ny, Nz)
+    src_comp = 'Ey' if pol == 'p' else 'Ez'
+    print(f'\n===== 3D time-domain stitch (Huygens/TF-SF): {label} =====')
+    print(f'Domain {lx_um:.1f}x{ly_um:.1f}x{lz_um:.1f}um, {n_win} windows of {window_w_um}um '
+          f'(overlap {O_cells * dx_um:.1f}um), grid/window {Nx_win}x{Ny}x{Nz}, Sponge {n_sp}, '
+          f'{N} steps, {len(n_j)} frames, engine {backend.engine_name()}')
+    src_ix, j_src, k_src, vs = _build_gauss_source(
+        n_sp, t_wg_um, y0_um, z0_um, dx_um, dx, LAM, Nj, Nk, vcsel_waist_um, vcsel_waist_z_um,
+        vcsel_offset_y_um, vcsel_offset_z_um, vcsel_tilt_deg,
+        n_src_fn=_n_source_fn_3d(x_src_um, y0_um, dx_um, wg_n, t_wg_um,
+                                 n_aq if stack is None else _stack_n_below(stack, n_co),
+                                 x_wg_start_um, pol),
+        pol=pol, wg_width_um=wg_width_um, dt=dt, z_prop_um=_zsrc)
+    env_t = np.exp(-((np.arange(N) * dt - tc) / sp) ** 2)
+    if det_in_um is None:
+        det_in_um = 0.2 * lx_um
+    if det_out_um is None:
+        det_out_um = 0.8 * lx_um
+    iin = int(round(det_in_um / dx_um))
+    iout = int(round(det_out_um / dx_um))
+    j_wg0 = int(round((0.0 - y0_um) / dx_um))
+    j_wg1 = int(round((t_wg_um - y0_um) / dx_um))
+    iin = guard_in_plane(iin, max(src_ix, int(round(x_wg_start_um / dx_um))), Nx_full)
+    vol_dt = np.float16 if vol_dtype == 'float16' else np.float32
+    os.makedirs('results', exist_ok=True)
+    safe = ''.join(c if (c.isalnum() or c in '-_.') else '_' for c in str(label))
+    paths = []
+    p = os.path.join('results', f'_stream3d_{safe}_td_ez.npy')
+    paths.append(p)
+    vol_buf = np.lib.format.open_memmap(p, mode='w+', dtype=vol_dt, shape=(len(n_j), Nx_full, Nj, Nk))
+    vol_buf[:] = 0
+    p = os.path.join('results', f'_stream3d_{safe}_td_Iavg.npy')
+    paths.append(p)
+    Iavg = np.lib.format.open_memmap(p, mode='w+', dtype=np.float32, shape=(Nx_full, Nj, Nk))
+    Iavg[:] = 0
+    m_I = max(1, int(round((1.0 / f0) / dt)) // 8)
+    n_I_glob = max(1, (N + m_I - 1) // m_I)        # mean over the whole time axis
+    Ch = np.float32(dt / (MU0 * dx))
+    plane = {'Ey': (Ny - 1, Nz), 'Ez': (Ny, Nz - 1), 'Hy': (Ny, Nz - 1), 'Hz': (Ny - 1, Nz)}
+    mode = _engine_mode(engine_mode)
+    rec_in = None
+    P = dict(P_in=0.0, P_out=0.0, S_in=0.0, S_out=0.0, S_up=0.0, S_dn=0.0)
+    for w in range(n_win):
+        t1 = time.time()
+        g0 = gx0(w)
+        last = w == n_win - 1
+        Nxw = Nx_win + (0 if last else EXT)
+        x0_um = g0 * dx_um
+        tp = {}
+        eps = build_eps3d(Nxw, Ny, Nz, dx_um, y0_um, z0_um, wg_n, t_wg_um, t_lip_um, t_aq_um,
+                          t_mu_um, n_lip, n_aq, n_mu, n_co, bead, wg_width_um=wg_width_um,
+                          n_clad_side=n_clad_side, x0_um=x0_um, x_wg_start_um=x_wg_start_um,
+                          x_wg_end_um=x_wg_end_um, tensor_patch=tp, stack=stack)
+        eps, eim = split_eps(eps)
+        inv, da = inv_loss(eps, eim, dt, dx, LAM)
+        del eps, eim
+        cex, cey, cez = ce_arrays(inv, tp, dt, dx, dx_um, da=da)
+        da = stagger_da(da)
+        del inv
+        eng = backend.make_engine('yee3d', (Nxw, Ny, Nz), mode)
+        eng.set_coeffs(Ch, cex=cex, cey=cey, cez=cez,
+                       dax=None if da is None else da[0], day=None if da is None else da[1],
+                       daz=None if da is None else da[2])
+        del cex, cey, cez
+        # CPML on all faces (the TF/SF plane behind it), else the sponge on all faces
+        # (fdtd3d_core.py:1958-1965)
+        use_pml = bkind == 'cpml' and i0 >= 11
+        if bkind == 'cpml' and not use_pml:
+            print('  [WARN] TF/SF plane too close to the boundary for CPML -> sponge')
+        if use_pml:
+            make_boundary_3d('cpml', dx, dt, ALL_FACES, shape=(Nxw, Ny, Nz), npml=npml,
+                             lam_m=LAM).attach(eng)
+        else:
+            make_boundary_3d('cpml', dx, dt, (), shape=(Nxw, Ny, Nz), npml=npml,
+                             lam_m=LAM).attach(eng)
+            Sponge3D(n_sp, sponge_alpha, ALL_FACES).attach_faces(eng)
+        eng.finalize()
+        i_rec = (gx0(w + 1) - g0 + i0) if not last else None
+        if w == 0:
+            eng.set_source(src_comp, src_ix, vs.amp, vs.ph1, vs.ph2)
+            n_start, n_in_last = 0, int(np.nonzero(env_t > 1e-7)[0][-1])
+        else:
+            # per step max|Ez_inc| + max|Ey_inc|, summed in float32 (fdtd3d_core.py:1951-1952)
+            amp = np.empty(N, dtype=np.float64)
+            for a in range(0, N, 1024):
+                b = min(a + 1024, N)
+                amp[a:b] = (np.abs(rec_in['Ez'][a:b]).max(axis=(1, 2))
+                            + np.abs(rec_in['Ey'][a:b]).max(axis=(1, 2)))
+            n_start, n_in_last = signal_span(amp, N)
+            tfsf = {'Ez': eng.add_tfsf('Hy', i0 - 1, -1, None, [(0, Ny), (0, Nz - 1)], TD_ROWS),
+                    'Ey': eng.add_tfsf('Hz', i0 - 1, +1, None, [(0, Ny - 1), (0, Nz)], TD_ROWS),
+                    'Hz': eng.add_tfsf('Ey', i0, +1, 'cey', [(0, Ny - 1), (1, Nz - 1)], TD_ROWS),
+                    'Hy': eng.add_tfsf('Ez', i0, -1, 'cez', [(1, Ny - 1), (0, Nz - 1)], TD_ROWS)}
+        rec, mrec = None, {}
+        if i_rec is not None:
+            rec = {}
+            for c, shp in plane.items():
+                pth = os.path.join('results', f'_stream3d_{safe}_td_rec{w}_{c}.npy')
+                rec[c] = np.lib.format.open_memmap(pth, mode='w+', dtype=np.float32, shape=(N,) + shp)
+                rec[c][:] = 0
+                paths.append(pth)
+                mrec[c] = eng.add_rec(c, i_rec if c[0] == 'E' else i_rec - 1,
+                                      0 if c[0] == 'E' else 1, TD_ROWS)
+        a_g = 0 if w == 0 else O_cells
+        b_g = (Nx_full - g0) if last else Nx_win
+        mon_I = eng.add_sq('3d_p' if pol == 'p' else '3d_s', Nx_win)
+        sums = {}
+        if w == 0:
+            sums['S_up'] = eng.add_sx(vs.plane_up)
+            sums['S_dn'] = eng.add_sx(vs.plane_dn)
+        for ixg, which in ((iin, 'in'), (iout, 'out')):
+            il = ixg - g0
+            if a_g <= il < b_g:
+                sums['P_' + which] = eng.add_box(((il, il + 1), (j_wg0, j_wg1), (n_sp, -n_sp)))
+                if il >= 1:
+                    sums['S_' + which] = eng.add_sx(il)
+        bits = sum(1 << m for m in list(sums.values()) + list(mrec.values()))
+        peak, n_stop, n = 0.0, N, n_start
+        while n < N:
+            e = min(td_next_event(n, n_in_last, frame_steps), N - 1, n + TD_ROWS - 1)
+            k = e - n + 1
+            ns = np.arange(n, e + 1)
+            tab = new_steps(k)
+            if w == 0:
+                env = env_t[n:e + 1]
+                fill_phasors(tab, omega, dt, ns * dt, np.where(env > 1e-9, env, 0.0),
+                             src_omega=vs.omega)
+            mk = np.full(k, bits, dtype=np.uint32)
+            mk[ns % m_I == 0] |= np.uint32(1 << mon_I)
+            tab['mask'] = mk
+            tab['row'] = np.arange(k)
+            if w > 0:
+                for c, tid in tfsf.items():
+                    eng.load_tfsf(tid, rec_in[c][n:e + 1])
+            eng.run(tab)
+            for c, mid in mrec.items():
+                rec[c][n:e + 1] = eng.read_monitor(mid)[:k]
+            hit = np.nonzero(n_j == e)[0]
+            if len(hit):
+                blk = eng.read_field(src_comp)[a_g:b_g].astype(vol_dt)
+                for jj in hit:
+                    vol_buf[jj, g0 + a_g:g0 + b_g] = blk
+            if e % TD_CHECK == 0 or (e > n_in_last and (e - n_in_last) % TD_CHECK == 0):
+                cur = eng.field_maxabs(src_comp)
+                peak = max(peak, cur)
+                if e > n_in_last and cur < 1e-4 * max(peak, 1e-30):
+                    n_stop = e
+                    break
+            n = e + 1
+        for name, mid in sums.items():
+            P[name] += eng.read_monitor(mid)
+        acc_I = eng.read_monitor(mon_I)
+        Iavg[g0 + a_g:g0 + b_g] = acc_I[a_g:b_g] * np.float32(1.0 / n_I_glob)
+        eng.close()
+        print(f'[Window {w + 1}/{n_win}] x0={x0_um:.1f}um  steps {n_start}..{n_stop} '
+              f'({n_stop - n_start} of {N})  {time.time() - t1:.0f}s', flush=True)
+        if rec_in is not None:                   # the previous window's series are used up
+            for c in list(rec_in):
+                pth = rec_in[c].filename
+                del rec_in[c]
+                try:
+                    os.remove(pth)
+                    paths.remove(pth)
+                except (OSError, ValueError):
+                    pass
+        rec_in = rec
+    vol_buf.flush()
+    Iavg.flush()
+    P_in, P_out = P['P_in'] * dx ** 3, P['P_out'] * dx ** 3
+    Tr_I = P_out / max(P_in, 1e-30)
+    S_in, S_out = P['S_in'], P['S_out']
+    Tr = S_out / S_in if abs(S_in) > 0 else Tr_I
+    tr_def = 'poynting' if abs(S_in) > 0 else 'intensity'
+    total_time = time.time() - t_total
+    print(f'\n[Result] {label} [3D time-domain stitch]: total {total_time:.0f}s  '
+          f'Transmission={Tr:.4f} ({tr_def})  [intensity: {Tr_I:.4f}]  ({n_win} windows)')
+    rs = _result_stack(stack, t_lip_um, t_aq_um, t_mu_um, n_lip, n_aq, n_mu)
+    ea = dx * dx * dt                              # area x time -> energy [J]
+    vres = _vcsel_res((P['S_dn'] - P['S_up']) * ea, -P['S_up'] * ea, S_in * ea, _vd, _air,
+                      x_wg_start_um, vcsel_waist_um, vcsel_waist_z_um, energy=True)
+    return dict(**vres, scenario=label, dim='3D', P_in=P_in, P_out=P_out, transmission=Tr,
+                vols_array=vol_buf, vols_ex=None, vols_ey=None, vector=False,
+                _stream_paths=paths, snap_steps=[int(k) for k in n_j],
+                snap_times=[float(k * dt) for k in n_j], Iavg=Iavg, total_time=total_time,
+                steps_total=N, dt=dt, dx_nm=dx_nm, lx_um=lx_um, ly_um=ly_um, lz_um=lz_um,
+                y0_um=y0_um, z0_um=z0_um, t_wg_um=t_wg_um, wg_n=wg_n, layers=rs[2],
+                stack=rs[0], stack_custom=rs[1], n_lip=n_lip, n_aq=n_aq, n_mu=n_mu, n_co=n_co,
+                lam_nm=lam_nm, source_type='pulse', bead=(bead if bead else {}),
+                det_in_um=iin * dx_um, det_out_um=iout * dx_um, x_wg_start_um=x_wg_start_um,
+                end_facet_um=0.0, x_src_um=x_src_um, polarization=pol,
+                view_primary='transient', cw_only=False, cad=None, cad_region=None,
+                transmission_intensity=Tr_I, transmission_def=tr_def)
 
 
 def _index_at(spec, lam_nm):

@@ -17,6 +17,7 @@ import io
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -30,6 +31,7 @@ from lens_sensor_amd.common.physics import (C0, EPS0, MU0, N_AIR, n_at,       # 
 from lens_sensor_amd.common.subpixel import circle_fraction, blend, cell_average_column  # noqa: E402
 from lens_sensor_amd.common.flux import PlaneFluxBook, FluxAccumulator, guard_in_plane  # noqa: E402
 from lens_sensor_amd.common import vcsel                                      # noqa: E402
+from lens_sensor_amd.common.td_stitch import TD_CHECK, TD_ROWS, td_next_event, signal_span  # noqa: E402
 from lens_sensor_amd.common.layer_stack import (stack_total_um, legacy_stack,  # noqa: E402
                                                 legacy_layers_tuple, stack_at_lambda,
                                                 describe as describe_stack)
@@ -467,15 +469,19 @@ def _emit_power(fl, dx):
     return (dn - up) * dx, -up * dx
 
 
-def _vcsel_result(res, p_emit, p_refl, distance_um, air_um, x_facet_um, vcsel_waist):
+def _vcsel_result(res, p_emit, p_refl, distance_um, air_um, x_facet_um, vcsel_waist,
+                  p_in=None, energy=False):
+    """VCSEL figures of a result (reference fdtd2d_core.py:415-435); energy=True: p_emit,
+    p_refl and p_in are pulse energies (J/m), so no source power is stored."""
     res['laser_power_mw'] = LASER_POWER_MW
     res['vcsel_distance_um'] = float(distance_um or 0.0)
     res['vcsel_air_um'] = float(air_um)
     res['vcsel_waist_um'] = float(vcsel_waist)
     res['x_facet_um'] = float(x_facet_um)
-    res['p_src_si'] = p_emit
+    res['p_src_si'] = None if energy else p_emit
     if p_emit and p_emit > 0:
-        p_in = res.get('p_in_si')
+        if p_in is None:
+            p_in = res.get('p_in_si')
         res['coupling_efficiency'] = (p_in / p_emit) if p_in is not None else float('nan')
         res['facet_reflection'] = (p_refl / p_emit) if p_refl is not None else float('nan')
 
@@ -508,9 +514,6 @@ def run_beads(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n_snaps
         raise NotImplementedError('CAD geometry import is outside the device path')
     if method == 'sliding':
         raise NotImplementedError("method 'sliding' is not part of the device path")
-    if str(source_type).lower() == 'pulse' and method == 'stitch' and pulse_mode == 'td':
-        raise NotImplementedError("pulse_mode 'td' (time-domain TF/SF stitch) is not part of "
-                                  "the device path; pulse_mode 'fourier' is")
     npml_v = NPML if npml is None else int(npml)
     if stack is not None and tear_buffer_um is None:
         need = stack_total_um(stack) + 1.5 + npml_v * dx_nm * 1e-3
@@ -543,6 +546,16 @@ def _run_beads_body(wg_mat, bead_mat, bead_d_um, dx_nm=20.0, save_frames=True, n
                     tear_buffer_um=None, stack=None, npml=None, engine_mode=None, beads=None,
                     steps_override=None):
     global LAM_LOSS
+    if method == 'stitch' and str(source_type).lower() == 'pulse' and pulse_mode == 'td':
+        return run_beads_stitched_td(
+            wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots, wg_thickness_um,
+            lambda_nm, vcsel_waist, vcsel_tilt, vcsel_offset, window_w_um, slide_um, place_bead,
+            pulse_cycles=pulse_cycles, pulse_tmax_fs=pulse_tmax_fs, cad=cad,
+            t_aqueous_um=t_aqueous_um, t_mucin_um=t_mucin_um, t_lipid_um=t_lipid_um,
+            input_gap_um=input_gap_um, length_um=length_um, bead_x_um=bead_x_um,
+            bead_y_um=bead_y_um, wg_n=wg_n, bead_n=bead_n, n_aqueous=n_aqueous, n_mucin=n_mucin,
+            n_cornea=n_cornea, n_lipid=n_lipid, polarization=polarization, curved=curved,
+            boundary=boundary, stack=stack, npml=npml, engine_mode=engine_mode, beads=beads)
     if method == 'stitch' and str(source_type).lower() == 'pulse':
         return run_beads_stitched_pulse(
             wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots, wg_thickness_um,
@@ -798,13 +811,14 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
                        n_mucin=None, n_cornea=None, n_lipid=None, polarization='s', curved=False,
                        boundary='cpml', cad=None, return_efull=False, stack=None, npml=None,
                        engine_mode=None, beads=None, steps_win_override=None, n_windows=None,
-                       window_hook=None):
+                       window_hook=None, tune_cache=None):
     """2D domain decomposition (reference fdtd2d_core.py:1197-1583): windows of
     ``window_w_um`` every ``slide_um``, each run to CW steady state; the previous window's
     complex field is imposed with a cosine taper over the overlap; inner seams end in a
     CPML extension that is not assembled.  ``steps_win_override`` / ``n_windows`` cap the
     work per window / the number of windows (benchmarks); ``window_hook(w, info)`` is
-    called after each window."""
+    called after each window; ``tune_cache`` (a dict) carries the blocking chosen for each
+    window shape across calls (the pulse comb's lines share their window shapes)."""
     if cad is not None:
         raise NotImplementedError('CAD geometry import is outside the device path')
     bkind = (boundary or 'cpml').lower()
@@ -886,7 +900,7 @@ def run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapsh
     fluxes = FluxAccumulator()
     mode = _engine_mode(engine_mode)
     win_info = []
-    tuned = {}                                   # blocking chosen for each window shape
+    tuned = {} if tune_cache is None else tune_cache   # blocking chosen for each window shape
     for w in range(n_win_run):
         g0 = gx0(w)
         x_start_um = g0 * dx * 1e6
@@ -1097,7 +1111,12 @@ def run_beads_stitched_pulse(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_
     frames_t = mm_path = None
     spec_T, spec_P = [], []
     r_c = E_c = None
+    cell_updates = 0
     amp = np.abs(comb.S).max()
+    tuned = {}                           # the lines share their window shapes
+    # the frame sums of line k run on a host thread while line k+1 steps on the device
+    pool = ThreadPoolExecutor(max_workers=1)
+    pending = None
     for k, (wk, lk, Sk) in enumerate(zip(comb.w, comb.lam_nm, comb.S)):
         t1 = time.time()
         args = dict(base)
@@ -1108,7 +1127,8 @@ def run_beads_stitched_pulse(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_
             r = run_beads_stitched(wg_mat, bead_mat, bead_d_um, dx_nm, False, 4,
                                    wg_thickness_um, float(lk), vcsel_waist, vcsel_tilt,
                                    vcsel_offset, 'cw', window_w_um, slide_um, place_bead,
-                                   return_efull=True, stack=stk, **fixed, **args)
+                                   return_efull=True, stack=stk, tune_cache=tuned, **fixed,
+                                   **args)
         E = np.asarray(r.pop('Efull'))
         if pol == 'p':                       # Ey (Nx, Ny-1) onto the node grid (Nx, Ny)
             En = np.empty((E.shape[0], E.shape[1] + 1), dtype=np.complex64)
@@ -1121,13 +1141,19 @@ def run_beads_stitched_pulse(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_
             frames_t = np.lib.format.open_memmap(mm_path, mode='w+', dtype=np.float32,
                                                  shape=(len(t_j),) + E.shape)
             frames_t[:] = 0.0
-        add_frequency(frames_t, E, Sk * comb.dw / np.pi, wk, t_j, wide=True)
+        if pending is not None:
+            pending.result()
+        pending = pool.submit(add_frequency, frames_t, E, Sk * comb.dw / np.pi, wk, t_j, True)
         spec_T.append(float(r['transmission']))
         spec_P.append(float(r['P_out']))
+        cell_updates += sum(wi['cells'] * wi['steps'] for wi in r.get('windows') or [])
         if k == comb.centre:
             r_c, E_c = r, E
         print(f'  [{k + 1}/{len(comb)}] lambda={lk:.2f} nm  |S|={abs(Sk) / amp:.3f}  '
               f'T={r["transmission"]:.4f}  ({time.time() - t1:.0f}s)', flush=True)
+    if pending is not None:
+        pending.result()
+    pool.shutdown()
     frames_t.flush()
     del frames_t
     frames = [{'x_start': 0.0, 'x_end': L_um, 'y_start': y0_um, 'y_end': y1_um, 'slide_i': 0,
@@ -1154,7 +1180,8 @@ def run_beads_stitched_pulse(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_
                spectrum=dict(lam_nm=comb.lam_nm, T=np.array(spec_T), P_out=np.array(spec_P),
                              weight=np.abs(comb.S) / amp),
                pulse=dict(cycles=pulse_cycles, s_fs=comb.sp * 1e15, tmax_fs=comb.t_max * 1e15,
-                          nfreq=len(comb), dispersion=bool(pulse_dispersion)))
+                          nfreq=len(comb), dispersion=bool(pulse_dispersion)),
+               cell_updates=cell_updates)
     if stack is not None:                    # the stack at lambda0
         res['stack'] = [dict(layer) for layer in stack]
         res['stack_custom'] = True
@@ -1162,6 +1189,236 @@ def run_beads_stitched_pulse(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_
     res['transmission'] = comb.weighted(spec_T)
     print(f'\n[Result] {label} [Fourier-Stitch]: {len(comb)} frequencies in '
           f'{res["total_time"]:.0f}s  T(weighted)={res["transmission"]:.4f}')
+    return res
+
+
+# ----------------------------------------------------------------------------- time-domain stitch
+def run_beads_stitched_td(wg_mat, bead_mat, bead_d_um, dx_nm, save_frames, n_snapshots,
+                          wg_thickness_um, lambda_nm, vcsel_waist, vcsel_tilt, vcsel_offset,
+                          window_w_um, slide_um, place_bead, pulse_cycles=30.0,
+                          pulse_tmax_fs=None, cad=None, t_eval_s=None, t_aqueous_um=None,
+                          t_mucin_um=None, t_lipid_um=0.0, input_gap_um=0.0, length_um=None,
+                          bead_x_um=None, bead_y_um=None, wg_n=None, bead_n=None,
+                          n_aqueous=None, n_mucin=None, n_cornea=None, n_lipid=None,
+                          polarization='s', curved=False, boundary='cpml', stack=None,
+                          npml=None, engine_mode=None, beads=None, **_ignored):
+    """Transient stitch in the time domain with Huygens (TF/SF) planes (reference
+    fdtd2d_core.py:1586-1863).
+
+    Every window runs on the same absolute time axis t = n dt, only while signal is
+    present (from the first incoming step to the stop test), with CPML on all edges.
+    Window w records, at the column where window w+1's TF/SF plane i0 = npml + 2 lies, E(n)
+    (TE Ez, TM Ey) at the start of each step and H(n+1/2) (TE Hy, TM Hz) after the H
+    update (device plane recorders, read back every TD_ROWS steps); window w+1 replays them
+    as the TF/SF boundary (H corrected after the H update, E after the E update).  Frames
+    at the steps nearest to t_j, time-integrated core intensity and Poynting flux at 10 % /
+    90 % of the length, the pulse energy around the source.  The materials are those at
+    lambda0 (no dispersion)."""
+    if cad is not None:
+        raise NotImplementedError('CAD geometry import is outside the device path')
+    global LAM_LOSS
+    pol = 'p' if str(polarization).lower() in ('p', 'tm') else 's'
+    kind = 'tm2d' if pol == 'p' else 'te2d'
+    t_total = time.time()
+    t_aq_um = t_aqueous_um if t_aqueous_um is not None else T_AQ * 1e6
+    wg_n = wg_n if wg_n is not None else n_at(wg_mat, lambda_nm)
+    bead_n = bead_n if bead_n is not None else n_at(bead_mat, lambda_nm)
+    n_aq_l = n_aqueous if n_aqueous is not None else n_at('aqueous', lambda_nm)
+    n_mu_l = n_mucin if n_mucin is not None else n_at('mucin', lambda_nm)
+    n_co_l = n_cornea if n_cornea is not None else cornea_default(lambda_nm)
+    n_lip_l = n_lipid if n_lipid is not None else n_at('lipid', lambda_nm)
+    L = (length_um * 1e-6) if length_um is not None else L_DEMO
+    L_um = L * 1e6
+    T_WG = wg_thickness_um * 1e-6
+    LAM = lambda_nm * 1e-9
+    LAM_LOSS = LAM
+    bx_um = bead_x_um if bead_x_um is not None else L_um / 2.0
+    label = f'WG-{wg_mat}_Bead-{bead_mat}_d{bead_d_um:g}_L{lambda_nm:g}' + ('' if place_bead else '_ref')
+    dx = dx_nm * 1e-9
+    dt = 0.5 * dx / (C0 * np.sqrt(2))
+    NP = NPML if npml is None else int(npml)
+    _vd = float(input_gap_um or 0.0)
+    x_facet_um, _air, _zsrc = _vcsel_geom_2d(NP + 2, dx, _vd)
+    input_gap_um = x_facet_um
+    y0_m = -TEAR_BUFFER
+    H_total = T_WG + AIR_BUFFER + TEAR_BUFFER
+    Ny = int(round(H_total / dx))
+    Nx_win = int(round(window_w_um * 1e-6 / dx))
+    S_cells = int(round(slide_um * 1e-6 / dx))
+    O_cells = Nx_win - S_cells
+    Nx_full = int(round(L / dx))
+    i0 = NP + 2                              # TF/SF column of the driven windows
+    if O_cells < i0 + 4:
+        raise ValueError(f'Overlap {O_cells} cells < {i0 + 4}: choose window_w >> slide.')
+    f0 = C0 / LAM
+    omega = 2 * np.pi * f0
+    sp = pulse_cycles / f0
+    tc = 3 * sp
+    t_max = (pulse_tmax_fs * 1e-15) if pulse_tmax_fs else (tc + 3 * sp + L * wg_n * 1.15 / C0)
+    N = int(np.ceil(t_max / dt)) + 1
+    n_win = (int(np.ceil((Nx_full - Nx_win) / S_cells)) + 1) if Nx_full > Nx_win else 1
+
+    def gx0(w):
+        return min(w * S_cells, max(Nx_full - Nx_win, 0))
+    EXT = NP + 4
+    len_mat = L_um + EXT * dx * 1e6
+    t_j = (np.asarray(t_eval_s, dtype=np.float64) if t_eval_s is not None
+           else np.linspace(0.0, t_max, max(int(n_snapshots), 24)))
+    n_j = np.clip(np.round(t_j / dt).astype(np.int64), 0, N - 1)
+    frame_steps = np.unique(n_j)
+    print(f'\n===== Time-domain stitch (Huygens/TF-SF): {label} =====')
+    print(f'Pulse {pulse_cycles:g} periods (s={sp * 1e15:.1f} fs), t_max={t_max * 1e15:.0f} fs = '
+          f'{N} steps, {n_win} windows of {window_w_um}um (overlap {O_cells * dx * 1e6:.2f}um), '
+          f'{len(n_j)} frames, engine {backend.engine_name()}')
+    if stack is not None:
+        print(f'Layers: {describe_stack(stack)}')
+    src_ix = NP + 2
+    iy0 = int(round((T_WG / 2 + vcsel_offset * 1e-6 - y0_m) / dx))
+    n_src = _n_source_2d(src_ix * dx * 1e6, iy0, y0_m, dx, wg_n, T_WG,
+                         n_aq_l if stack is None else (float(complex(stack[0]['n']).real) if stack
+                                                       else float(n_co_l)), input_gap_um)
+    _iy, vs, vs_ey = _vcsel_source_2d(Ny, dx, dt, T_WG, src_ix, vcsel_offset, vcsel_waist,
+                                      lambda_nm, vcsel_tilt, n_src, _zsrc, f0, y0_m=y0_m)
+    src = vs_ey if pol == 'p' else vs
+    env_t = np.exp(-((np.arange(N) * dt - tc) / sp) ** 2)
+    os.makedirs('results', exist_ok=True)
+    mm_path = os.path.join('results', f'_stream_beads_{_safe(label)}_td_{os.getpid()}.npy')
+    fr_mm = np.lib.format.open_memmap(mm_path, mode='w+', dtype=np.float32,
+                                      shape=(len(n_j), Nx_full, Ny))
+    fr_mm[:] = 0.0
+    Ch = np.float32(dt / (MU0 * dx))
+    Nyc = (Ny - 1) if pol == 'p' else Ny
+    e_comp, h_comp = ('Ey', 'Hz') if pol == 'p' else ('Ez', 'Hy')
+    iy_lo = int(round(TEAR_BUFFER / dx))
+    iy_hi = int(round((T_WG + TEAR_BUFFER) / dx))
+    ix_in = int(round(0.1 * Nx_full))
+    ix_out = int(round(0.9 * Nx_full))
+    ix_in = guard_in_plane(ix_in, max(src_ix, int(round(x_facet_um / (dx * 1e6)))), Nx_full)
+    mode = _engine_mode(engine_mode)
+    rec_in = None
+    P = dict(P_in=0.0, P_out=0.0, S_in=0.0, S_out=0.0, S_up=0.0, S_dn=0.0)
+    win_info = []
+    for w in range(n_win):
+        t1 = time.time()
+        g0 = gx0(w)
+        last = w == n_win - 1
+        Nxw = Nx_win + (0 if last else EXT)
+        x_start_um = g0 * dx * 1e6
+        mp = _material_parts(Nxw, Ny, dx, wg_n, bead_n, T_WG, bx_um, bead_d_um, x_start_um,
+                             place_bead, n_aq=n_aq_l, n_mucin=n_mu_l, n_cornea=n_co_l,
+                             t_aq_um=t_aq_um, t_mu_um=t_mucin_um, t_lip_um=t_lipid_um,
+                             n_lip=n_lip_l, input_gap_um=input_gap_um, length_um=len_mat,
+                             curved=curved, bead_y_um=bead_y_um, stack=stack, beads=beads)
+        eng = backend.make_engine(kind, (Nxw, Ny), mode)
+        packed, ch_tm = window_coeffs(pol, mp, dt, dx)
+        eng.build_coeffs(ch_tm if pol == 'p' else Ch, packed)
+        del packed, mp
+        # CPML on every edge of every window (fdtd2d_core.py:1726)
+        make_boundary('cpml', pol, Nxw, Ny, dx, dt, npml_x=(NP, NP), npml_y=NP,
+                      lam_m=LAM).attach(eng)
+        eng.finalize()
+        i_rec = (gx0(w + 1) - g0 + i0) if not last else None
+        if w == 0:
+            eng.set_source(e_comp, src.ix, src.amp, src.ph1, src.ph2)
+            n_start = 0
+            n_in_last = int(np.nonzero(env_t > 1e-7)[0][-1])
+        else:
+            n_start, n_in_last = signal_span(np.abs(rec_in[0]).max(axis=1), N)
+            # TF/SF (fdtd2d_core.py:1771, 1784 TE; 1779, 1790 TM): H left of i0 carries the
+            # scattered field only, E at i0 the total field
+            if pol == 'p':
+                tf_h = eng.add_tfsf('Hz', i0 - 1, +1, None, [(0, Ny - 1)], TD_ROWS)
+                tf_e = eng.add_tfsf('Ey', i0, +1, 'cay', [(0, Ny - 1)], TD_ROWS)
+            else:
+                tf_h = eng.add_tfsf('Hy', i0 - 1, -1, None, [(0, Ny)], TD_ROWS)
+                tf_e = eng.add_tfsf('Ez', i0, -1, 'ceH', [(1, Ny - 1)], TD_ROWS)
+        bits = 0
+        rec = None
+        if i_rec is not None:
+            rec = (np.zeros((N, Nyc), dtype=np.float32), np.zeros((N, Nyc), dtype=np.float32))
+            m_re = eng.add_rec(e_comp, i_rec, 0, TD_ROWS)
+            m_rh = eng.add_rec(h_comp, i_rec - 1, 1, TD_ROWS)
+            bits |= (1 << m_re) | (1 << m_rh)
+        a_g, b_g = (0, Nx_win) if w == 0 else (O_cells, Nx_win)
+        if last:
+            b_g = Nx_full - g0
+        sums = {}                                  # name -> monitor id (summed over steps)
+        if w == 0:
+            sums['S_up'] = eng.add_sx(src.plane_up)
+            sums['S_dn'] = eng.add_sx(src.plane_dn)
+        for ixg, which in ((ix_in, 'in'), (ix_out, 'out')):
+            il = ixg - g0
+            if a_g <= il < b_g:
+                sums['P_' + which] = eng.add_box(((il, il + 1), (iy_lo, iy_hi)),
+                                                 form=1 if pol == 'p' else 0)
+                if il >= 1:
+                    sums['S_' + which] = eng.add_sx(il)
+        for mid in sums.values():
+            bits |= 1 << mid
+        peak, n_stop, n = 0.0, N, n_start
+        while n < N:
+            e = min(td_next_event(n, n_in_last, frame_steps), N - 1, n + TD_ROWS - 1)
+            k = e - n + 1
+            tab = new_steps(k)
+            if w == 0:
+                env = env_t[n:e + 1]
+                fill_phasors(tab, omega, dt, np.arange(n, e + 1) * dt, np.where(env > 1e-9, env, 0.0),
+                             src_omega=src.omega)
+            tab['mask'] = bits
+            tab['row'] = np.arange(k)
+            if w > 0:
+                eng.load_tfsf(tf_h, rec_in[0][n:e + 1])
+                eng.load_tfsf(tf_e, rec_in[1][n:e + 1])
+            eng.run(tab)
+            if rec is not None:
+                rec[0][n:e + 1] = eng.read_monitor(m_re)[:k]
+                rec[1][n:e + 1] = eng.read_monitor(m_rh)[:k]
+            for jj in np.nonzero(n_j == e)[0]:
+                F = eng.read_field(e_comp)
+                F = _tm_node_field(F) if pol == 'p' else F
+                fr_mm[jj, g0 + a_g:g0 + b_g] = F[a_g:b_g]
+            if e > n_in_last and (e - n_in_last) % TD_CHECK == 0:
+                cur = eng.field_maxabs(e_comp)
+                peak = max(peak, cur)
+                if cur < 1e-4 * max(peak, 1e-30):
+                    n_stop = e
+                    break
+            if e % TD_CHECK == 0:
+                peak = max(peak, eng.field_maxabs(e_comp))
+            n = e + 1
+        for name, mid in sums.items():
+            P[name] += eng.read_monitor(mid)
+        eng.close()
+        tw = time.time() - t1
+        win_info.append(dict(window=w, cells=Nxw * Ny, steps=n_stop - n_start, seconds=tw))
+        print(f'[Window {w + 1}/{n_win}] x0={x_start_um:.1f}um  steps {n_start}..{n_stop} '
+              f'({n_stop - n_start} of {N})  {tw:.0f}s', flush=True)
+        rec_in = rec
+    fr_mm.flush()
+    del fr_mm
+    total_time = time.time() - t_total
+    Tr_I = P['P_out'] / max(P['P_in'], 1e-30)
+    S_in, S_out = P['S_in'], P['S_out']
+    Tr = S_out / S_in if abs(S_in) > 0 else Tr_I
+    tr_def = 'poynting' if abs(S_in) > 0 else 'intensity'
+    print(f'\n[Result] {label} [time-domain stitch]: total {total_time:.0f}s  Transmission={Tr:.4f}')
+    frames = [{'x_start': 0.0, 'x_end': L_um, 'y_start': y0_m * 1e6, 'y_end': (y0_m + H_total) * 1e6,
+               'slide_i': 0, 'step': int(k), 'time': float(k * dt)} for k in n_j]
+    res = _result_dict(label, P['P_in'], P['P_out'], Tr, frames, total_time, bead_d_um, wg_mat,
+                       bead_mat, wg_n, bead_n, wg_thickness_um, lambda_nm, vcsel_waist,
+                       vcsel_tilt, vcsel_offset, 'pulse', 'stitch-td', place_bead, n_aq_l,
+                       n_mu_l, n_co_l, frames_memmap=mm_path, t_aq_um=t_aq_um,
+                       t_mu_um=t_mucin_um, t_lip_um=t_lipid_um, bead_x_um=bx_um,
+                       x_wg_start_um=(input_gap_um or 0.0), x_wg_end_um=L_um, polarization=pol,
+                       stack=stack, n_lip=n_lip_l)
+    res['transmission_intensity'] = Tr_I
+    res['transmission_def'] = tr_def
+    _vcsel_result(res, (P['S_dn'] - P['S_up']) * dx, -P['S_up'] * dx, _vd, _air, x_facet_um,
+                  vcsel_waist, p_in=S_in * dx, energy=True)
+    res.update(view_primary='transient', cw_only=False, windows=win_info, n_windows=n_win,
+               cell_updates=sum(wi['cells'] * wi['steps'] for wi in win_info),
+               pulse=dict(cycles=pulse_cycles, s_fs=sp * 1e15, tmax_fs=t_max * 1e15,
+                          dispersion=False))
     return res
 
 

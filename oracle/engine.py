@@ -18,6 +18,8 @@ run on it unchanged:
   <|E|^2> accumulators                     fdtd2d_core.py:848-850, fdtd3d_core.py:438-474
   CW phasor DFTs                           fdtd2d_core.py:838-847, 1434-1441; fdtd3d_core.py:1150-1160, 1636-1648
   flux running sums                        common/flux.py:30-43
+  time-domain stitch: plane records, TF/SF corrections, instantaneous Poynting sums,
+  max |F|                                  fdtd2d_core.py:1767-1835, fdtd3d_core.py:1966-2004
 
 Modes: 'ref32' is the reference's float32 path (pinned against the golden master of
 tests/test_characterization.py); 'f64' runs every array in float64 (the FP64 oracle,
@@ -99,6 +101,7 @@ class OracleEngine:
         self.mons = []
         self.sponge = None
         self.mur = []
+        self.tfsf = []
 
     @property
     def wide_acc(self):
@@ -201,10 +204,63 @@ class OracleEngine:
         self.mons.append(m)
         return len(self.mons) - 1
 
-    def add_box(self, box):
+    def add_box(self, box, form=0):
+        """form 1 (2D TM): the node Ey^2 alone (fdtd2d_core.py:1811-1815)."""
+        if form and self.kind != 'tm2d':
+            raise RuntimeError('box form 1 is 2D TM only')
         comps = {'te2d': ('node',), 'tm2d': ('node',), 'yee3d': ('Ex', 'Ey', 'Ez')}[self.kind]
         rng = {c: resolve_box(comp_shape(self.kind, self.dims, c), box) for c in comps}
-        return self._add(dict(t='box', rng=rng, val=0.0))
+        return self._add(dict(t='box', rng=rng, val=0.0, form=int(form)))
+
+    # -- time-domain stitch (fdtd2d_core.py:1586-1863, fdtd3d_core.py:1785-2084)
+    def add_sx(self, il):
+        """Sum over the masked steps of the instantaneous flux in +x through plane il
+        (fdtd2d_core.py:489-497, fdtd3d_core.py:762-767)."""
+        if not 1 <= il <= self.dims[0] - 2:
+            raise RuntimeError('add_sx: plane index out of range')
+        return self._add(dict(t='sx', il=int(il), val=0.0))
+
+    def add_rec(self, comp, il, phase, rows):
+        """Plane il of comp into row step['row'] of a (rows,) + plane buffer; phase 0 at the
+        start of the step, 1 after the H update and its TF/SF corrections."""
+        shp = comp_shape(self.kind, self.dims, comp)
+        if not 0 <= il < shp[0] or phase not in (0, 1) or rows < 1:
+            raise RuntimeError('add_rec: bad plane, phase or rows')
+        return self._add(dict(t='rec', comp=comp, il=int(il), phase=int(phase),
+                              a=np.zeros((int(rows),) + shp[1:], dtype=self.fdt)))
+
+    def add_tfsf(self, comp, il, sign, coef, rng, rows):
+        """F[il][rng] = F[il][rng] +- c inc[row][rng]; c = Ch (coef None, H components)
+        or the coefficient array coef at the cell (E components)."""
+        shp = comp_shape(self.kind, self.dims, comp)
+        sl = tuple(slice(a, b) for a, b in rng)
+        self.tfsf.append(dict(comp=comp, il=int(il), sign=1 if sign > 0 else -1, coef=coef,
+                              sl=sl, a=np.zeros((int(rows),) + shp[1:], dtype=self.fdt)))
+        return len(self.tfsf) - 1
+
+    def load_tfsf(self, idx, rows):
+        rows = np.asarray(rows, dtype=np.float32)
+        self.tfsf[idx]['a'][:len(rows)] = rows
+
+    def field_maxabs(self, comp):
+        return float(np.max(np.abs(self.F[comp])))
+
+    def _record(self, mask, r, phase):
+        for i, m in enumerate(self.mons):
+            if m['t'] == 'rec' and m['phase'] == phase and (mask >> i) & 1:
+                m['a'][r] = self.F[m['comp']][m['il']]
+
+    def _apply_tfsf(self, r, h):
+        for t in self.tfsf:
+            if (t['comp'][0] == 'H') != h:
+                continue
+            F = self.F[t['comp']][t['il']]
+            inc = t['a'][r][t['sl']]
+            c = self.ch if t['coef'] is None else self.coef[t['coef']][t['il']][t['sl']]
+            if t['sign'] > 0:
+                F[t['sl']] += c * inc
+            else:
+                F[t['sl']] -= c * inc
 
     def add_sq(self, form, nx_acc=None):
         if self.kind == 'yee3d':
@@ -247,8 +303,10 @@ class OracleEngine:
 
     def zero_monitor(self, mid):
         m = self.mons[mid]
-        if m['t'] == 'box':
+        if m['t'] in ('box', 'sx'):
             m['val'] = 0.0
+        elif m['t'] == 'rec':
+            m['a'][...] = 0
         elif m['t'] == 'sq':
             m['acc'][...] = 0
         elif m['t'] == 'dft':
@@ -269,17 +327,17 @@ class OracleEngine:
 
     def read_monitor(self, mid):
         m = self.mons[mid]
-        if m['t'] == 'box':
+        if m['t'] in ('box', 'sx'):
             return float(m['val'])
         if m['t'] == 'sq':
             return m['acc'].copy()
-        if m['t'] == 'dft':
+        if m['t'] in ('dft', 'rec'):
             return m['a'].copy()
         return [(p[2].copy(), p[3].copy()) for p in m['pairs']]
 
     def write_monitor(self, mid, value):
         m = self.mons[mid]
-        if m['t'] == 'box':
+        if m['t'] in ('box', 'sx'):
             m['val'] = value
         elif m['t'] == 'sq':
             m['acc'][...] = value
@@ -305,9 +363,21 @@ class OracleEngine:
                         self._monitor(m, row)
 
     def _step(self, row):
+        td = bool(self.tfsf) or any(m['t'] == 'rec' for m in self.mons)
+        mask, r = int(row['mask']), int(row['row'])
+        if td:
+            self._record(mask, r, 0)
         if self.mur:
             self._mur_capture()
-        self._yee_step(row)
+        if td:
+            # fdtd2d_core.py:1768-1794, fdtd3d_core.py:1966-1990
+            self._h_half()
+            self._apply_tfsf(r, True)
+            self._record(mask, r, 1)
+            self._e_half(row)
+            self._apply_tfsf(r, False)
+        else:
+            self._yee_step(row)
         if self.sponge is not None:
             self._apply_sponge()
         for f in self.mur:
@@ -379,13 +449,23 @@ class OracleEngine:
                 F[:, :, -n:] *= g[None, None, ::-1]
 
     def _yee_step(self, row):
+        self._h_half()
+        self._e_half(row)
+
+    def _h_half(self):
         if self.kind == 'te2d':
             self._te_H()
+        elif self.kind == 'tm2d':
+            self._tm_H()
+        else:
+            self._yee_H()
+
+    def _e_half(self, row):
+        if self.kind == 'te2d':
             self._te_E()
             self._excite(row)
             self._pec2d()
         elif self.kind == 'tm2d':
-            self._tm_H()
             dax = self.coef.get('dax')
             if dax is not None:                        # fdtd2d_core.py:796-797
                 self.F['Ex'] *= dax
@@ -394,7 +474,6 @@ class OracleEngine:
             self._excite(row)
             self._pec2d()
         else:
-            self._yee_H()
             self._yee_E()                              # includes loss and PEC
             self._excite(row)
 
@@ -645,8 +724,27 @@ class OracleEngine:
                 m['val'] += tot
             else:
                 (a0, a1), (b0, b1) = m['rng']['node']
-                I = self._intensity2d()
-                m['val'] += np.sum(I[a0:a1, b0:b1], dtype=np.float64)
+                if m.get('form'):                      # fdtd2d_core.py:1814
+                    f = self._node_ey()[a0:a1, b0:b1]
+                    m['val'] += np.sum(f ** 2, dtype=np.float64)
+                else:
+                    I = self._intensity2d()
+                    m['val'] += np.sum(I[a0:a1, b0:b1], dtype=np.float64)
+        elif t == 'sx':                                # fdtd2d_core.py:489-497, fdtd3d_core.py:762-767
+            il, F = m['il'], self.F
+            if self.kind == 'te2d':
+                hy = 0.5 * (F['Hy'][il - 1] + F['Hy'][il])
+                m['val'] += -np.sum(F['Ez'][il] * hy, dtype=np.float64)
+            elif self.kind == 'tm2d':
+                hz = 0.5 * (F['Hz'][il - 1] + F['Hz'][il])
+                m['val'] += np.sum(F['Ey'][il] * hz, dtype=np.float64)
+            else:
+                hz = 0.5 * (F['Hz'][il - 1] + F['Hz'][il])
+                hy = 0.5 * (F['Hy'][il - 1] + F['Hy'][il])
+                m['val'] += (np.sum(F['Ey'][il] * hz, dtype=np.float64)
+                             - np.sum(F['Ez'][il] * hy, dtype=np.float64))
+        elif t == 'rec':
+            pass                                       # recorded inside the step
         elif t == 'sq':
             if self.kind == 'yee3d':                   # fdtd3d_core.py:465-474
                 nx = m['nx']

@@ -78,6 +78,11 @@ STACK_PULSE = [dict(name='Aqueous', t_um=1.0, n=complex(1.329, 0.001), kappa=0.0
                dict(name='Mucin', t_um=0.5, n=complex(1.34, 0.0), kappa=0.0, mat=None)]
 CASES_2D['te_stitch_pulse'] = dict(CASES_2D['te_stitch'], pulse_dispersion=False, **PULSE)
 CASES_2D['tm_stitch_pulse_stack'] = dict(CASES_2D['tm_stitch_lossy'], stack=STACK_PULSE, **PULSE)
+# the time-domain TF/SF stitch (fdtd2d_core.py:1586-1863): windows of 4 um every 2 um
+# (the TF/SF plane needs an overlap of npml + 6 cells), TE and lossy TM
+TD = dict(source_type='pulse', pulse_mode='td', pulse_cycles=10.0, slide_um=2.0)
+CASES_2D['te_stitch_td'] = dict(CASES_2D['te_stitch'], **TD)
+CASES_2D['tm_stitch_td_lossy'] = dict(CASES_2D['tm_stitch_lossy'], **TD)
 
 CASES_3D = {
     'te_full_char': dict(fn='run_3d', label='golden3d', wg_n=1.491, lx_um=4.0, air_um=1.0,
@@ -108,6 +113,22 @@ for _n in ('te_full_char', 'tm_full_lossy', 'te_stitch_char'):
 CASES_3D['te_stitch_char_pulse'] = dict(
     CASES_3D['te_stitch_char'], fn='run_3d_stitched_pulse', pulse_cycles=10.0, pulse_nfreq=3,
     disp_map={'wg_n': 'pmma', 'bead': 'polystyrol', 'n_aq': 'aqueous', 'n_co': 'cornea_layered'})
+# the 3D time-domain TF/SF stitch (fdtd3d_core.py:1785-2084): CPML on every face, and the
+# sponge on every face for the lossy TM case
+CASES_3D['te_stitch_char_td'] = dict(
+    CASES_3D['te_stitch_char'], fn='run_3d_stitched_td', pulse_cycles=10.0, window_w_um=3.0,
+    slide_um=1.2, n_sponge=10)
+CASES_3D['tm_stitch_td_sponge'] = dict(
+    CASES_3D['tm_full_lossy'], fn='run_3d_stitched_td', label='tm3dtd', pulse_cycles=10.0,
+    window_w_um=3.0, slide_um=1.0, n_sponge=8, boundary='sponge')
+for _k in ('steps_factor',):
+    CASES_3D['tm_stitch_td_sponge'].pop(_k, None)
+# windows of at least 54 cells on every axis let the TF/SF plane (i0 = n_sp + 2 = 11) lie
+# behind a 10-cell CPML: the CPML path of the 3D time-domain stitch
+CASES_3D['te_stitch_td_cpml'] = dict(
+    CASES_3D['te_stitch_char'], fn='run_3d_stitched_td', label='tdcpml', pulse_cycles=10.0,
+    lx_um=8.0, window_w_um=5.5, slide_um=1.5, tear_um=3.5, lz_um=5.4, n_sponge=None,
+    bead=dict(x_um=4.0, d_um=0.5, n=1.59))
 
 
 def run_2d(d2, name, case):

@@ -284,12 +284,13 @@ def bench_pulse(args):
     solve, set-up included."""
     from lens_sensor_amd.planar_beads import fdtd2d_core as d2
     out = {}
+    case = dict(PULSE_CASE, pulse_nfreq=args.pulse_lines)
     d2.KAPPA = {'aq': 0.001, 'bead': 0.002}
     try:
-        for mode in ('fourier', 'td'):
+        for mode in args.pulse_modes.split(','):
             print(f'[pulse] {mode}', file=sys.stderr, flush=True)
             t0 = time.time()
-            r = d2.run_beads(pulse_mode=mode, **PULSE_CASE)
+            r = d2.run_beads(pulse_mode=mode, **case)
             wall = time.time() - t0
             out[mode] = dict(wall_s=wall, cell_updates=int(r['cell_updates']),
                              mcups=r['cell_updates'] / wall / 1e6,
@@ -378,6 +379,9 @@ def main():
     ap.add_argument('--workload', default='tm2d_stitch',
                     choices=('tm2d_stitch', 'yee3d', 'sweep', 'serve', 'pulse2d'))
     ap.add_argument('--requests', type=int, default=24, help='serve: requests in the stream')
+    ap.add_argument('--pulse-lines', type=int, default=None,
+                    help='pulse2d: comb lines of the Fourier stitch (default: the whole band)')
+    ap.add_argument('--pulse-modes', default='fourier,td', help='pulse2d: fourier, td or both')
     ap.add_argument('--windows', type=int, default=None, help='tm2d_stitch: windows to run')
     ap.add_argument('--steps-override', type=int, default=None,
                     help='steps per window (reduced runs for profiling only)')
@@ -420,11 +424,14 @@ def main():
         r = bench_pulse(args)
         tot = sum(v['cell_updates'] for v in r.values())
         wall = sum(v['wall_s'] for v in r.values())
+        r.setdefault('fourier', None)
+        r.setdefault('td', None)
         out = dict(metric='Mcell-updates/s', value=tot / wall / 1e6, unit='Mcell-updates/s',
                    n_gpus=1, steps=None, warmup=0, ms_per_step=None, higher_is_better=True,
                    scaling='weak', vs_baseline=None, dtype='fp32', data='synthetic',
                    config=dict(workload='pulse2d_tm_40um_fourier_and_td', dx_nm=20.0,
                                window_um=30.0, slide_um=18.0, pulse_cycles=30.0,
+                               comb_lines=args.pulse_lines or 'band', modes=args.pulse_modes,
                                fields='fp32', phasors='fp64'),
                    device=dev, fourier=r['fourier'], td=r['td'])
         print(json.dumps(out), flush=True)

@@ -182,11 +182,13 @@ int lf_set_temporal_mon(lf_ctx *ctx, int depth);
 int lf_tb_stats(lf_ctx *ctx, double *ms, int *launches, int *depth);
 /* Launch order of the classes of a 2D blocked pass (no reference counterpart: an engine
  * tuning knob): class `cls` (0 slab-free, 1 y-slab rows, 2 x-slab planes, 3 excitation
- * planes) is dispatched first, the others behind it on side streams.  Results are
- * bit-identical for every order.  3D accepts only 0. */
+ * planes) is dispatched first, the others behind it on side streams; 4 launches every
+ * class in turn on the context stream without events (small windows, where queuing a
+ * pass costs the host more than the device needs for it).  Results are bit-identical for
+ * every order.  3D accepts only 0. */
 int lf_set_tb_lead(lf_ctx *ctx, int cls);
 /* Autotuner: times every candidate temporal-block depth and x chunk (2D: depth 2/4/6/8 x
- * chunk 32/64/128, then the lead class 0..3 at the fastest of those; 3D: depth 2/3 x
+ * chunk 32/64/128, then the lead class 0..4 at the fastest of those; 3D: depth 2/3 x
  * chunk 32/48/64/96) on this context with `reps` passes of monitor-free copies of `row`,
  * restores every field and psi slab, and keeps the fastest (results are bit-identical
  * for every choice).  No counterpart in the reference, whose step loop has no blocking;

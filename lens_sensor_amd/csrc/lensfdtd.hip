@@ -2136,6 +2136,12 @@ int fork_tb_groups(lf_ctx *c, bool interior, int ng, int lead, int solo, H has, 
         c->tb_chain = false;
         return 0;
     }
+    if (c->tb_lead == kTBSerial) {       // every group on the context stream, no events
+        for (int u = 0; u < ng; ++u)
+            if (has(u)) one(u, c->stream);
+        c->tb_chain = false;
+        return 0;
+    }
     if (int e = ensure_side_streams(c)) return e;
     if (lead < 0 || lead >= ng || !has(lead)) lead = 0;
     int side_of[kTBClasses];
@@ -2525,7 +2531,7 @@ extern "C" int lf_set_temporal(lf_ctx *c, int depth, int lx) {
 
 extern "C" int lf_set_tb_lead(lf_ctx *c, int cls) {
     if (int e = check_ctx(c)) return e;
-    if (cls < 0 || cls > 3) return fail(-1, "lf_set_tb_lead: class must be 0, 1, 2 or 3");
+    if (cls < 0 || cls > kTBSerial) return fail(-1, "lf_set_tb_lead: class must be 0, 1, 2, 3 or 4");
     if (c->kind == LF_YEE3D && cls != 0) return fail(-1, "lf_set_tb_lead: 3D passes take class 0 only");
     c->tb_lead = cls;
     return 0;
@@ -3756,21 +3762,23 @@ extern "C" int lf_autotune(lf_ctx *c, const lf_step *row, int reps, int *depth, 
         }
         if (rc) break;
     }
-    // 2D: the launch order of the pass classes at the chosen depth and chunk.  The orders
-    // differ by a few percent, about the noise of one short trial, so every order is timed
-    // twice, interleaved, over three times the passes, and keeps its faster time.
+    // 2D: the launch order of the pass classes at the chosen depth and chunk, and the
+    // serial order on one stream (no events: the choice where queuing a pass costs the
+    // host more than the device needs for it).  The orders differ by a few percent, about
+    // the noise of one short trial, so every order is timed twice, interleaved, over three
+    // times the passes, and keeps its faster time.
     if (rc == 0 && c->kind != LF_YEE3D && bestT > 1) {
-        double lt[kTBClasses];
-        for (int u = 0; u < kTBClasses; ++u) lt[u] = 1e30;
+        double lt[kTBSerial + 1];
+        for (int u = 0; u <= kTBSerial; ++u) lt[u] = 1e30;
         for (int round = 0; round < 2 && rc == 0; ++round)
-            for (int ld = 0; ld < kTBClasses; ++ld) {
+            for (int ld = 0; ld <= kTBSerial; ++ld) {
                 double per = 0;
                 if ((rc = trial(bestT, bestLx, ld, per, 3 * reps)) != 0) break;
                 lt[ld] = std::min(lt[ld], per);
             }
         if (rc == 0) {
             best = lt[0];
-            for (int ld = 1; ld < kTBClasses; ++ld)
+            for (int ld = 1; ld <= kTBSerial; ++ld)
                 if (lt[ld] < best) { best = lt[ld]; bestLead = ld; }
         }
     }

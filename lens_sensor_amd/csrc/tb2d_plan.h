@@ -50,6 +50,7 @@ struct TBClass {
     int wy0[4], nwy[4], end[4], sexc[4];
 };
 constexpr int kTBClasses = 4;
+constexpr int kTBSerial = 4;      // lf_set_tb_lead value: every class on the context stream
 struct TBPlan {
     TBClass cls[kTBClasses];
     bool interior;

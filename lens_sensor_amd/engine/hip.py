@@ -488,7 +488,8 @@ class HipEngine:
 
     def set_tb_lead(self, cls):
         """Pass class dispatched first in the 2D blocked passes (0 slab-free, 1 y-slab rows,
-        2 x-slab planes, 3 excitation planes); a speed knob, results are identical."""
+        2 x-slab planes, 3 excitation planes; 4 every class in turn on one stream); a speed
+        knob, results are identical."""
         _check(self._lib.lf_set_tb_lead(self.h, int(cls)), 'lf_set_tb_lead')
 
     def autotune(self, row, reps=None):

@@ -42,8 +42,9 @@ def test_autotune_keeps_the_state(kind, dims):
 
 @pytest.mark.parametrize('kind', ['tm2d', 'te2d'])
 def test_lead_class_is_bit_exact(kind):
-    """Every launch order of the pass classes gives the depth-1 result bit for bit (source
-    and stitch drive inside the grid, so all four classes have work)."""
+    """Every launch order of the pass classes, and the serial order on one stream (4),
+    gives the depth-1 result bit for bit (source and stitch drive inside the grid, so all
+    four classes have work)."""
     from lens_sensor_amd.engine.hip import HipEngine
     dims = (300, 200)
     ref = HipEngine(kind, dims, 'fp64acc')
@@ -51,7 +52,7 @@ def test_lead_class_is_bit_exact(kind):
     ref.set_temporal(1, 64)
     tab = _table(40, 0, 40)
     ref.run(tab)
-    for lead in range(4):
+    for lead in range(5):
         print(f'[lead class] {kind} lead={lead}', flush=True)
         e = HipEngine(kind, dims, 'fp64acc')
         _setup(e, kind, dims, np.random.default_rng(12), source=True, drive=True, pec=False)
@@ -66,7 +67,7 @@ def test_lead_class_is_bit_exact(kind):
 def test_lead_class_setter():
     from lens_sensor_amd.engine.hip import HipEngine
     e = HipEngine('tm2d', (20, 20), 'ref32')
-    for bad in (-1, 4):
+    for bad in (-1, 5):
         with pytest.raises(RuntimeError):
             e.set_tb_lead(bad)
     e.close()

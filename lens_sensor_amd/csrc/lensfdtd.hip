@@ -3401,7 +3401,15 @@ int launch_monitors(lf_ctx *c, const MonArgs &ma, const lf_step &t, const StepAr
         if (!((mask >> id) & 1u)) continue;
         Monitor &m = c->mon[id];
         if (m.type == 0) {
-            box_kernel<<<kBoxBlocks, 256, 0, c->stream>>>(ma, m.box);
+            // as many blocks as the box fills (at most kBoxBlocks): a fixed grid per box,
+            // so its sum keeps one order; the other partial slots stay 0
+            long long tot = 0;
+            for (int q = 0; q < m.box.ncomp; ++q)
+                tot = std::max(tot, (long long)(m.box.hi[q][0] - m.box.lo[q][0]) *
+                                        (m.box.hi[q][1] - m.box.lo[q][1]) *
+                                        (m.box.hi[q][2] - m.box.lo[q][2]));
+            const int nb = (int)std::min<long long>(kBoxBlocks, std::max<long long>(1, (tot + 255) / 256));
+            box_kernel<<<nb, 256, 0, c->stream>>>(ma, m.box);
         } else if (m.type == 1) {
             const dim3 mb = mon_block(m.n1);
             dim3 gq((m.n2 + mb.x - 1) / mb.x, (m.n1 + mb.y - 1) / mb.y, m.n0);
@@ -3415,7 +3423,9 @@ int launch_monitors(lf_ctx *c, const MonArgs &ma, const lf_step &t, const StepAr
             dft_kernel<<<gq, mb, 0, c->stream>>>(ma, d, t.pr, t.pi, fr, fi);
         } else if (m.type == 4) {
             SxArgs a = m.sx;
-            sx_kernel<<<kSxBlocks, 256, 0, c->stream>>>(ma, a);
+            const long long n = c->kind == LF_YEE3D ? (long long)c->ny * c->nz : c->ny;
+            const int nb = (int)std::min<long long>(kSxBlocks, (n + 255) / 256);
+            sx_kernel<<<nb, 256, 0, c->stream>>>(ma, a);
         } else if (m.type == 3) {
             const FluxArgs &f = m.flux;
             const int n1 = f.n1[0] > (f.npair > 1 ? f.n1[1] : 0) ? f.n1[0] : f.n1[1];

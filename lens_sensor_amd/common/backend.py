@@ -93,8 +93,8 @@ def tune(eng, row, nsteps, cache=None, key=None):
     shape and setup, e.g. the windows of a stitch run.  Returns (depth, lx, lead) or None."""
     if not AUTOTUNE or nsteps < AUTOTUNE_MIN_STEPS or not hasattr(eng, 'autotune'):
         return None
-    if getattr(eng, 'absorbers', False):      # sponge / Mur runs advance one step per launch
-        return None
+    if getattr(eng, 'absorbers', False) or getattr(eng, 'td', False):
+        return None                          # sponge / Mur / TF/SF runs advance one step per launch
     if cache is not None and key in cache:
         depth, lx, lead = cache[key]
         eng.set_temporal(depth, lx)

@@ -198,6 +198,7 @@ class HipEngine:
         self._lib = lib
         self.mon_kind = []
         self.absorbers = False          # sponge / Mur faces set: one step per launch
+        self.td = False                 # TF/SF corrections or plane records: one step per launch
 
     @property
     def wide_acc(self):
@@ -344,6 +345,7 @@ class HipEngine:
         shp = (int(rows),) + comp_shape(self.kind, self.dims, comp)[1:]
         rc = self._lib.lf_add_rec(self.h, ARR[comp], int(il), int(phase), int(rows),
                                   ctypes.byref(mid))
+        self.td = True
         return self._new_id(rc, mid, dict(t='rec', shape=shp))
 
     def add_tfsf(self, comp, il, sign, coef, rng, rows):
@@ -358,7 +360,7 @@ class HipEngine:
         _check(self._lib.lf_add_tfsf(self.h, ARR[comp], int(il), int(sign),
                                      -1 if coef is None else ARR[coef], j0, j1, k0, k1, int(rows),
                                      ctypes.byref(idx)), 'lf_add_tfsf')
-        self.absorbers = True
+        self.td = True
         return idx.value
 
     def load_tfsf(self, idx, rows):

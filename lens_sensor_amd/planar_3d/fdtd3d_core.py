@@ -714,11 +714,14 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
                     det_out_um=None, vol_dtype='float32', wg_width_um=None, n_clad_side=N_AIR,
                     vcsel_tilt_deg=0.0, vcsel_offset_y_um=0.0, vcsel_offset_z_um=0.0,
                     input_gap_um=0.0, polarization='s', boundary='cpml', cad=None,
-                    return_efull=False, stack=None, npml=None, engine_mode=None):
+                    return_efull=False, stack=None, npml=None, engine_mode=None,
+                    tune_cache=None):
     """3D x-domain decomposition (reference fdtd3d_core.py:1311-1782): windows run to CW
     steady state one after another; the previous window's complex E (all three
     components) is imposed with a cosine taper in the overlap; inner seams end in a
-    non-assembled CPML extension; the assembled Efull/Iavg skip overlap and extension."""
+    non-assembled CPML extension; the assembled Efull/Iavg skip overlap and extension.
+    ``tune_cache`` (a dict) carries the blocking chosen for each window shape across calls
+    (the pulse comb's lines share their windows)."""
     if cad is not None:
         raise NotImplementedError('CAD geometry import is outside the device path')
     t_total = time.time()
@@ -789,6 +792,7 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
     go = min(max(int(round(det_out_um / dx_um)), 1), Nx_full - 2)
     fluxes = FluxAccumulator()
     mode = _engine_mode(engine_mode)
+    tuned = {} if tune_cache is None else tune_cache
     for w in range(n_win):
         g0 = gx0(w)
         x0_um = g0 * dx_um
@@ -859,6 +863,9 @@ def run_3d_stitched(label, wg_n, window_w_um=20.0, slide_um=12.0, t_wg_um=5.0, t
         acc_n = int(win.sum())
         n_I = int(selI.sum())
         book.note_samples(list(fpl), acc_n)
+        # blocking tuned once per window shape and set-up (source window, driven windows)
+        backend.tune(eng, tab[steps_win // 2], steps_win, cache=tuned,
+                     key=(Nxw, Ny, Nz, w > 0 and Ehand is not None))
         chunk = max(1, steps_win // 10)
         for c0 in range(0, steps_win, chunk):
             eng.run(tab[c0:c0 + chunk])
@@ -1236,6 +1243,7 @@ def run_3d_stitched_pulse(label, wg_n, window_w_um=20.0, slide_um=12.0, pulse_cy
         print(f'Layers: {describe_stack(stack)}'
               + ('  (material layers follow n(lambda_k))' if pulse_dispersion else ''))
     disp = dict(disp_map or {}) if pulse_dispersion else {}
+    tuned = {}                                   # the lines share their window shapes
     fr = fr_path = None
     spec_T = []
     r_c = E_c = None
@@ -1257,7 +1265,7 @@ def run_3d_stitched_pulse(label, wg_n, window_w_um=20.0, slide_um=12.0, pulse_cy
         with contextlib.redirect_stdout(io.StringIO()):
             r = run_3d_stitched(f'{label}_f{k}', wg_k, window_w_um=window_w_um,
                                 slide_um=slide_um, lam_nm=float(lk), n_snapshots=1,
-                                return_efull=True, stack=stk, **args)
+                                return_efull=True, stack=stk, tune_cache=tuned, **args)
         E = r.pop('Efull')
         if fr is None:
             fr_path = os.path.join('results', f'_stream3d_{safe}_fourier_ez.npy')

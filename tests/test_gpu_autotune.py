@@ -30,7 +30,7 @@ def test_autotune_keeps_the_state(kind, dims):
     depth, lx, lead, ms = a.autotune(tab[12], reps=2)
     depths, lxs = CHOICES[kind]
     assert depth in depths and lx in lxs and ms > 0
-    assert lead in ((0,) if kind == 'yee3d' else (0, 1, 2, 3))
+    assert lead in ((0,) if kind == 'yee3d' else (0, 1, 2, 3, 4))
     b.set_temporal(depth, lx)
     b.set_tb_lead(lead)
     a.run(tab[12:])
@@ -76,3 +76,28 @@ def test_lead_class_setter():
         e3.set_tb_lead(2)
     e3.set_tb_lead(0)
     e3.close()
+
+
+def test_tuned_3d_stitch_windows_equal_untuned(in_tmp):
+    """run_3d_stitched tunes the blocking of each window shape (source window, driven
+    windows) once; the tuned run gives the untuned run's results bit for bit."""
+    from lens_sensor_amd.common import backend
+    from lens_sensor_amd.planar_3d import fdtd3d_core as f3d
+    case = dict(label='tune3d', wg_n=1.491, window_w_um=4.8, slide_um=2.4, lx_um=7.2,
+                air_um=1.0, tear_um=2.0, lz_um=2.0, dx_nm=50.0, t_wg_um=1.0, t_aq_um=1.0,
+                t_mu_um=0.5, n_snapshots=1, vcsel_waist_um=1.0, vcsel_waist_z_um=1.0,
+                bead=dict(x_um=3.6, d_um=0.5, n=1.59), engine_mode='fp64acc')
+    out = {}
+    old = backend.AUTOTUNE
+    try:
+        for on in (True, False):
+            backend.AUTOTUNE = on
+            print(f'[3d stitch] autotune={on}', flush=True)
+            out[on] = f3d.run_3d_stitched(**case)
+    finally:
+        backend.AUTOTUNE = old
+    a, b = out[True], out[False]
+    assert a['steps_total'] // a['n_windows'] >= backend.AUTOTUNE_MIN_STEPS
+    assert a['transmission'] == b['transmission']
+    np.testing.assert_array_equal(np.asarray(a['Iavg']), np.asarray(b['Iavg']))
+    np.testing.assert_array_equal(np.asarray(a['vols_array']), np.asarray(b['vols_array']))

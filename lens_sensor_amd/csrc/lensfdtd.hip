@@ -17,7 +17,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <set>
 #include <tuple>
 #include <type_traits>
@@ -1247,8 +1249,8 @@ __global__ void __launch_bounds__(256) maxabs_kernel(const float *__restrict__ f
 static inline dim3 mon_block(int n1) { return n1 == 1 ? dim3(256, 1, 1) : dim3(64, 4, 1); }
 
 // 2D coefficient array from its compact form: row j of column i takes the band value where
-// j0 <= j < j0 + rows, else the value of row j in table row xclass[i]
-__global__ void __launch_bounds__(256) coeff_assemble_kernel(float *dst, const unsigned char *xc,
+// j0 <= j < j0 + rows, else the value of row j in table row cls[i] (rows of P floats)
+__global__ void __launch_bounds__(256) coeff_assemble_kernel(float *dst, const int *cls,
                                                              const float *tab, const float *band,
                                                              int ny, int j0, int rows, int P) {
     const int j = blockIdx.x * 256 + threadIdx.x;
@@ -1256,7 +1258,7 @@ __global__ void __launch_bounds__(256) coeff_assemble_kernel(float *dst, const u
     if (j >= ny) return;
     const int r = j - j0;
     dst[(long long)i * P + j] = (r >= 0 && r < rows) ? band[(long long)i * rows + r]
-                                                      : tab[(long long)xc[i] * ny + j];
+                                                      : tab[(long long)cls[i] * P + j];
 }
 
 __global__ void scatter_kernel(float *dst, const float *src, int nx, int ny, int nz, int NY, int P) {
@@ -1472,6 +1474,26 @@ struct lf_ctx {
     std::vector<TfsfArgs> tfsf;
     int nrec = 0;
     float *mx_part = nullptr;            // per-block maxima of lf_field_maxabs
+    // compact forms of the 2D coefficient arrays built by lf_build_coeff, on the host:
+    // per-plane table row (planes from the array's extent on: the zero row ncls), table
+    // rows of P floats, band rows [j0, j0 + rows) of every plane
+    struct CoefForm {
+        std::vector<int> cls;
+        std::vector<float> tab, band;
+        int j0 = 0, rows = 0;
+    };
+    CoefForm cform[LF_NARRAYS];
+    // the forms of a context's arrays merged into one map on the device (CoefMap: one
+    // table row per plane and one band for all arrays), which the blocked passes read;
+    // on == false: they read the plain arrays through iota (plane p -> row p)
+    struct CoefMerged {
+        bool on = false;
+        int *cls = nullptr;
+        float *tab[LF_NARRAYS] = {}, *band[LF_NARRAYS] = {};
+        int j0 = 0, rows = 0;
+    };
+    CoefMerged cmerge;
+    int *iota = nullptr;
     bool finalized = false;
     // timing
     bool timing = false;
@@ -1514,6 +1536,102 @@ void comp_extent(const lf_ctx *c, int id, int &ex, int &ey, int &ez) {
 }
 
 size_t grid_elems(const lf_ctx *c) { return (size_t)c->g.nx * c->g.ny * c->g.P; }
+
+void drop_coef_merge(lf_ctx *c) {
+    auto &m = c->cmerge;
+    if (m.cls) (void)hipFree(m.cls);
+    for (auto p : m.tab) if (p) (void)hipFree(p);
+    for (auto p : m.band) if (p) (void)hipFree(p);
+    m = lf_ctx::CoefMerged{};
+}
+
+// Whether the blocked 2D passes read coefficients in compact form where one exists
+// (LENSFDTD_COMPACT_COEFF=0: always the plain arrays; the values are the same).
+bool compact_coeff_on() {
+    static const bool on = [] {
+        const char *v = std::getenv("LENSFDTD_COMPACT_COEFF");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+// Merges the compact forms of the context's present coefficient arrays into one map: a
+// plane's merged class is the tuple of its classes in every array, and every array's band
+// widens to the union of the bands, the added rows taken from its own table.  Element
+// (p, j) of every array keeps its value (that of the assembled array, zero outside the
+// array's extent).  A present array without a form (uploaded plainly) leaves the map off.
+int merge_coef_forms(lf_ctx *c) {
+    drop_coef_merge(c);
+    const int te[2] = {LF_CEE, LF_CEH}, tm[4] = {LF_CAX, LF_CAY, LF_DAX, LF_DAY};
+    const int *ids = c->kind == LF_TE2D ? te : tm;
+    const int n = c->kind == LF_TE2D ? 2 : 4;
+    std::vector<int> use;
+    int J0 = 1 << 30, J1 = 0;
+    for (int u = 0; u < n; ++u) {
+        if (!c->arr[ids[u]]) continue;
+        const auto &f = c->cform[ids[u]];
+        if (f.cls.empty()) return 0;
+        use.push_back(ids[u]);
+        if (f.rows > 0) { J0 = std::min(J0, f.j0); J1 = std::max(J1, f.j0 + f.rows); }
+    }
+    if (use.empty()) return 0;
+    if (J1 == 0) J0 = J1 = 0;
+    const int NX = c->g.nx, P = c->g.P, R = J1 - J0;
+    std::map<std::vector<int>, int> key;
+    std::vector<std::vector<int>> rows_of;            // merged class -> class per array
+    std::vector<int> cls(NX);
+    for (int i = 0; i < NX; ++i) {
+        std::vector<int> k;
+        for (int id : use) k.push_back(c->cform[id].cls[i]);
+        auto it = key.find(k);
+        if (it == key.end()) {
+            if (key.size() >= 4096) return 0;          // no layer structure to share
+            it = key.emplace(k, (int)rows_of.size()).first;
+            rows_of.push_back(k);
+        }
+        cls[i] = it->second;
+    }
+    const size_t nc = rows_of.size();
+    auto &m = c->cmerge;
+    LF_HIP(hipMalloc(&m.cls, NX * sizeof(int)));
+    LF_HIP(hipMemcpyAsync(m.cls, cls.data(), NX * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    std::vector<float> tab(nc * P), band((size_t)NX * R);
+    for (size_t u = 0; u < use.size(); ++u) {
+        const int id = use[u];
+        const auto &f = c->cform[id];
+        for (size_t k = 0; k < nc; ++k)
+            std::memcpy(&tab[k * P], &f.tab[(size_t)rows_of[k][u] * P], P * sizeof(float));
+        for (int i = 0; i < NX; ++i)
+            for (int r = 0; r < R; ++r) {
+                const int j = J0 + r, rb = j - f.j0;
+                band[(size_t)i * R + r] = (rb >= 0 && rb < f.rows) ? f.band[(size_t)i * f.rows + rb]
+                                                                   : f.tab[(size_t)f.cls[i] * P + j];
+            }
+        LF_HIP(hipMalloc(&m.tab[id], tab.size() * sizeof(float)));
+        LF_HIP(hipMemcpy(m.tab[id], tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (R) {
+            LF_HIP(hipMalloc(&m.band[id], band.size() * sizeof(float)));
+            LF_HIP(hipMemcpy(m.band[id], band.data(), band.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    LF_HIP(hipStreamSynchronize(c->stream));
+    m.j0 = J0; m.rows = R;
+    m.on = true;
+    c->tb_chain = false;
+    return 0;
+}
+
+// the blocked passes' view of coefficient array id (tab == nullptr when absent) and of
+// the map its context's arrays share
+Coef2 coef_src(const lf_ctx *c, int id) {
+    if (!c->arr[id]) return Coef2{nullptr, nullptr};
+    if (c->cmerge.on && compact_coeff_on()) return Coef2{c->cmerge.tab[id], c->cmerge.band[id]};
+    return Coef2{c->arr[id], nullptr};
+}
+CoefMap coef_map(const lf_ctx *c) {
+    if (c->cmerge.on && compact_coeff_on()) return CoefMap{c->cmerge.cls, c->cmerge.j0, c->cmerge.rows};
+    return CoefMap{c->iota, 0, 0};
+}
 
 int ensure_array(lf_ctx *c, int id) {
     if (c->arr[id]) return 0;
@@ -1673,6 +1791,14 @@ int lf_create(int kind, int mode, int nx, int ny, int nz, lf_ctx **out) {
         int r = ensure_array(c, cs[q]);
         if (r) { lf_destroy(c); return r; }
     }
+    if (kind != LF_YEE3D) {
+        std::vector<int> io(c->g.nx);
+        for (int i = 0; i < c->g.nx; ++i) io[i] = i;
+        e = hipMalloc(&c->iota, io.size() * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpy(c->iota, io.data(), io.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { lf_destroy(c); return fail(-2, hipGetErrorString(e)); }
+    }
     *out = c;
     return 0;
 }
@@ -1707,6 +1833,8 @@ int lf_destroy(lf_ctx *c) {
     if (c->nd_out) (void)hipFree(c->nd_out);
     for (auto &t : c->tfsf) if (t.series) (void)hipFree(t.series);
     if (c->mx_part) (void)hipFree(c->mx_part);
+    drop_coef_merge(c);
+    if (c->iota) (void)hipFree(c->iota);
     for (int m = 0; m < c->nmon; ++m) {
         if (c->mon[m].buf) (void)hipFree(c->mon[m].buf);
         if (c->mon[m].type == 3)
@@ -1736,6 +1864,10 @@ int lf_upload(lf_ctx *c, int id, const float *host, long long count) {
         return fail(-1, "lf_upload: element count " + std::to_string(count) + " != expected " +
                             std::to_string((long long)ex * ey * ez));
     if (int r = ensure_array(c, id)) return r;
+    if (id >= LF_CEE && c->kind != LF_YEE3D) {      // the plain array is the data now
+        c->cform[id] = lf_ctx::CoefForm{};
+        if (int r = merge_coef_forms(c)) return r;
+    }
     float *tmp = nullptr;
     LF_HIP(hipMalloc(&tmp, (size_t)count * sizeof(float)));
     LF_HIP(hipMemcpyAsync(tmp, host, (size_t)count * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -1760,22 +1892,39 @@ int lf_build_coeff(lf_ctx *c, int id, const unsigned char *xclass, int ncls, con
     for (int i = 0; i < ex; ++i)
         if (xclass[i] >= ncls) return fail(-1, "lf_build_coeff: class index out of range");
     if (int r = ensure_array(c, id)) return r;
-    const size_t nt = (size_t)ncls * ez, nb = (size_t)ex * rows;
-    unsigned char *dxc = nullptr;
+    // the compact form, kept on the host (merge_coef_forms), equals the assembled array
+    // over the whole allocation: planes from ex on (the staggered arrays' padding plane)
+    // take the zero table row ncls and zero band rows, table rows are zero from ez to P,
+    // as the zero-filled array is
+    const int NX = c->g.nx, P = c->g.P;
+    lf_ctx::CoefForm f{};
+    f.cls.assign(NX, ncls);
+    for (int i = 0; i < ex; ++i) f.cls[i] = xclass[i];
+    f.tab.assign((size_t)(ncls + 1) * P, 0.f);
+    for (int k = 0; k < ncls; ++k)
+        std::memcpy(&f.tab[(size_t)k * P], table + (size_t)k * ez, (size_t)ez * sizeof(float));
+    f.band.assign((size_t)NX * rows, 0.f);
+    if (rows) std::memcpy(f.band.data(), band, (size_t)ex * rows * sizeof(float));
+    f.j0 = j0; f.rows = rows;
+    int *dcls = nullptr;
     float *dtab = nullptr, *dband = nullptr;
-    LF_HIP(hipMalloc(&dxc, (size_t)ex));
-    LF_HIP(hipMalloc(&dtab, nt * sizeof(float)));
-    if (nb) LF_HIP(hipMalloc(&dband, nb * sizeof(float)));
-    LF_HIP(hipMemcpyAsync(dxc, xclass, (size_t)ex, hipMemcpyHostToDevice, c->stream));
-    LF_HIP(hipMemcpyAsync(dtab, table, nt * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    if (nb) LF_HIP(hipMemcpyAsync(dband, band, nb * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    LF_HIP(hipMalloc(&dcls, f.cls.size() * sizeof(int)));
+    LF_HIP(hipMalloc(&dtab, f.tab.size() * sizeof(float)));
+    if (rows) LF_HIP(hipMalloc(&dband, f.band.size() * sizeof(float)));
+    LF_HIP(hipMemcpyAsync(dcls, f.cls.data(), f.cls.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    LF_HIP(hipMemcpyAsync(dtab, f.tab.data(), f.tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (rows)
+        LF_HIP(hipMemcpyAsync(dband, f.band.data(), f.band.size() * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
     coeff_assemble_kernel<<<dim3((ez + 255) / 256, ex, 1), 256, 0, c->stream>>>(
-        c->arr[id], dxc, dtab, dband, ez, j0, rows, c->g.P);
+        c->arr[id], dcls, dtab, dband, ez, j0, rows, P);
     LF_HIP(hipGetLastError());
     LF_HIP(hipStreamSynchronize(c->stream));
-    LF_HIP(hipFree(dxc));
+    LF_HIP(hipFree(dcls));
     LF_HIP(hipFree(dtab));
     if (dband) LF_HIP(hipFree(dband));
+    c->cform[id] = std::move(f);
+    if (int r = merge_coef_forms(c)) return r;
     c->tb_chain = false;
     return 0;
 }
@@ -2248,6 +2397,9 @@ TBTMArgs tm_tb_args(lf_ctx *c) {
     f.Ex0 = c->arr[LF_EX]; f.Ey0 = c->arr[LF_EY]; f.Hz0 = c->arr[LF_HZ];
     f.Ex1 = c->alt[LF_EX]; f.Ey1 = c->alt[LF_EY]; f.Hz1 = c->alt[LF_HZ];
     f.cax = c->arr[LF_CAX]; f.cay = c->arr[LF_CAY]; f.dax = c->arr[LF_DAX]; f.day = c->arr[LF_DAY];
+    f.kax = coef_src(c, LF_CAX); f.kay = coef_src(c, LF_CAY);
+    f.kdx = coef_src(c, LF_DAX); f.kdy = coef_src(c, LF_DAY);
+    f.cm = coef_map(c);
     f.ch = c->ch;
     f.xe = prof_of(c, 0, false); f.xh = prof_of(c, 0, true);
     f.ye = prof_of(c, 1, false); f.yh = prof_of(c, 1, true);
@@ -2263,6 +2415,8 @@ TBTEArgs te_tb_args(lf_ctx *c) {
     f.Ez0 = c->arr[LF_EZ]; f.Hx0 = c->arr[LF_HX]; f.Hy0 = c->arr[LF_HY];
     f.Ez1 = c->alt[LF_EZ]; f.Hx1 = c->alt[LF_HX]; f.Hy1 = c->alt[LF_HY];
     f.ceE = c->arr[LF_CEE]; f.ceH = c->arr[LF_CEH];
+    f.kE = coef_src(c, LF_CEE); f.kH = coef_src(c, LF_CEH);
+    f.cm = coef_map(c);
     f.ch = c->ch;
     f.xe = prof_of(c, 0, false); f.xh = prof_of(c, 0, true);
     f.ye = prof_of(c, 1, false); f.yh = prof_of(c, 1, true);

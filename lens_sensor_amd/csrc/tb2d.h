@@ -32,11 +32,47 @@ __device__ __forceinline__ float div_kappa(float d, float k) {
 // tm_h_kernel + tm_e_kernel (boundaries.py:243-258, fdtd2d_core.py:796).
 constexpr int kMaxTB = 8;
 
+// Per-cell coefficient arrays of a 2D grid, as the blocked passes read them.  The arrays
+// of one context share a map (CoefMap): value (p, j) of an array is band[p * rows + j - j0]
+// inside the band of rows [j0, j0 + rows), else element j of table row cls[p] (rows of P
+// floats).  A window's planes of one layer column share their table row, so those loads
+// hit the L2 and only the band (the y range of the beads) streams from HBM.  Plain arrays
+// are the form cls[p] = p, tab = the array, no band.
+struct Coef2 {
+    const float *tab;
+    const float *band;
+};
+struct CoefMap {
+    const int *cls;
+    int j0, rows;
+};
+
+// where element (p, j) of a context's coefficient arrays sits: in the band (boff) or in
+// table row q (toff); j and the band test are fixed per lane, q is read a plane ahead
+struct CoefAt {
+    bool inb;
+    long long boff, toff;
+};
+__device__ __forceinline__ CoefAt coef_pos(const CoefMap &m, int p, int q, int j, int P) {
+    const int r = j - m.j0;
+    return CoefAt{(unsigned)r < (unsigned)m.rows, (long long)p * m.rows + r, (long long)q * P + j};
+}
+__device__ __forceinline__ float coef_at(Coef2 k, const CoefAt &w) {
+    return *(w.inb ? k.band + w.boff : k.tab + w.toff);
+}
+
+// table row of plane p (clamped to the grid), as a wave-uniform value
+__device__ __forceinline__ int coef_row(const CoefMap &m, int p, int Nx) {
+    return __builtin_amdgcn_readfirstlane(m.cls[p < Nx ? p : Nx - 1]);
+}
+
 struct TBTMArgs {
     Grid g;
     const float *Ex0, *Ey0, *Hz0;
     float *Ex1, *Ey1, *Hz1;
-    const float *cax, *cay, *dax, *day;
+    const float *cax, *cay, *dax, *day;   // plain arrays (dax / day: nullptr when lossless)
+    Coef2 kax, kay, kdx, kdy;             // the same arrays as the slab-free bodies read them
+    CoefMap cm;
     float ch;
     Prof xe, xh, ye, yh;
     Strip s0[4], s1[4];       // Hzx Hzy Exy Eyx: read / write buffers
@@ -60,7 +96,7 @@ struct TMPlane {              // one plane of one column, as loaded from the ste
 // lane tests); bit 2: planes may lie in an x slab or on an x edge (psi_Hzx, psi_Eyx,
 // plane tests).  MODE 0 is the slab-free interior: no psi registers, no edge tests.
 template <int MODE>
-__device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j, bool jin, bool jey,
+__device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int q, int j, bool jin, bool jey,
                                                  int sHzy, int sExy, bool lossy_x, bool lossy_y) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
     TMPlane v{};
@@ -72,11 +108,12 @@ __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j
         // straight into the iteration that uses it, with no copy that would wait for it.
         const long long c = (long long)(p < Nx ? p : Nx - 1) * P + j;
         v.ex = a.Ex0[c];
-        v.ca = a.cax[c];
-        v.da = (lossy_x ? a.dax : a.cax)[c];          // read only when lossy
+        const CoefAt w = coef_pos(a.cm, p < Nx ? p : Nx - 1, q, j, P);
+        v.ca = coef_at(a.kax, w);
+        v.da = coef_at(lossy_x ? a.kdx : a.kax, w);          // read only when lossy
         v.ey = a.Ey0[c];
-        v.cy = a.cay[c];
-        v.dy = (lossy_y ? a.day : a.cay)[c];
+        v.cy = coef_at(a.kay, w);
+        v.dy = coef_at(lossy_y ? a.kdy : a.kay, w);
         v.hz = a.Hz0[c];
         return v;
     }
@@ -87,6 +124,8 @@ __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int j
     const int jc = j < 0 ? 0 : (j < P ? j : P - 1);
     const long long c = (long long)pc * P + jc;
     v.ex = a.Ex0[c];
+    // (the slab classes read the plain arrays: their bodies hold more registers, and the
+    // compact form's address arithmetic cost them more than its bytes save)
     v.ca = a.cax[c];
     v.da = (lossy_x ? a.dax : a.cax)[c];
     v.ey = a.Ey0[c];
@@ -458,9 +497,13 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     // (monitored passes carry their accumulators in registers as well: they keep one
     // plane in flight, in one buffer)
     constexpr int PFD = MON == 0 ? 2 : 1;
-    TMPlane bufA = tm_load_plane<MODE>(a, ps, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    // the coefficient table row of the next plane to load is read one load ahead: its
+    // scalar wait would otherwise stand in front of the plane's vector loads
+    const int q0 = MODE == 0 ? coef_row(a.cm, ps, Nx) : 0, q1 = MODE == 0 ? coef_row(a.cm, ps + 1, Nx) : 0;
+    TMPlane bufA = tm_load_plane<MODE>(a, ps, q0, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
     TMPlane bufB{};
-    if constexpr (PFD == 2) bufB = tm_load_plane<MODE>(a, ps + 1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    if constexpr (PFD == 2) bufB = tm_load_plane<MODE>(a, ps + 1, q1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    int qn = MODE == 0 ? coef_row(a.cm, ps + PFD, Nx) : 0;
     // iteration p takes plane p from the buffer loaded two iterations earlier and
     // reloads it with plane p + 2; the loop runs two iterations per trip so that the
     // two buffers alternate without register copies (a copy of a load in flight
@@ -482,7 +525,8 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = tm_load_plane<MODE>(a, p + PFD, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+        nxt = tm_load_plane<MODE>(a, p + PFD, qn, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+        if constexpr (MODE == 0) qn = coef_row(a.cm, p + PFD + 1, Nx);
         // ---- levels
 #pragma unroll
         for (int s = 0; s < T; ++s) {
@@ -684,7 +728,9 @@ struct TBTEArgs {
     Grid g;
     const float *Ez0, *Hx0, *Hy0;
     float *Ez1, *Hx1, *Hy1;
-    const float *ceE, *ceH;
+    const float *ceE, *ceH;               // plain arrays
+    Coef2 kE, kH;                         // as the slab-free bodies read them
+    CoefMap cm;
     float ch;
     Prof xe, xh, ye, yh;
     Strip s0[4], s1[4];       // Hxy Hyx Ezx Ezy: read / write buffers
@@ -701,16 +747,17 @@ struct TEPlane {
 };
 
 template <int MODE>
-__device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int j, bool jin, bool jhx,
+__device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int q, int j, bool jin, bool jhx,
                                                  int sHxy, int sEzy) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
     TEPlane v{};
     const int Nx = a.g.nx, P = a.g.P;
     if constexpr (MODE == 0) {                         // unconditional, as for TM
         const long long c = (long long)(p < Nx ? p : Nx - 1) * P + j;
+        const CoefAt w = coef_pos(a.cm, p < Nx ? p : Nx - 1, q, j, P);
         v.ez = a.Ez0[c];
-        v.ce = a.ceE[c];
-        v.cH = a.ceH[c];
+        v.ce = coef_at(a.kE, w);
+        v.cH = coef_at(a.kH, w);
         v.hy = a.Hy0[c];
         v.hx = a.Hx0[c];
         return v;
@@ -720,7 +767,7 @@ __device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int j
     const int jc = j < 0 ? 0 : (j < P ? j : P - 1);
     const long long c = (long long)pc * P + jc;
     v.ez = a.Ez0[c];
-    v.ce = a.ceE[c];
+    v.ce = a.ceE[c];                                  // plain arrays, as for TM
     v.cH = a.ceH[c];
     v.hy = a.Hy0[c];
     v.hx = a.Hx0[c];
@@ -845,9 +892,11 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         mr.load(m, ps, j, own, X0, X1);
     }
     constexpr int PFD = MON == 0 ? 2 : 1;                                   // as for TM
-    TEPlane bufA = te_load_plane<MODE>(a, ps, j, jin, jhx, sHxy, sEzy);
+    const int q0 = MODE == 0 ? coef_row(a.cm, ps, Nx) : 0, q1 = MODE == 0 ? coef_row(a.cm, ps + 1, Nx) : 0;
+    TEPlane bufA = te_load_plane<MODE>(a, ps, q0, j, jin, jhx, sHxy, sEzy);
     TEPlane bufB{};
-    if constexpr (PFD == 2) bufB = te_load_plane<MODE>(a, ps + 1, j, jin, jhx, sHxy, sEzy);
+    if constexpr (PFD == 2) bufB = te_load_plane<MODE>(a, ps + 1, q1, j, jin, jhx, sHxy, sEzy);
+    int qn = MODE == 0 ? coef_row(a.cm, ps + PFD, Nx) : 0;                 // read one load ahead, as for TM
     // iteration p takes plane p from the buffer loaded two iterations earlier and
     // reloads it with plane p + 2; the loop runs two iterations per trip so that the
     // two buffers alternate without register copies (a copy of a load in flight
@@ -865,7 +914,8 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = te_load_plane<MODE>(a, p + PFD, j, jin, jhx, sHxy, sEzy);
+        nxt = te_load_plane<MODE>(a, p + PFD, qn, j, jin, jhx, sHxy, sEzy);
+        if constexpr (MODE == 0) qn = coef_row(a.cm, p + PFD + 1, Nx);
 #pragma unroll
         for (int s = 0; s < T; ++s) {
             const int q = p - 1 - s;

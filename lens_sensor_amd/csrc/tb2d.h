@@ -95,7 +95,7 @@ struct TMPlane {              // one plane of one column, as loaded from the ste
 // MODE bit 1: wavefront lanes may lie in a y slab or on a y edge (psi_Hzy, psi_Exy,
 // lane tests); bit 2: planes may lie in an x slab or on an x edge (psi_Hzx, psi_Eyx,
 // plane tests).  MODE 0 is the slab-free interior: no psi registers, no edge tests.
-template <int MODE>
+template <int MODE, bool CMP = false>
 __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int q, int j, bool jin, bool jey,
                                                  int sHzy, int sExy, bool lossy_x, bool lossy_y) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
@@ -108,12 +108,19 @@ __device__ __forceinline__ TMPlane tm_load_plane(const TBTMArgs &a, int p, int q
         // straight into the iteration that uses it, with no copy that would wait for it.
         const long long c = (long long)(p < Nx ? p : Nx - 1) * P + j;
         v.ex = a.Ex0[c];
-        const CoefAt w = coef_pos(a.cm, p < Nx ? p : Nx - 1, q, j, P);
-        v.ca = coef_at(a.kax, w);
-        v.da = coef_at(lossy_x ? a.kdx : a.kax, w);          // read only when lossy
+        if constexpr (CMP) {
+            const CoefAt w = coef_pos(a.cm, p < Nx ? p : Nx - 1, q, j, P);
+            v.ca = coef_at(a.kax, w);
+            v.da = coef_at(lossy_x ? a.kdx : a.kax, w);      // read only when lossy
+            v.cy = coef_at(a.kay, w);
+            v.dy = coef_at(lossy_y ? a.kdy : a.kay, w);
+        } else {
+            v.ca = a.cax[c];
+            v.da = (lossy_x ? a.dax : a.cax)[c];
+            v.cy = a.cay[c];
+            v.dy = (lossy_y ? a.day : a.cay)[c];
+        }
         v.ey = a.Ey0[c];
-        v.cy = coef_at(a.kay, w);
-        v.dy = coef_at(lossy_y ? a.kdy : a.kay, w);
         v.hz = a.Hz0[c];
         return v;
     }
@@ -499,11 +506,15 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
     constexpr int PFD = MON == 0 ? 2 : 1;
     // the coefficient table row of the next plane to load is read one load ahead: its
     // scalar wait would otherwise stand in front of the plane's vector loads
-    const int q0 = MODE == 0 ? coef_row(a.cm, ps, Nx) : 0, q1 = MODE == 0 ? coef_row(a.cm, ps + 1, Nx) : 0;
-    TMPlane bufA = tm_load_plane<MODE>(a, ps, q0, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    // (only the slab-free class without excitation or monitors reads the compact form:
+    // with it the drive, source and monitored bodies measured slower, their cost being
+    // instructions and registers rather than bytes)
+    constexpr bool CMP = MODE == 0 && EXC == 0 && MON == 0;
+    const int q0 = CMP ? coef_row(a.cm, ps, Nx) : 0, q1 = CMP ? coef_row(a.cm, ps + 1, Nx) : 0;
+    TMPlane bufA = tm_load_plane<MODE, CMP>(a, ps, q0, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
     TMPlane bufB{};
-    if constexpr (PFD == 2) bufB = tm_load_plane<MODE>(a, ps + 1, q1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
-    int qn = MODE == 0 ? coef_row(a.cm, ps + PFD, Nx) : 0;
+    if constexpr (PFD == 2) bufB = tm_load_plane<MODE, CMP>(a, ps + 1, q1, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+    int qn = CMP ? coef_row(a.cm, ps + PFD, Nx) : 0;
     // iteration p takes plane p from the buffer loaded two iterations earlier and
     // reloads it with plane p + 2; the loop runs two iterations per trip so that the
     // two buffers alternate without register copies (a copy of a load in flight
@@ -525,8 +536,8 @@ __device__ __forceinline__ void tm_tb_body(const TBTMArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = tm_load_plane<MODE>(a, p + PFD, qn, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
-        if constexpr (MODE == 0) qn = coef_row(a.cm, p + PFD + 1, Nx);
+        nxt = tm_load_plane<MODE, CMP>(a, p + PFD, qn, j, jin, jey, sHzy, sExy, lossy_x, lossy_y);
+        if constexpr (CMP) qn = coef_row(a.cm, p + PFD + 1, Nx);
         // ---- levels
 #pragma unroll
         for (int s = 0; s < T; ++s) {
@@ -746,7 +757,7 @@ struct TEPlane {
     float ez, hx, hy, ce, cH, pxy, pyx, pzx, pzy;
 };
 
-template <int MODE>
+template <int MODE, bool CMP = false>
 __device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int q, int j, bool jin, bool jhx,
                                                  int sHxy, int sEzy) {
     constexpr bool PX = MODE & 2, PY = MODE & 1;
@@ -754,10 +765,15 @@ __device__ __forceinline__ TEPlane te_load_plane(const TBTEArgs &a, int p, int q
     const int Nx = a.g.nx, P = a.g.P;
     if constexpr (MODE == 0) {                         // unconditional, as for TM
         const long long c = (long long)(p < Nx ? p : Nx - 1) * P + j;
-        const CoefAt w = coef_pos(a.cm, p < Nx ? p : Nx - 1, q, j, P);
         v.ez = a.Ez0[c];
-        v.ce = coef_at(a.kE, w);
-        v.cH = coef_at(a.kH, w);
+        if constexpr (CMP) {
+            const CoefAt w = coef_pos(a.cm, p < Nx ? p : Nx - 1, q, j, P);
+            v.ce = coef_at(a.kE, w);
+            v.cH = coef_at(a.kH, w);
+        } else {
+            v.ce = a.ceE[c];
+            v.cH = a.ceH[c];
+        }
         v.hy = a.Hy0[c];
         v.hx = a.Hx0[c];
         return v;
@@ -892,11 +908,12 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
         mr.load(m, ps, j, own, X0, X1);
     }
     constexpr int PFD = MON == 0 ? 2 : 1;                                   // as for TM
-    const int q0 = MODE == 0 ? coef_row(a.cm, ps, Nx) : 0, q1 = MODE == 0 ? coef_row(a.cm, ps + 1, Nx) : 0;
-    TEPlane bufA = te_load_plane<MODE>(a, ps, q0, j, jin, jhx, sHxy, sEzy);
+    constexpr bool CMP = MODE == 0 && EXC == 0 && MON == 0;                 // as for TM
+    const int q0 = CMP ? coef_row(a.cm, ps, Nx) : 0, q1 = CMP ? coef_row(a.cm, ps + 1, Nx) : 0;
+    TEPlane bufA = te_load_plane<MODE, CMP>(a, ps, q0, j, jin, jhx, sHxy, sEzy);
     TEPlane bufB{};
-    if constexpr (PFD == 2) bufB = te_load_plane<MODE>(a, ps + 1, q1, j, jin, jhx, sHxy, sEzy);
-    int qn = MODE == 0 ? coef_row(a.cm, ps + PFD, Nx) : 0;                 // read one load ahead, as for TM
+    if constexpr (PFD == 2) bufB = te_load_plane<MODE, CMP>(a, ps + 1, q1, j, jin, jhx, sHxy, sEzy);
+    int qn = CMP ? coef_row(a.cm, ps + PFD, Nx) : 0;                        // read one load ahead
     // iteration p takes plane p from the buffer loaded two iterations earlier and
     // reloads it with plane p + 2; the loop runs two iterations per trip so that the
     // two buffers alternate without register copies (a copy of a load in flight
@@ -914,8 +931,8 @@ __device__ __forceinline__ void te_tb_body(const TBTEArgs &a, const Excite &x, c
             mr.take();
             mr.load(m, p + 1, j, own, X0, X1);
         }
-        nxt = te_load_plane<MODE>(a, p + PFD, qn, j, jin, jhx, sHxy, sEzy);
-        if constexpr (MODE == 0) qn = coef_row(a.cm, p + PFD + 1, Nx);
+        nxt = te_load_plane<MODE, CMP>(a, p + PFD, qn, j, jin, jhx, sHxy, sEzy);
+        if constexpr (CMP) qn = coef_row(a.cm, p + PFD + 1, Nx);
 #pragma unroll
         for (int s = 0; s < T; ++s) {
             const int q = p - 1 - s;

@@ -49,10 +49,11 @@ BYTES_YEE3D = 6 * 8 + 3 * 4           # 6 fields r/w + ce_x, ce_y, ce_z = 60 B
 # (fetch x2.0, write x1.0); profiles/*_pmc.md hold the runs.  Keyed by (workload, depth,
 # x chunk): a pass of another blocking setting has no entry and reports traffic None.
 TRAFFIC_BYTES = {
-    # mean over the ten windows: the source window's pass (408.84 MB, four launches) and
-    # nine driven windows' passes (457.47 MB, three launches); the slab-free classes read
-    # the coefficients' compact form (layer-column rows from L2, the bead band from HBM)
-    ('tm2d_stitch', 8, 128): 452_609_650,
+    # mean over the ten windows: the source window's pass (409.36 MB, four launches) and
+    # nine driven windows' passes (526.20 MB, three launches); the slab-free class without
+    # excitation reads the coefficients' compact form (layer-column rows from L2, the
+    # bead band from HBM)
+    ('tm2d_stitch', 8, 128): 514_517_984,
     # yee_tb_kernel<3, false>: FETCH 1,479,486.5 KiB x 2048 + WRITE 1,043,593.9 KiB x 1024
     ('yee3d', 3, 96): 3_029_988_357 + 1_068_640_115,
 }
